@@ -1,0 +1,626 @@
+// C ABI of libdvg (include/dvg.h): context, device-resident scenes, and the
+// synchronous scene / packed entry points built on them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dvg.h"
+#include "dvg_kernels.h"
+#include "dvg_pack.h"
+
+#define DVG_VERSION_STRING "dvg-mi355x 0.1 (gfx950)"
+
+struct dvg_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    const char *last_error = "";
+    std::string err_buf;
+    // work buffers, grown on demand
+    float *weight = nullptr;
+    size_t weight_cap = 0;
+    float *accum = nullptr;
+    size_t accum_cap = 0;
+};
+
+struct dvg_device_scene {
+    dvgk::DevScene S;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    uint32_t num_groups = 0, num_shapes = 0;
+    uint32_t grad_len[14] = {};  // lengths of the 14 gradient arrays
+};
+
+namespace {
+
+int set_err(dvg_ctx *ctx, int code, const char *msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int hip_err(dvg_ctx *ctx, hipError_t e, const char *where) {
+    if (e == hipSuccess) return DVG_OK;
+    if (ctx) {
+        ctx->err_buf = std::string(where) + ": " + hipGetErrorString(e);
+        ctx->last_error = ctx->err_buf.c_str();
+    }
+    return DVG_DEVICE_ERROR;
+}
+
+#define HIP_TRY(ctx, expr)                                  \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return hip_err(ctx, _e, #expr); \
+    } while (0)
+
+// Bump allocator over one device allocation.
+struct Arena {
+    std::vector<std::pair<const void *, size_t>> srcs;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    size_t add(const void *src, size_t bytes) {
+        size_t o = (total + 255) & ~(size_t)255;
+        srcs.push_back({src, bytes});
+        offs.push_back(o);
+        total = o + std::max(bytes, (size_t)4);
+        return offs.size() - 1;
+    }
+};
+
+int ensure(dvg_ctx *ctx, float *&buf, size_t &cap, size_t n) {
+    if (cap >= n) return DVG_OK;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr;
+    cap = 0;
+    HIP_TRY(ctx, hipMalloc(&buf, std::max(n, (size_t)1) * sizeof(float)));
+    cap = n;
+    return DVG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dvg_version(void) { return DVG_VERSION_STRING; }
+
+dvg_ctx *dvg_ctx_create(int device_ordinal) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) return nullptr;
+    if (hipSetDevice(device_ordinal) != hipSuccess) return nullptr;
+    dvg_ctx *ctx = new dvg_ctx();
+    ctx->device = device_ordinal;
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return nullptr;
+    }
+    ctx->stream = ctx->own_stream;
+    return ctx;
+}
+
+void dvg_ctx_destroy(dvg_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->weight) (void)hipFree(ctx->weight);
+    if (ctx->accum) (void)hipFree(ctx->accum);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+const char *dvg_last_error(const dvg_ctx *ctx) { return ctx ? ctx->last_error : "no context"; }
+
+int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *s) {
+    if (!ctx) return DVG_DEVICE_ERROR;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return DVG_OK;
+}
+
+int32_t dvg_sync(dvg_ctx *ctx) {
+    if (!ctx) return DVG_DEVICE_ERROR;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return DVG_OK;
+}
+
+int32_t dvg_prepare(const dvg_scene_desc *scene, const dvg_render_options *options, dvg_prepared **out,
+                    const char **error_message) {
+    const char *err = "";
+    dvg_prepared *p = new dvg_prepared();
+    int st = dvgh::prepare(scene, options, *p, &err);
+    if (error_message) *error_message = err;
+    if (st != DVG_OK) {
+        delete p;
+        *out = nullptr;
+        return st;
+    }
+    *out = p;
+    return DVG_OK;
+}
+
+void dvg_prepared_free(dvg_prepared *p) { delete p; }
+
+void dvg_prepared_pack(const dvg_prepared *p, dvg_scene_pack *pack, dvg_backward_pack *bpack,
+                       dvg_render_params *params) {
+    dvgh::pack_views(*p, pack, bpack, params);
+}
+
+int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backward_pack *bp,
+                                const dvg_render_params *prm, dvg_device_scene **out) {
+    if (!ctx || !pk || !prm || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint32_t G = pk->num_groups, SP = pk->num_prepared_shapes;
+    const uint32_t W = prm->width, H = prm->height;
+    uint32_t nsx = std::max(prm->samples_x, 1u), nsy = std::max(prm->samples_y, 1u);
+    if ((uint64_t)W * H * nsx * nsy > 0xffffffffull)
+        return set_err(ctx, DVG_INVALID_SCENE, "too many samples for 1d launch");
+    uint32_t tiles_x = (W + dvgk::TILE - 1) / dvgk::TILE, tiles_y = (H + dvgk::TILE - 1) / dvgk::TILE;
+    uint32_t ntiles = tiles_x * tiles_y;
+
+    // Tile range of every group (binning.rs:30-110): its group-local bounds
+    // mapped to canvas by shape_to_canvas, floored to 16-px tiles, clamped.
+    std::vector<int32_t> ranges((size_t)std::max(G, 1u) * 4, -1);
+    uint64_t n_entries = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        const float *b = pk->group_bounds + 4 * (size_t)g;
+        int32_t *r = &ranges[4 * (size_t)g];
+        r[0] = r[1] = 1;
+        r[2] = r[3] = 0;  // empty
+        if (!(b[0] <= b[2] && b[1] <= b[3]) || tiles_x == 0 || tiles_y == 0) continue;
+        const float *m = pk->group_shape_xform + 6 * (size_t)g;
+        float xs[4] = {m[0] * b[0] + m[1] * b[1] + m[2], m[0] * b[2] + m[1] * b[1] + m[2],
+                       m[0] * b[2] + m[1] * b[3] + m[2], m[0] * b[0] + m[1] * b[3] + m[2]};
+        float ys[4] = {m[3] * b[0] + m[4] * b[1] + m[5], m[3] * b[2] + m[4] * b[1] + m[5],
+                       m[3] * b[2] + m[4] * b[3] + m[5], m[3] * b[0] + m[4] * b[3] + m[5]};
+        float mnx = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
+        float mxx = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
+        float mny = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
+        float mxy = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
+        if (mxx < 0.f || mxy < 0.f || mnx >= (float)W || mny >= (float)H) continue;
+        auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+        int tx0 = clampi((int)std::floor(mnx / (float)dvgk::TILE), (int)tiles_x - 1);
+        int tx1 = clampi((int)std::floor(mxx / (float)dvgk::TILE), (int)tiles_x - 1);
+        int ty0 = clampi((int)std::floor(mny / (float)dvgk::TILE), (int)tiles_y - 1);
+        int ty1 = clampi((int)std::floor(mxy / (float)dvgk::TILE), (int)tiles_y - 1);
+        if (tx0 > tx1 || ty0 > ty1) continue;
+        r[0] = tx0; r[1] = ty0; r[2] = tx1; r[3] = ty1;
+        n_entries += (uint64_t)(tx1 - tx0 + 1) * (uint64_t)(ty1 - ty0 + 1);
+    }
+
+    // scene_to_prepared: first instance of each scene shape
+    uint32_t S = bp ? bp->num_shapes : 0;
+    std::vector<uint32_t> s2p(std::max(S, 1u), 0);
+    if (bp && bp->prepared_to_scene)
+        for (uint32_t i = SP; i-- > 0;) s2p[bp->prepared_to_scene[i]] = i;
+
+    Arena A;
+    auto addv = [&](const void *p, size_t bytes) { return A.add(p, p ? bytes : 0); };
+    size_t i_shape = addv(pk->shape_data, SP * 15 * 4), i_curve = addv(pk->curve_data, pk->curve_data_len * 4),
+           i_sb = addv(pk->shape_bounds, SP * 16), i_gd = addv(pk->group_data, (size_t)G * 64),
+           i_gx = addv(pk->group_xform, (size_t)G * 24), i_gsx = addv(pk->group_shape_xform, (size_t)G * 24),
+           i_gis = addv(pk->group_inv_scale, (size_t)G * 4), i_gs = addv(pk->group_shapes, pk->group_shapes_len * 4),
+           i_sx = addv(pk->shape_xform, SP * 24), i_grd = addv(pk->gradient_data, pk->gradient_data_len * 4),
+           i_so = addv(pk->stop_offsets, pk->stop_offsets_len * 4),
+           i_sc = addv(pk->stop_colors, pk->stop_offsets_len * 16), i_rng = addv(ranges.data(), ranges.size() * 4),
+           i_toff = addv(nullptr, 0), i_tent = addv(nullptr, 0), i_tcnt = addv(nullptr, 0);
+    // tile work buffers (no host source)
+    A.srcs[i_toff].second = ((size_t)ntiles + 1) * 4;
+    A.srcs[i_tent].second = std::max<uint64_t>(n_entries, 1) * 4;
+    A.srcs[i_tcnt].second = (size_t)std::max(ntiles, 1u) * 4;
+    // recompute offsets after resizing the sourceless entries
+    {
+        size_t t = 0;
+        for (size_t k = 0; k < A.offs.size(); ++k) {
+            size_t o = (t + 255) & ~(size_t)255;
+            A.offs[k] = o;
+            t = o + std::max(A.srcs[k].second, (size_t)4);
+        }
+        A.total = t;
+    }
+    const float *bg_src = prm->background_image;
+    size_t i_bg = addv(bg_src, bg_src ? (size_t)W * H * 16 : 0);
+    size_t i_p2s = 0, i_st = 0, i_pp = 0, i_pc = 0, i_pt = 0, i_spo = 0, i_spc = 0, i_sco = 0, i_scc = 0, i_sto = 0,
+           i_stc = 0, i_scl = 0, i_sl = 0, i_cdf = 0, i_pmf = 0, i_sid = 0, i_gid = 0, i_pcdf = 0, i_ppmf = 0,
+           i_ppid = 0, i_pco = 0, i_pcc = 0, i_ppo = 0, i_s2p = 0;
+    uint32_t n_cdf = 0;
+    if (bp) {
+        i_p2s = addv(bp->prepared_to_scene, SP * 4);
+        i_st = addv(bp->shape_transform, (size_t)S * 24);
+        i_pp = addv(bp->path_points, bp->path_points_len * 4);
+        i_pc = addv(bp->path_num_control_points, bp->path_num_control_points_len * 4);
+        i_pt = addv(bp->path_thickness, bp->path_thickness_len * 4);
+        i_spo = addv(bp->shape_path_offsets, (size_t)S * 4);
+        i_spc = addv(bp->shape_path_point_counts, (size_t)S * 4);
+        i_sco = addv(bp->shape_path_ctrl_offsets, (size_t)S * 4);
+        i_scc = addv(bp->shape_path_ctrl_counts, (size_t)S * 4);
+        i_sto = addv(bp->shape_path_thickness_offsets, (size_t)S * 4);
+        i_stc = addv(bp->shape_path_thickness_counts, (size_t)S * 4);
+        i_scl = addv(bp->shape_path_is_closed, (size_t)S * 4);
+        n_cdf = (uint32_t)bp->shape_cdf_len;
+        if (n_cdf > 0 && n_cdf != SP)
+            return set_err(ctx, DVG_INVALID_SCENE, "boundary tables do not match the prepared shapes");
+        i_sl = addv(bp->shape_lengths, n_cdf ? (size_t)S * 4 : 0);
+        i_cdf = addv(bp->shape_cdf, (size_t)n_cdf * 4);
+        i_pmf = addv(bp->shape_pmf, (size_t)n_cdf * 4);
+        i_sid = addv(bp->shape_ids, (size_t)n_cdf * 4);
+        i_gid = addv(bp->group_ids, (size_t)n_cdf * 4);
+        i_pcdf = addv(bp->path_cdf, bp->path_cdf_len * 4);
+        i_ppmf = addv(bp->path_pmf, bp->path_cdf_len * 4);
+        i_ppid = addv(bp->path_point_ids, bp->path_point_ids_len * 4);
+        i_pco = addv(bp->path_cdf_offsets, n_cdf ? (size_t)S * 4 : 0);
+        i_pcc = addv(bp->path_cdf_counts, n_cdf ? (size_t)S * 4 : 0);
+        i_ppo = addv(bp->path_point_offsets, n_cdf ? (size_t)S * 4 : 0);
+        i_s2p = addv(s2p.data(), (size_t)std::max(S, 1u) * 4);
+    }
+
+    dvg_device_scene *ds = new dvg_device_scene();
+    hipError_t e = hipMalloc(&ds->arena, A.total);
+    if (e != hipSuccess) {
+        delete ds;
+        return hip_err(ctx, e, "hipMalloc(scene arena)");
+    }
+    ds->arena_bytes = A.total;
+    // stage everything in one pinned-free host buffer, then one copy
+    std::vector<uint8_t> host(A.total, 0);
+    for (size_t k = 0; k < A.srcs.size(); ++k)
+        if (A.srcs[k].first && A.srcs[k].second)
+            std::memcpy(host.data() + A.offs[k], A.srcs[k].first, A.srcs[k].second);
+    e = hipMemcpyAsync(ds->arena, host.data(), A.total, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(ds->arena);
+        delete ds;
+        return hip_err(ctx, e, "scene upload");
+    }
+    uint8_t *base = (uint8_t *)ds->arena;
+    auto at = [&](size_t i) { return (void *)(base + A.offs[i]); };
+    dvgk::DevScene &D = ds->S;
+    std::memset(&D, 0, sizeof(D));
+    D.W = W; D.H = H; D.nsx = nsx; D.nsy = nsy; D.spp = nsx * nsy; D.seed = prm->seed;
+    D.jitter = prm->use_prefiltering ? 0u : prm->jitter;
+    D.prefilter = prm->use_prefiltering;
+    D.filter_type = prm->filter_type;
+    D.filter_radius = prm->filter_radius;
+    D.ri = (int)std::fmax(std::ceil(prm->filter_radius), 0.0f);
+    for (int c = 0; c < 4; ++c) D.bg[c] = prm->background[c];
+    D.bg_image = bg_src ? (const float *)at(i_bg) : nullptr;
+    D.num_groups = G;
+    D.num_prepared = SP;
+    D.shape_data = (const float *)at(i_shape);
+    D.curve_data = (const float *)at(i_curve);
+    D.shape_bounds = (const float *)at(i_sb);
+    D.group_data = (const float *)at(i_gd);
+    D.group_xform = (const float *)at(i_gx);
+    D.group_shape_xform = (const float *)at(i_gsx);
+    D.group_inv_scale = (const float *)at(i_gis);
+    D.group_shapes = (const float *)at(i_gs);
+    D.shape_xform = (const float *)at(i_sx);
+    D.gradient_data = (const float *)at(i_grd);
+    D.stop_offsets = (const float *)at(i_so);
+    D.stop_colors = (const float *)at(i_sc);
+    D.tiles_x = tiles_x;
+    D.tiles_y = tiles_y;
+    D.tile_offsets = (const uint32_t *)at(i_toff);
+    D.tile_entries = (const uint32_t *)at(i_tent);
+    if (bp) {
+        D.num_shapes = S;
+        D.prepared_to_scene = (const uint32_t *)at(i_p2s);
+        D.shape_transform = (const float *)at(i_st);
+        D.path_points = (const float *)at(i_pp);
+        D.path_ctrl = (const uint32_t *)at(i_pc);
+        D.path_thickness = (const float *)at(i_pt);
+        D.sp_off = (const uint32_t *)at(i_spo);
+        D.sp_cnt = (const uint32_t *)at(i_spc);
+        D.sc_off = (const uint32_t *)at(i_sco);
+        D.sc_cnt = (const uint32_t *)at(i_scc);
+        D.st_off = (const uint32_t *)at(i_sto);
+        D.st_cnt = (const uint32_t *)at(i_stc);
+        D.s_closed = (const uint32_t *)at(i_scl);
+        D.n_cdf = n_cdf;
+        D.shape_lengths = (const float *)at(i_sl);
+        D.shape_cdf = (const float *)at(i_cdf);
+        D.shape_pmf = (const float *)at(i_pmf);
+        D.shape_ids = (const uint32_t *)at(i_sid);
+        D.group_ids = (const uint32_t *)at(i_gid);
+        D.path_cdf = (const float *)at(i_pcdf);
+        D.path_pmf = (const float *)at(i_ppmf);
+        D.path_point_ids = (const uint32_t *)at(i_ppid);
+        D.path_cdf_off = (const uint32_t *)at(i_pco);
+        D.path_cdf_cnt = (const uint32_t *)at(i_pcc);
+        D.path_point_off = (const uint32_t *)at(i_ppo);
+        D.scene_to_prepared = (const uint32_t *)at(i_s2p);
+    }
+    ds->num_groups = G;
+    ds->num_shapes = S;
+    {
+        size_t npix = (size_t)W * H;
+        uint64_t lens[14] = {8ull * S,
+                             bp ? bp->path_points_len : 0,
+                             bp ? bp->path_thickness_len : 0,
+                             S,
+                             9ull * S,
+                             9ull * G,
+                             16ull * G,
+                             pk->gradient_data_len,
+                             pk->stop_offsets_len,
+                             4 * pk->stop_offsets_len,
+                             1,
+                             4,
+                             bg_src ? npix * 4 : 0,
+                             npix * 2};
+        for (int k = 0; k < 14; ++k) ds->grad_len[k] = (uint32_t)lens[k];
+    }
+    // tile lists: count, scan, write (sorted by group id by construction)
+    if (ntiles > 0) {
+        dvgk::launch_bin_count((const float *)at(i_rng), G, tiles_x, tiles_y, (uint32_t *)at(i_tcnt), ctx->stream);
+        dvgk::launch_scan((const uint32_t *)at(i_tcnt), ntiles, (uint32_t *)at(i_toff), ctx->stream);
+        dvgk::launch_bin_write((const float *)at(i_rng), G, tiles_x, tiles_y, (const uint32_t *)at(i_toff),
+                               (uint32_t *)at(i_tent), ctx->stream);
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            (void)hipFree(ds->arena);
+            delete ds;
+            return hip_err(ctx, e, "tile binning launch");
+        }
+    }
+    *out = ds;
+    return DVG_OK;
+}
+
+void dvg_device_scene_destroy(dvg_device_scene *ds) {
+    if (!ds) return;
+    if (ds->arena) (void)hipFree(ds->arena);
+    delete ds;
+}
+
+int32_t dvg_device_render(dvg_ctx *ctx, dvg_device_scene *ds, float *out) {
+    if (!ctx || !ds || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    const dvgk::DevScene &S = ds->S;
+    size_t npix = (size_t)S.W * S.H;
+    if (npix == 0) return DVG_OK;
+    int st = ensure(ctx, ctx->weight, ctx->weight_cap, npix);
+    if (st) return st;
+    st = ensure(ctx, ctx->accum, ctx->accum_cap, npix * 4);
+    if (st) return st;
+    {
+        float *p = ctx->accum;
+        uint32_t n = (uint32_t)(npix * 4);
+        dvgk::launch_zero(&p, &n, 1, ctx->stream);
+    }
+    dvgk::launch_weights(S, ctx->weight, false, ctx->stream);
+    dvgk::launch_forward(S, ctx->weight, ctx->accum, ctx->stream);
+    dvgk::launch_resolve(S, ctx->weight, ctx->accum, out, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
+static void grads_to_dev(const dvg_grads *g, dvgk::DevGrads &G) {
+    G.d_shape_params = g->d_shape_params;
+    G.d_shape_points = g->d_shape_points;
+    G.d_shape_thickness = g->d_shape_thickness;
+    G.d_shape_stroke_width = g->d_shape_stroke_width;
+    G.d_shape_transform = g->d_shape_transform;
+    G.d_group_transform = g->d_group_transform;
+    G.d_group_data = g->d_group_data;
+    G.d_gradient_data = g->d_gradient_data;
+    G.d_stop_offsets = g->d_stop_offsets;
+    G.d_stop_colors = g->d_stop_colors;
+    G.d_filter_radius = g->d_filter_radius;
+    G.d_background = g->d_background;
+    G.d_background_image = g->d_background_image;
+    G.d_translation = g->d_translation;
+}
+
+int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const float *d_render,
+                                   const dvg_grads *grads) {
+    if (!ctx || !ds || !d_render || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    const dvgk::DevScene &S = ds->S;
+    if (S.prefilter) return set_err(ctx, DVG_INVALID_SCENE, "prefiltered backward is not available in this build");
+    size_t npix = (size_t)S.W * S.H;
+    if (npix == 0 || S.num_groups == 0) return DVG_OK;
+    int st = ensure(ctx, ctx->weight, ctx->weight_cap, npix);
+    if (st) return st;
+    dvgk::DevGrads G;
+    grads_to_dev(grads, G);
+    // the outputs are accumulations into zeroed arrays
+    dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
+    dvgk::launch_weights(S, ctx->weight, true, ctx->stream);
+    dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
+    if (S.n_cdf > 0) dvgk::launch_boundary(S, ctx->weight, d_render, G, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
+}  // extern "C"
+
+// ---- synchronous host-pointer entry points ---------------------------------
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct GradLayout {
+    size_t off[14], len[14], total = 0;
+};
+
+// Lengths of the 14 gradient arrays (backward_gpu.rs:187-200).
+GradLayout grad_layout(const dvg_scene_pack *pk, const dvg_backward_pack *bp, const dvg_render_params *prm) {
+    GradLayout L;
+    size_t S = bp ? bp->num_shapes : 0, G = pk->num_groups;
+    size_t npix = (size_t)prm->width * prm->height;
+    size_t lens[14] = {8 * S,
+                       bp ? bp->path_points_len : 0,
+                       bp ? bp->path_thickness_len : 0,
+                       S,
+                       9 * S,
+                       9 * G,
+                       16 * G,
+                       pk->gradient_data_len,
+                       pk->stop_offsets_len,
+                       4 * pk->stop_offsets_len,
+                       1,
+                       4,
+                       prm->background_image ? npix * 4 : 0,
+                       prm->compute_translation ? npix * 2 : 0};
+    for (int k = 0; k < 14; ++k) {
+        L.off[k] = L.total;
+        L.len[k] = lens[k];
+        L.total += (lens[k] + 63) & ~(size_t)63;
+    }
+    return L;
+}
+
+float **grad_slot(const dvg_grads *g, int k) {
+    float *const *p = &g->d_shape_params;
+    return const_cast<float **>(p + k);
+}
+
+int render_backward_host(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backward_pack *bp,
+                         const dvg_render_params *prm, const float *d_render, const dvg_grads *grads) {
+    size_t npix = (size_t)prm->width * prm->height;
+    GradLayout L = grad_layout(pk, bp, prm);
+    // zero the caller's arrays first (outputs are accumulated into zeroed buffers)
+    for (int k = 0; k < 14; ++k) {
+        float *dst = *grad_slot(grads, k);
+        if (dst && L.len[k]) std::memset(dst, 0, L.len[k] * sizeof(float));
+    }
+    if (npix == 0 || pk->num_groups == 0) return DVG_OK;
+    if (prm->use_prefiltering)
+        return set_err(ctx, DVG_INVALID_SCENE, "prefiltered backward is not available in this build");
+    dvg_device_scene *ds = nullptr;
+    int st = dvg_device_scene_create(ctx, pk, bp, prm, &ds);
+    if (st) return st;
+    DevBuf gbuf, dr;
+    hipError_t e = hipMalloc(&gbuf.p, std::max(L.total, (size_t)1) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dr.p, npix * 16);
+    if (e == hipSuccess) e = hipMemsetAsync(gbuf.p, 0, std::max(L.total, (size_t)1) * sizeof(float), ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr.p, d_render, npix * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        dvg_device_scene_destroy(ds);
+        return hip_err(ctx, e, "backward buffers");
+    }
+    dvg_grads dg;
+    float *gb = (float *)gbuf.p;
+    for (int k = 0; k < 14; ++k) *grad_slot(&dg, k) = L.len[k] ? gb + L.off[k] : nullptr;
+    if (!(*grad_slot(grads, 12))) dg.d_background_image = nullptr;
+    if (!(*grad_slot(grads, 13))) dg.d_translation = nullptr;
+    st = dvg_device_render_backward(ctx, ds, (const float *)dr.p, &dg);
+    if (st == DVG_OK) {
+        std::vector<float> host(L.total);
+        e = hipMemcpyAsync(host.data(), gbuf.p, L.total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hip_err(ctx, e, "gradient readback");
+        else
+            for (int k = 0; k < 14; ++k) {
+                float *dst = *grad_slot(grads, k);
+                if (dst && L.len[k]) std::memcpy(dst, host.data() + L.off[k], L.len[k] * sizeof(float));
+            }
+    }
+    dvg_device_scene_destroy(ds);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dvg_render_packed(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_render_params *prm, float *out) {
+    if (!ctx || !pk || !prm || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    size_t npix = (size_t)prm->width * prm->height;
+    if (npix == 0) return DVG_OK;
+    dvg_device_scene *ds = nullptr;
+    int st = dvg_device_scene_create(ctx, pk, nullptr, prm, &ds);
+    if (st) return st;
+    DevBuf ob;
+    hipError_t e = hipMalloc(&ob.p, npix * 16);
+    if (e != hipSuccess) {
+        dvg_device_scene_destroy(ds);
+        return hip_err(ctx, e, "hipMalloc(output)");
+    }
+    st = dvg_device_render(ctx, ds, (float *)ob.p);
+    if (st == DVG_OK) {
+        e = hipMemcpyAsync(out, ob.p, npix * 16, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hip_err(ctx, e, "image readback");
+    }
+    dvg_device_scene_destroy(ds);
+    return st;
+}
+
+int32_t dvg_render_backward_packed(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backward_pack *bp,
+                                   const dvg_render_params *prm, const float *d_render, const dvg_grads *grads) {
+    if (!ctx || !pk || !bp || !prm || !d_render || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return render_backward_host(ctx, pk, bp, prm, d_render, grads);
+}
+
+int32_t dvg_render(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options, float *out) {
+    if (!ctx || !scene || !options || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    // Renderer::render early-outs (renderer.rs:40-66)
+    size_t npix = (size_t)scene->width * scene->height;
+    if (scene->background_image && scene->background_image_len != npix * 4)
+        return set_err(ctx, DVG_INVALID_SCENE, "background image size mismatch");
+    if (npix == 0) return DVG_OK;
+    if (scene->num_groups == 0) {
+        for (size_t i = 0; i < npix; ++i)
+            for (int c = 0; c < 4; ++c)
+                out[i * 4 + c] = scene->background_image ? scene->background_image[i * 4 + c] : scene->background[c];
+        return DVG_OK;
+    }
+    dvg_prepared P;
+    const char *err = "";
+    int st = dvgh::prepare(scene, options, P, &err);
+    if (st) return set_err(ctx, st, err);
+    dvg_scene_pack pk;
+    dvg_render_params prm;
+    dvgh::pack_views(P, &pk, nullptr, &prm);
+    return dvg_render_packed(ctx, &pk, &prm, out);
+}
+
+int32_t dvg_render_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                            const dvg_backward_options *bopt, const float *d_render, const dvg_grads *grads) {
+    if (!ctx || !scene || !options || !d_render || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    size_t npix = (size_t)scene->width * scene->height;
+    if (scene->background_image && scene->background_image_len != npix * 4)
+        return set_err(ctx, DVG_INVALID_SCENE, "background image size mismatch");
+    dvg_prepared P;
+    const char *err = "";
+    int st = dvgh::prepare(scene, options, P, &err);
+    if (st) return set_err(ctx, st, err);
+    dvg_scene_pack pk;
+    dvg_backward_pack bp;
+    dvg_render_params prm;
+    dvgh::pack_views(P, &pk, &bp, &prm);
+    prm.compute_translation = (bopt && bopt->compute_translation && grads->d_translation) ? 1u : 0u;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    st = render_backward_host(ctx, &pk, &bp, &prm, d_render, grads);
+    if (st) return st;
+    // finalize_background_gradients (background.rs:29-61)
+    if (scene->background_image) {
+        if (grads->d_background_image)
+            for (size_t i = 0; i < npix; ++i) {
+                float *c = grads->d_background_image + i * 4;
+                const float *im = scene->background_image + i * 4;
+                float dr = c[0] * im[3], dg = c[1] * im[3], db = c[2] * im[3];
+                float da = c[0] * im[0] + c[1] * im[1] + c[2] * im[2] + c[3];
+                c[0] = dr; c[1] = dg; c[2] = db; c[3] = da;
+            }
+    } else if (grads->d_background) {
+        float *d = grads->d_background;
+        const float *b = scene->background;
+        float nr = d[0] * b[3], ng = d[1] * b[3], nb = d[2] * b[3], na = d[0] * b[0] + d[1] * b[1] + d[2] * b[2] + d[3];
+        d[0] = nr; d[1] = ng; d[2] = nb; d[3] = na;
+    }
+    return DVG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,477 @@
+// Device-side geometry, paint, filter and RNG helpers for the gfx950 kernels.
+//
+// The arithmetic follows the reference kernels operation by operation (the
+// file:line of each restated function is given) so that images agree with the
+// reference forward to f32 rounding; everything is f32 and branch-light.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dvg.h"
+
+namespace dvgk {
+
+#define DVG_DEV __device__ __forceinline__
+
+constexpr float BIG = 1.0e20f;
+
+DVG_DEV float clamp01(float v) { return v < 0.f ? 0.f : (v > 1.f ? 1.f : v); }
+DVG_DEV float minf_(float a, float b) { return a < b ? a : b; }
+DVG_DEV float maxf_(float a, float b) { return a > b ? a : b; }
+DVG_DEV float absf_(float a) { return a < 0.f ? -a : a; }
+
+// math.rs:7-10
+DVG_DEV float smoothstep_unit(float d) {
+    float t = clamp01((d + 1.0f) * 0.5f);
+    return t * t * (3.0f - 2.0f * t);
+}
+// math.rs:168-175
+DVG_DEV float d_smoothstep_unit(float d, float d_ret) {
+    if (d >= -1.0f && d <= 1.0f) {
+        float t = (d + 1.0f) * 0.5f;
+        float dt = d_ret * (6.0f * t - 6.0f * t * t);
+        return dt * 0.5f;
+    }
+    return 0.f;
+}
+
+// math.rs:47-64
+DVG_DEV float rect_sdf(float px, float py, float x0, float y0, float x1, float y1) {
+    float dx = maxf_(maxf_(x0 - px, 0.f), px - x1);
+    float dy = maxf_(maxf_(y0 - py, 0.f), py - y1);
+    float outside = sqrtf(dx * dx + dy * dy);
+    float inside = minf_(minf_(px - x0, x1 - px), minf_(py - y0, y1 - py));
+    return outside > 0.f ? outside : -inside;
+}
+
+DVG_DEV float dist_pt(float px, float py, float ax, float ay) {
+    float dx = px - ax, dy = py - ay;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+// curves.rs:39-66
+DVG_DEV void seg_dist_t(float px, float py, float ax, float ay, float bx, float by, float &dist, float &t) {
+    float vx = bx - ax, vy = by - ay, wx = px - ax, wy = py - ay;
+    float c1 = vx * wx + vy * wy;
+    float c2 = vx * vx + vy * vy;
+    t = 0.f;
+    if (c2 > 0.f && c1 > 0.f) t = clamp01(c1 / c2);
+    float qx = ax + t * vx, qy = ay + t * vy;
+    dist = sqrtf((px - qx) * (px - qx) + (py - qy) * (py - qy));
+}
+
+// curves.rs:70-93
+DVG_DEV float seg_dist(float px, float py, float ax, float ay, float bx, float by) {
+    float vx = bx - ax, vy = by - ay, wx = px - ax, wy = py - ay;
+    float c1 = vx * wx + vy * wy;
+    float c2 = vx * vx + vy * vy;
+    float d2 = (px - ax) * (px - ax) + (py - ay) * (py - ay);
+    if (c1 > 0.f) {
+        if (c2 <= c1) {
+            d2 = (px - bx) * (px - bx) + (py - by) * (py - by);
+        } else {
+            float t = c1 / c2;
+            float qx = ax + t * vx, qy = ay + t * vy;
+            d2 = (px - qx) * (px - qx) + (py - qy) * (py - qy);
+        }
+    }
+    return sqrtf(d2);
+}
+
+// curves.rs:865-893
+DVG_DEV int solve_quadratic(float a, float b, float c, float &r0, float &r1) {
+    float disc = b * b - 4.0f * a * c;
+    if (disc < 0.f) return 0;
+    float root = sqrtf(disc);
+    float q = (b < 0.f) ? -0.5f * (b - root) : -0.5f * (b + root);
+    float t0 = q / a, t1 = c / q;
+    if (t0 > t1) { r0 = t1; r1 = t0; }
+    else { r0 = t0; r1 = t1; }
+    return 2;
+}
+
+// curves.rs:949-959 (cube root through powf(x, 1/3))
+DVG_DEV float cbrt_ref(float x) {
+    const float third = 1.0f / 3.0f;
+    if (x > 0.f) return powf(x, third);
+    if (x < 0.f) return -powf(-x, third);
+    return 0.f;
+}
+
+// curves.rs:897-946
+DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
+    const float eps = 1.0e-6f;
+    if (absf_(a) < eps) {
+        return solve_quadratic(b, c, d, r[0], r[1]);
+    }
+    float bb = b / a, cc = c / a, dd = d / a;
+    const float third = 1.0f / 3.0f;
+    float q = (bb * bb - 3.0f * cc) / 9.0f;
+    float rr = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
+    float r2 = rr * rr, q3 = q * q * q;
+    if (r2 < q3) {
+        float theta = acosf(rr / sqrtf(q3));
+        float sq = sqrtf(q);
+        const float two_pi = 2.0f * 3.14159265f;
+        float off = bb * third;
+        r[0] = -2.0f * sq * cosf(theta * third) - off;
+        r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
+        r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+        return 3;
+    }
+    float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));
+    float br = (absf_(ar) > eps) ? q / ar : 0.f;
+    r[0] = (ar + br) - bb * third;
+    return 1;
+}
+
+// curves.rs:97-123
+DVG_DEV void winding_line(float px, float py, float x0, float y0, float x1, float y1, uint32_t rule, float &w,
+                          float &cr) {
+    bool a = y0 <= py, b = y1 <= py;
+    if (a != b) {
+        float t = (py - y0) / (y1 - y0);
+        float xi = x0 + t * (x1 - x0);
+        if (xi > px) {
+            if (rule == 1u) cr = 1.f - cr;
+            else w += (y1 > y0) ? 1.f : -1.f;
+        }
+    }
+}
+
+// curves.rs:127-166
+DVG_DEV void winding_quad(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                          uint32_t rule, float &w, float &cr) {
+    float ay = y0 - 2.0f * y1 + y2;
+    float by = -2.0f * y0 + 2.0f * y1;
+    float cy = y0 - py;
+    float r[2];
+    int n = solve_quadratic(ay, by, cy, r[0], r[1]);
+    for (int i = 0; i < n; ++i) {
+        float t = r[i];
+        if (t >= 0.f && t <= 1.f) {
+            float tt = 1.f - t;
+            float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
+            if (x > px) {
+                if (rule == 1u) cr = 1.f - cr;
+                else {
+                    float dy = 2.0f * ay * t + by;
+                    w += (dy > 0.f) ? 1.f : -1.f;
+                }
+            }
+        }
+    }
+}
+
+// curves.rs:170-214
+DVG_DEV void winding_cubic(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                           float x3, float y3, uint32_t rule, float &w, float &cr) {
+    float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
+    float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
+    float c = -3.0f * y0 + 3.0f * y1;
+    float d = y0 - py;
+    float r[3];
+    int n = solve_cubic(a, b, c, d, r);
+    for (int i = 0; i < n; ++i) {
+        float t = r[i];
+        if (t >= 0.f && t <= 1.f) {
+            float tt = 1.f - t;
+            float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+            if (x > px) {
+                if (rule == 1u) cr = 1.f - cr;
+                else {
+                    float dy = 3.0f * a * t * t + 2.0f * b * t + c;
+                    w += (dy > 0.f) ? 1.f : -1.f;
+                }
+            }
+        }
+    }
+}
+
+DVG_DEV float det2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
+
+// curves.rs:792-857
+DVG_DEV void quad_closest_approx(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                                 float &ox, float &oy, float &ot) {
+    float b0x = x0 - px, b0y = y0 - py, b1x = x1 - px, b1y = y1 - py, b2x = x2 - px, b2y = y2 - py;
+    float a = det2(b0x, b0y, b2x, b2y);
+    float b = 2.0f * det2(b1x, b1y, b0x, b0y);
+    float d = 2.0f * det2(b2x, b2y, b1x, b1y);
+    float f = b * d - a * a;
+    float d21x = b2x - b1x, d21y = b2y - b1y, d10x = b1x - b0x, d10y = b1y - b0y, d20x = b2x - b0x,
+          d20y = b2y - b0y;
+    float gfx = 2.0f * (b * d21x + d * d10x + a * d20x);
+    float gfy = 2.0f * (b * d21y + d * d10y + a * d20y);
+    float tmp = gfx;
+    gfx = gfy;
+    gfy = -tmp;
+    float gf_dot = gfx * gfx + gfy * gfy;
+    float t = 0.f;
+    if (gf_dot > 1.0e-8f) {
+        float ppx = -f * gfx / gf_dot, ppy = -f * gfy / gf_dot;
+        float d0px = b0x - ppx, d0py = b0y - ppy;
+        float ap = det2(d0px, d0py, d20x, d20y);
+        float bp = 2.0f * det2(d10x, d10y, d0px, d0py);
+        float denom = 2.0f * a + b + d;
+        if (absf_(denom) > 1.0e-8f) t = clamp01((ap + bp) / denom);
+        else t = clamp01((ap + bp) * 0.5f);
+    }
+    float tt = 1.f - t;
+    ox = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
+    oy = (tt * tt) * y0 + (2.0f * tt * t) * y1 + (t * t) * y2;
+    ot = t;
+}
+
+// curves.rs:218-279 (closest_point_quadratic_with_t; distance_to_quadratic is
+// the same arithmetic)
+DVG_DEV void quad_closest(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                          bool approx, float &dist, float &tout) {
+    float best_t = 0.f;
+    float best = dist_pt(px, py, x0, y0);
+    if (approx) {
+        float ax, ay, at;
+        quad_closest_approx(px, py, x0, y0, x1, y1, x2, y2, ax, ay, at);
+        best = dist_pt(px, py, ax, ay);
+        best_t = at;
+    } else {
+        float de = dist_pt(px, py, x2, y2);
+        if (de < best) { best = de; best_t = 1.f; }
+        float ax = x0 - 2.0f * x1 + x2, ay = y0 - 2.0f * y1 + y2;
+        float bx = -x0 + x1, by = -y0 + y1;
+        float cx = x0 - px, cy = y0 - py;
+        float a = ax * ax + ay * ay;
+        float b = 3.0f * (ax * bx + ay * by);
+        float c = 2.0f * (bx * bx + by * by) + (ax * cx + ay * cy);
+        float d = bx * cx + by * cy;
+        float r[3];
+        int n = solve_cubic(a, b, c, d, r);
+        for (int i = 0; i < n; ++i) {
+            float t = r[i];
+            if (t >= 0.f && t <= 1.f) {
+                float tt = 1.f - t;
+                float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
+                float y = (tt * tt) * y0 + (2.0f * tt * t) * y1 + (t * t) * y2;
+                float dd = dist_pt(px, py, x, y);
+                if (dd < best) { best = dd; best_t = t; }
+            }
+        }
+    }
+    dist = best;
+    tout = best_t;
+}
+
+DVG_DEV float eval_quintic(float t, float b, float c, float d, float e, float f) {  // curves.rs:777
+    return ((((t + b) * t + c) * t + d) * t + e) * t + f;
+}
+DVG_DEV float eval_quintic_deriv(float t, float b, float c, float d, float e) {  // curves.rs:783
+    return (((5.0f * t + 4.0f * b) * t + 3.0f * c) * t + 2.0f * d) * t + e;
+}
+
+// curves.rs:507-556
+DVG_DEV void cubic_closest_approx(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                                  float x3, float y3, float &dist, float &tout) {
+    const float inv = 1.0f / 8.0f;
+    float prev_x = x0, prev_y = y0, prev_t = 0.f;
+    float best = dist_pt(px, py, x0, y0), best_t = 0.f;
+    for (int i = 1; i <= 8; ++i) {
+        float t = (float)i * inv;
+        float tt = 1.0f - t, tt2 = tt * tt, t2 = t * t;
+        float a = tt2 * tt, b = 3.0f * tt2 * t, c = 3.0f * tt * t2, d = t2 * t;
+        float cx = a * x0 + b * x1 + c * x2 + d * x3;
+        float cy = a * y0 + b * y1 + c * y2 + d * y3;
+        float sd, st;
+        seg_dist_t(px, py, prev_x, prev_y, cx, cy, sd, st);
+        float lt = prev_t + st * (t - prev_t);
+        if (sd < best) { best = sd; best_t = lt; }
+        prev_x = cx; prev_y = cy; prev_t = t;
+    }
+    dist = best;
+    tout = best_t;
+}
+
+// curves.rs:561-731 (closest_point_cubic_with_t; distance_to_cubic,
+// curves.rs:341-505, is the same arithmetic)
+DVG_DEV void cubic_closest(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2, float x3,
+                           float y3, bool approx, float &dist, float &tout) {
+    if (approx) {
+        cubic_closest_approx(px, py, x0, y0, x1, y1, x2, y2, x3, y3, dist, tout);
+        return;
+    }
+    float best_t = 0.f;
+    float best = dist_pt(px, py, x0, y0);
+    float de = dist_pt(px, py, x3, y3);
+    if (de < best) { best = de; best_t = 1.f; }
+    float ax = -x0 + 3.0f * x1 - 3.0f * x2 + x3;
+    float ay = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
+    float bx = 3.0f * x0 - 6.0f * x1 + 3.0f * x2;
+    float by = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
+    float cx = -3.0f * x0 + 3.0f * x1;
+    float cy = -3.0f * y0 + 3.0f * y1;
+    float dx = x0 - px, dy = y0 - py;
+    float a = 3.0f * (ax * ax + ay * ay);
+    if (absf_(a) > 1.0e-8f) {
+        float b = 5.0f * (ax * bx + ay * by);
+        float c = 4.0f * (ax * cx + ay * cy) + 2.0f * (bx * bx + by * by);
+        float d = 3.0f * ((bx * cx + by * cy) + (ax * dx + ay * dy));
+        float e = (cx * cx + cy * cy) + 2.0f * (dx * bx + dy * by);
+        float f = dx * cx + dy * cy;
+        b = b / a; c = c / a; d = d / a; e = e / a; f = f / a;
+        float p1a = (2.0f / 5.0f) * c - (4.0f / 25.0f) * b * b;
+        float p1b = (3.0f / 5.0f) * d - (3.0f / 25.0f) * b * c;
+        float p1c = (4.0f / 5.0f) * e - (2.0f / 25.0f) * b * d;
+        float p1d = f - b * e / 25.0f;
+        float q_root = -b / 5.0f;
+        float pr[3];
+        int num_p = solve_cubic(p1a, p1b, p1c, p1d, pr);
+        float iv[4];
+        int ni = 0;
+        if (q_root >= 0.f && q_root <= 1.f) iv[ni++] = q_root;
+        for (int i = 0; i < num_p; ++i) iv[ni++] = pr[i];
+        for (int j = 1; j < ni; ++j) {
+            int k = j;
+            while (k > 0) {
+                float u = iv[k - 1], v = iv[k];
+                if (u <= v) break;
+                iv[k - 1] = v;
+                iv[k] = u;
+                --k;
+            }
+        }
+        float lower = 0.f;
+        int idx = 0;
+        while (idx <= ni) {
+            if (idx < ni && iv[idx] < 0.f) { ++idx; continue; }
+            float upper = (idx < ni) ? minf_(iv[idx], 1.f) : 1.f;
+            float lb = lower, ub = upper;
+            float lbe = eval_quintic(lb, b, c, d, e, f);
+            float ube = eval_quintic(ub, b, c, d, e, f);
+            if (lbe * ube > 0.f) {
+                lower = upper;
+                ++idx;
+                continue;
+            }
+            if (lbe > ube) {
+                float tmp = lb; lb = ub; ub = tmp;
+            }
+            float t = (lb + ub) * 0.5f;
+            for (int it = 0; it < 20; ++it) {
+                if (t < lb || t > ub) t = (lb + ub) * 0.5f;
+                float value = eval_quintic(t, b, c, d, e, f);
+                if (absf_(value) < 1.0e-5f || it == 19) break;
+                if (value > 0.f) ub = t; else lb = t;
+                float der = eval_quintic_deriv(t, b, c, d, e);
+                t = t - value / der;
+            }
+            if (t >= 0.f && t <= 1.f) {
+                float tt = 1.f - t;
+                float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+                float y = (tt * tt * tt) * y0 + (3.0f * tt * tt * t) * y1 + (3.0f * tt * t * t) * y2 + (t * t * t) * y3;
+                float dd = dist_pt(px, py, x, y);
+                if (dd < best) { best = dd; best_t = t; }
+            }
+            if (upper >= 1.f) break;
+            lower = upper;
+            ++idx;
+        }
+    }
+    dist = best;
+    tout = best_t;
+}
+
+// Closest point on an axis-aligned ellipse, src/distance/shape.rs:117-164
+// (the CPU query used by the backward pass).
+DVG_DEV void ellipse_closest(float cx, float cy, float rx_, float ry_, float px, float py, float &ox, float &oy) {
+    float rx = absf_(rx_), ry = absf_(ry_);
+    const float eps = 1.0e-6f;
+    if (rx < eps && ry < eps) { ox = cx; oy = cy; return; }
+    if (rx < eps) { ox = cx; oy = cy + fminf(fmaxf(py - cy, -ry), ry); return; }
+    if (ry < eps) { ox = cx + fminf(fmaxf(px - cx, -rx), rx); oy = cy; return; }
+    float dx = px - cx, dy = py - cy;
+    if (absf_(dx) < eps && absf_(dy) < eps) { ox = cx + rx; oy = cy; return; }
+    float sx = dx < 0.f ? -1.f : 1.f, sy = dy < 0.f ? -1.f : 1.f;
+    dx = absf_(dx);
+    dy = absf_(dy);
+    float t = atan2f(dy * rx, dx * ry);
+    for (int i = 0; i < 20; ++i) {
+        float s = sinf(t), co = cosf(t);
+        float g = rx * dx * s - ry * dy * co + (ry * ry - rx * rx) * s * co;
+        float gt = rx * dx * co + ry * dy * s + (ry * ry - rx * rx) * (co * co - s * s);
+        if (absf_(gt) < 1.0e-12f) break;
+        float next = t - g / gt;
+        next = next < 0.f ? 0.f : (next > 1.57079632679489661923f ? 1.57079632679489661923f : next);
+        if (absf_(next - t) < 1.0e-6f) { t = next; break; }
+        t = next;
+    }
+    float s = sinf(t), co = cosf(t);
+    ox = cx + sx * rx * co;
+    oy = cy + sy * ry * s;
+}
+
+// ---- PCG32, src/renderer/rng.rs:16-43 (native 64-bit integer arithmetic) ----
+struct Pcg32 {
+    uint64_t state, inc;
+    DVG_DEV Pcg32(uint64_t idx, uint64_t seed) {
+        state = 0;
+        inc = ((idx + 1) << 1) | 1;
+        next_u32();
+        state = state + (0x853c49e6748fea9bULL + seed);
+        next_u32();
+    }
+    DVG_DEV uint32_t next_u32() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + (inc | 1);
+        uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+        uint32_t rot = (uint32_t)(old >> 59);
+        return (xs >> rot) | (xs << ((32u - rot) & 31u));
+    }
+    DVG_DEV float next_f32() {
+        return (float)(next_u32() >> 9) * (1.0f / 8388608.0f);
+    }
+};
+
+// rng.rs:167-197: the stratified jitter of sample (x, y, sx, sy); the stream
+// index is ((y*W + x)*nsy + sy)*nsx + sx (u32 arithmetic, as the kernels do).
+DVG_DEV void jitter_xy(uint32_t x, uint32_t y, uint32_t sx, uint32_t sy, uint32_t W, uint32_t nsx, uint32_t nsy,
+                       uint32_t seed, bool jitter, float &rx, float &ry) {
+    rx = 0.5f;
+    ry = 0.5f;
+    if (jitter) {
+        uint32_t idx = ((y * W + x) * nsy + sy) * nsx + sx;
+        Pcg32 rng((uint64_t)idx, (uint64_t)seed);
+        rx = rng.next_f32();
+        ry = rng.next_f32();
+    }
+}
+
+// sampling.rs:143-181
+DVG_DEV float filter_weight(uint32_t type, float dx, float dy, float radius) {
+    float adx = absf_(dx), ady = absf_(dy);
+    if (radius <= 0.f) return 1.f;
+    if (adx > radius || ady > radius) return 0.f;
+    if (type == DVG_FILTER_BOX) {
+        float denom = (2.0f * radius) * (2.0f * radius);
+        return 1.0f / denom;
+    }
+    if (type == DVG_FILTER_TENT) {
+        float fx = radius - adx, fy = radius - ady;
+        float norm = 1.0f / (radius * radius);
+        return fx * fy * norm * norm;
+    }
+    if (type == DVG_FILTER_RADIAL_PARABOLIC) {
+        float rx = 1.0f - (dx / radius) * (dx / radius);
+        float ry = 1.0f - (dy / radius) * (dy / radius);
+        return (4.0f / 3.0f) * rx * (4.0f / 3.0f) * ry;
+    }
+    if (type == DVG_FILTER_HANN) {
+        const float pi = 3.14159265f;
+        float ndx = (dx / (2.0f * radius)) + 0.5f;
+        float ndy = (dy / (2.0f * radius)) + 0.5f;
+        float fx = 0.5f * (1.0f - cosf(2.0f * pi * ndx));
+        float fy = 0.5f * (1.0f - cosf(2.0f * pi * ndy));
+        float norm = 1.0f / (radius * radius);
+        return fx * fy * norm;
+    }
+    return 1.f;
+}
+
+}  // namespace dvgk

@@ -1,0 +1,68 @@
+// Device scene layout and kernel launch interface shared by the host code and
+// dvg_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dvgk {
+
+constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.rs:21)
+constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
+constexpr int FRAG_WINDOW = 16;     // fragments held per lane in the adjoint walk
+constexpr int LDS_GROUP_CAP = 512;  // tile groups whose paint grads are staged in LDS
+
+// Device-resident scene: the packed buffers of PreparedScene /
+// PreparedBackwardScene / BoundarySamplingData plus the tile lists.
+struct DevScene {
+    uint32_t W, H, nsx, nsy, spp, seed, jitter, prefilter, filter_type;
+    float filter_radius;
+    int ri;
+    float bg[4];            // straight RGBA
+    const float *bg_image;  // W*H*4 or null
+    // forward pack
+    uint32_t num_groups, num_prepared;
+    const float *shape_data, *curve_data, *shape_bounds, *group_data, *group_xform, *group_shape_xform,
+        *group_inv_scale, *group_shapes, *shape_xform, *gradient_data, *stop_offsets, *stop_colors;
+    // tiles
+    uint32_t tiles_x, tiles_y;
+    const uint32_t *tile_offsets, *tile_entries;
+    // backward pack (scene-indexed)
+    uint32_t num_shapes;
+    const uint32_t *prepared_to_scene;
+    const float *shape_transform;  // 6*S
+    const float *path_points;
+    const uint32_t *path_ctrl;
+    const float *path_thickness;
+    const uint32_t *sp_off, *sp_cnt, *sc_off, *sc_cnt, *st_off, *st_cnt, *s_closed;
+    uint32_t n_cdf;
+    const float *shape_lengths, *shape_cdf, *shape_pmf;
+    const uint32_t *shape_ids, *group_ids;
+    const float *path_cdf, *path_pmf;
+    const uint32_t *path_point_ids, *path_cdf_off, *path_cdf_cnt, *path_point_off;
+    const uint32_t *scene_to_prepared;  // S: first prepared instance of each scene shape
+};
+
+struct DevGrads {
+    float *d_shape_params, *d_shape_points, *d_shape_thickness, *d_shape_stroke_width, *d_shape_transform,
+        *d_group_transform, *d_group_data, *d_gradient_data, *d_stop_offsets, *d_stop_colors, *d_filter_radius,
+        *d_background, *d_background_image, *d_translation;
+};
+
+// Kernel launchers (dvg_kernels.hip).  All asynchronous on `stream`.
+void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
+                      hipStream_t stream);
+void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream);
+void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
+                      const uint32_t *offsets, uint32_t *entries, hipStream_t stream);
+// Filter weight sums; `cpu` positions samples as the CPU backward does.
+void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream);
+void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream);
+void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
+void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                              hipStream_t stream);
+void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                     hipStream_t stream);
+// Zero `count` (<= 14) device float arrays of the given lengths.
+void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
+
+}  // namespace dvgk

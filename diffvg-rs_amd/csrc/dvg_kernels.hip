@@ -1,0 +1,1401 @@
+// gfx950 kernels of the render/backward path.
+//
+//   k_bin_count / k_scan / k_bin_write   tile lists (binning.rs:10-313, tiles.rs:12-38)
+//   k_weights                            filter weight sums (forward.rs:1506-1565), by gather
+//   k_forward                            per-sample scene evaluation + splat (forward.rs:1568-1735)
+//   k_resolve                            splat resolve (forward.rs:295-334)
+//   k_backward_interior                  colour adjoint per sample (backward/sampling.rs:76-224)
+//   k_boundary                           edge sampling (backward/boundary.rs:113-1148)
+//
+// One workgroup of 256 lanes (4 wave64) owns one 16x16 tile; its samples are
+// visited in 4x4-pixel blocks so that a wave covers a compact footprint and
+// the tile's group list is wave-uniform (scalar loads).  Gradients of the
+// tile's solid paints are staged in LDS and flushed once per tile; scalar
+// gradients (background, filter radius) are wave-reduced before any global
+// atomic.
+#include "dvg_device.h"
+#include "dvg_kernels.h"
+
+namespace dvgk {
+
+// ---------------------------------------------------------------------------
+// Tile binning: group g belongs to tile t iff t lies in g's clamped tile range
+// (binning.rs:30-110).  One lane per tile scans the groups in ascending order,
+// so the lists come out sorted without the host sort of renderer.rs:364-367.
+__global__ void k_bin_count(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
+                            uint32_t *__restrict__ counts) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles_x * tiles_y) return;
+    int tx = (int)(t % tiles_x), ty = (int)(t / tiles_x);
+    uint32_t c = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        int4 r = rng[g];
+        c += (tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w) ? 1u : 0u;
+    }
+    counts[t] = c;
+}
+
+__global__ void k_bin_write(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
+                            const uint32_t *__restrict__ offsets, uint32_t *__restrict__ entries) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles_x * tiles_y) return;
+    int tx = (int)(t % tiles_x), ty = (int)(t / tiles_x);
+    uint32_t o = offsets[t];
+    for (uint32_t g = 0; g < G; ++g) {
+        int4 r = rng[g];
+        if (tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w) entries[o++] = g;
+    }
+}
+
+// Exclusive scan of n counts into offsets[0..n] by one workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ counts, uint32_t n,
+                                               uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t part[1024];
+    uint32_t tid = threadIdx.x;
+    uint32_t per = (n + 1023) / 1024;
+    uint32_t b = tid * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += counts[i];
+    part[tid] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid == 0 ? 0u : part[tid - 1];
+    for (uint32_t i = b; i < e; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (tid == 1023) offsets[n] = part[1023];
+}
+
+// ---------------------------------------------------------------------------
+// Sample addressing inside a tile: pixel p of the tile is visited in 4x4
+// blocks (16 consecutive p share a block), sample s of pixel p at p*spp + s.
+DVG_DEV void tile_sample(const DevScene &S, uint32_t tile, uint32_t s, uint32_t &x, uint32_t &y, uint32_t &sx,
+                         uint32_t &sy) {
+    uint32_t pl = s / S.spp, si = s - pl * S.spp;
+    uint32_t lx = (pl & 3u) | (((pl >> 4) & 3u) << 2);
+    uint32_t ly = ((pl >> 2) & 3u) | ((pl >> 6) << 2);
+    x = (tile % S.tiles_x) * TILE + lx;
+    y = (tile / S.tiles_x) * TILE + ly;
+    sx = si % S.nsx;
+    sy = si / S.nsx;
+}
+
+// Sample position: forward.rs:1647-1650 multiplies by 1/n; the CPU backward
+// (render.rs:108-109, filters.rs:45-46) divides by n.
+template <bool CPU>
+DVG_DEV void sample_point(const DevScene &S, uint32_t x, uint32_t y, uint32_t sx, uint32_t sy, float &px,
+                          float &py) {
+    float rx, ry;
+    jitter_xy(x, y, sx, sy, S.W, S.nsx, S.nsy, S.seed, S.jitter != 0, rx, ry);
+    if (CPU) {
+        px = (float)x + ((float)sx + rx) / (float)S.nsx;
+        py = (float)y + ((float)sy + ry) / (float)S.nsy;
+    } else {
+        float inv_sx = 1.0f / (float)S.nsx, inv_sy = 1.0f / (float)S.nsy;
+        px = (float)x + ((float)sx + rx) * inv_sx;
+        py = (float)y + ((float)sy + ry) * inv_sy;
+    }
+}
+
+// ---- paints ------------------------------------------------------------------
+// sampling.rs:11-139 (forward semantics: clamped radius and stop gap, t <= first)
+DVG_DEV float4 paint_fwd(const DevScene &S, uint32_t kind, uint32_t gidx, const float *solid, float px, float py) {
+    float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (kind == DVG_PAINT_SOLID) return make_float4(solid[0], solid[1], solid[2], solid[3]);
+    if (kind != DVG_PAINT_LINEAR && kind != DVG_PAINT_RADIAL) return out;
+    const float *gd = S.gradient_data + gidx * 8u;
+    uint32_t gtype = (uint32_t)gd[0];
+    float p0 = gd[1], p1 = gd[2], p2 = gd[3], p3 = gd[4];
+    uint32_t so = (uint32_t)gd[5], sc = (uint32_t)gd[6];
+    float t;
+    if (gtype == 0) {
+        float vx = p2 - p0, vy = p3 - p1;
+        float denom = maxf_(vx * vx + vy * vy, 1.0e-3f);
+        t = ((px - p0) * vx + (py - p1) * vy) / denom;
+    } else {
+        float dx = (px - p0) / maxf_(p2, 1.0e-3f);
+        float dy = (py - p1) / maxf_(p3, 1.0e-3f);
+        t = sqrtf(dx * dx + dy * dy);
+    }
+    if (sc == 0) return out;
+    const float *offs = S.stop_offsets + so;
+    const float *cols = S.stop_colors + so * 4u;
+    if (t <= offs[0]) return make_float4(cols[0], cols[1], cols[2], cols[3]);
+    if (t >= offs[sc - 1]) {
+        const float *c = cols + (sc - 1) * 4u;
+        return make_float4(c[0], c[1], c[2], c[3]);
+    }
+    for (uint32_t i = 0; i + 1 < sc; ++i) {
+        float cu = offs[i], nx = offs[i + 1];
+        if (t >= cu && t < nx) {
+            float tt = (t - cu) / maxf_(nx - cu, 1.0e-5f);
+            const float *c0 = cols + i * 4u, *c1 = c0 + 4;
+            return make_float4(c0[0] * (1.f - tt) + c1[0] * tt, c0[1] * (1.f - tt) + c1[1] * tt,
+                               c0[2] * (1.f - tt) + c1[2] * tt, c0[3] * (1.f - tt) + c1[3] * tt);
+        }
+    }
+    return out;
+}
+
+// backward/paint.rs:8-74 (CPU semantics)
+DVG_DEV float4 paint_cpu(const DevScene &S, uint32_t kind, uint32_t gidx, const float *solid, float px, float py) {
+    if (kind == DVG_PAINT_SOLID) return make_float4(solid[0], solid[1], solid[2], solid[3]);
+    float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (kind != DVG_PAINT_LINEAR && kind != DVG_PAINT_RADIAL) return zero;
+    const float *gd = S.gradient_data + gidx * 8u;
+    uint32_t gtype = (uint32_t)gd[0];
+    float p0 = gd[1], p1 = gd[2], p2 = gd[3], p3 = gd[4];
+    uint32_t so = (uint32_t)gd[5], sc = (uint32_t)gd[6];
+    float t;
+    if (gtype == 0) {
+        float ex = p2 - p0, ey = p3 - p1;
+        float denom = fmaxf(ex * ex + ey * ey, 1.0e-3f);
+        t = ((px - p0) * ex + (py - p1) * ey) / denom;
+    } else {
+        float nx = (px - p0) / p2, ny = (py - p1) / p3;
+        t = sqrtf(nx * nx + ny * ny);
+    }
+    if (sc == 0) return zero;
+    const float *offs = S.stop_offsets + so;
+    const float *cols = S.stop_colors + so * 4u;
+    if (t < offs[0]) return make_float4(cols[0], cols[1], cols[2], cols[3]);
+    for (uint32_t i = 0; i + 1 < sc; ++i) {
+        float cu = offs[i], nx = offs[i + 1];
+        if (t >= cu && t < nx) {
+            float tt = (t - cu) / (nx - cu);
+            const float *c0 = cols + i * 4u, *c1 = c0 + 4;
+            return make_float4(c0[0] * (1.0f - tt) + c1[0] * tt, c0[1] * (1.0f - tt) + c1[1] * tt,
+                               c0[2] * (1.0f - tt) + c1[2] * tt, c0[3] * (1.0f - tt) + c1[3] * tt);
+        }
+    }
+    const float *c = cols + (sc - 1) * 4u;
+    return make_float4(c[0], c[1], c[2], c[3]);
+}
+
+// ---- per-shape coverage --------------------------------------------------------
+struct Cov {
+    float fill_min, winding, crossings, stroke_min, stroke_r, stroke_hit;
+};
+
+DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, float r3) {
+    if (kind == 0) return r0 + t * (r1 - r0);
+    float tt = 1.f - t;
+    if (kind == 1) return tt * tt * r0 + 2.0f * tt * t * r1 + t * t * r2;
+    return tt * tt * tt * r0 + 3.0f * tt * tt * t * r1 + 3.0f * tt * t * t * r2 + t * t * t * r3;
+}
+
+// accumulate_shape_fill, forward.rs:409-560 (distances only when prefiltering:
+// the non-prefiltered blend reads the winding alone).
+DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool prefilter, float px, float py,
+                        Cov &c) {
+    uint32_t kind = (uint32_t)sd[0];
+    float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
+    if (kind != DVG_SHAPE_PATH) {
+        float dist;
+        if (kind == DVG_SHAPE_CIRCLE) {
+            float dx = px - q0, dy = py - q1;
+            dist = sqrtf(dx * dx + dy * dy) - q2;
+        } else if (kind == DVG_SHAPE_ELLIPSE) {
+            float dx = (px - q0) / maxf_(absf_(q2), 1.0e-3f);
+            float dy = (py - q1) / maxf_(absf_(q3), 1.0e-3f);
+            float len = sqrtf(dx * dx + dy * dy);
+            dist = (len - 1.0f) * minf_(absf_(q2), absf_(q3));
+        } else {
+            dist = rect_sdf(px, py, q0, q1, q2, q3);
+        }
+        float ad = absf_(dist);
+        if (ad < c.fill_min) c.fill_min = ad;
+        if (dist <= 0.f) {
+            if (rule == 1u) c.crossings = 1.f - c.crossings;
+            else c.winding += 1.f;
+        }
+        return;
+    }
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    bool approx = sd[14] > 0.5f;
+    float lmin = BIG, lw = 0.f, lc = 0.f;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = S.curve_data + (off + s) * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)), bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)), by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+        if (prefilter) {
+            // forward.rs:889-937: segments farther than the running minimum
+            // cannot lower it.
+            float ddx = maxf_(maxf_(bx0 - px, 0.f), px - bx1), ddy = maxf_(maxf_(by0 - py, 0.f), py - by1);
+            if (sqrtf(ddx * ddx + ddy * ddy) <= lmin) {
+                float d, t;
+                if (k == 0) d = seg_dist(px, py, x0, y0, x1, y1);
+                else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+                else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+                if (d < lmin) lmin = d;
+            }
+        }
+        // forward.rs:938-1018: a +x ray only meets segments spanning py with
+        // some control point right of px.
+        if (py < by0 || py > by1 || px > bx1) continue;
+        if (k == 0) winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
+        else if (k == 1) winding_quad(px, py, x0, y0, x1, y1, x2, y2, rule, lw, lc);
+        else winding_cubic(px, py, x0, y0, x1, y1, x2, y2, x3, y3, rule, lw, lc);
+    }
+    if (lmin < c.fill_min) c.fill_min = lmin;
+    if (rule == 1u) {
+        if (lc > 0.f) c.crossings = 1.f - c.crossings;
+    } else {
+        c.winding += lw;
+    }
+}
+
+// accumulate_shape_stroke, forward.rs:565-709.  With CPU = true the ellipse
+// uses the exact closest point of src/distance/shape.rs:117-164 and a shape
+// needs stroke_width > 0, as the CPU query within_distance_bvh does
+// (distance/api.rs:79-135).
+template <bool CPU>
+DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, float px, float py, Cov &c) {
+    uint32_t kind = (uint32_t)sd[0];
+    float sw = sd[3];
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    if (CPU) {
+        if (!(sw > 0.f)) return;
+    } else {
+        if (sw <= 0.f && cnt == 0) return;
+    }
+    float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
+    if (kind != DVG_SHAPE_PATH) {
+        float dist;
+        if (kind == DVG_SHAPE_CIRCLE) {
+            float dx = px - q0, dy = py - q1;
+            dist = absf_(sqrtf(dx * dx + dy * dy) - q2);
+        } else if (kind == DVG_SHAPE_ELLIPSE) {
+            if (CPU) {
+                // ellipse_signed_distance (distance/shape.rs:204-212)
+                float rx = fmaxf(absf_(q2), 1.0e-6f), ry = fmaxf(absf_(q3), 1.0e-6f);
+                float ox, oy;
+                ellipse_closest(q0, q1, rx, ry, px, py, ox, oy);
+                dist = dist_pt(px, py, ox, oy);
+            } else {
+                float dx = (px - q0) / maxf_(absf_(q2), 1.0e-3f);
+                float dy = (py - q1) / maxf_(absf_(q3), 1.0e-3f);
+                float len = sqrtf(dx * dx + dy * dy);
+                dist = absf_((len - 1.0f) * minf_(absf_(q2), absf_(q3)));
+            }
+        } else {
+            dist = absf_(rect_sdf(px, py, q0, q1, q2, q3));
+        }
+        if (prefilter) {
+            if (dist < c.stroke_min) { c.stroke_min = dist; c.stroke_r = sw; }
+        } else if (dist < sw) {
+            c.stroke_hit = 1.f;
+        }
+        return;
+    }
+    bool approx = sd[14] > 0.5f;
+    float lmin = BIG, lr = 0.f;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = S.curve_data + (off + s) * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+        float r0 = cv[9], r1 = cv[10], r2 = cv[11], r3 = cv[12];
+        float pad = fmaxf(fmaxf(absf_(r0), absf_(r1)), fmaxf(absf_(r2), absf_(r3)));
+        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)) - pad, bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) + pad;
+        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)) - pad, by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) + pad;
+        if (prefilter) {
+            float ddx = maxf_(maxf_(bx0 + pad - px, 0.f), px - (bx1 - pad));
+            float ddy = maxf_(maxf_(by0 + pad - py, 0.f), py - (by1 - pad));
+            if (sqrtf(ddx * ddx + ddy * ddy) > lmin) continue;
+        } else if (px < bx0 || px > bx1 || py < by0 || py > by1) {
+            continue;  // forward.rs:1048-1050: outside the radius-padded bounds
+        }
+        float d, t;
+        if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
+        else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+        else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        float r = seg_radius(k, t, r0, r1, r2, r3);
+        if (prefilter) {
+            if (d < lmin) { lmin = d; lr = r; }
+        } else if (d < r) {
+            c.stroke_hit = 1.f;
+            return;
+        }
+    }
+    if (prefilter && lmin < c.stroke_min) {
+        c.stroke_min = lmin;
+        c.stroke_r = lr;
+    }
+}
+
+// accumulate_group_shapes + accumulate_shape_in_group (forward.rs:1295-1503):
+// every shape instance of group g whose padded bounds hold the group-local
+// point contributes; (lx, ly) is the group-local point.
+template <bool CPU>
+DVG_DEV void group_cov(const DevScene &S, const float *gd, float lx, float ly, bool do_fill, bool do_stroke,
+                       uint32_t rule, bool prefilter, Cov &c) {
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    for (uint32_t i = 0; i < cnt; ++i) {
+        uint32_t si = (uint32_t)S.group_shapes[off + i];
+        const float *b = S.shape_bounds + si * 4u;
+        bool inb = b[0] <= b[2] && b[1] <= b[3];
+        if (inb && (lx < b[0] || lx > b[2] || ly < b[1] || ly > b[3])) inb = false;
+        if (!inb) continue;
+        const float *xf = S.shape_xform + si * 6u;
+        float spx = xf[0] * lx + xf[1] * ly + xf[2];
+        float spy = xf[3] * lx + xf[4] * ly + xf[5];
+        const float *sd = S.shape_data + si * 15u;
+        if (do_fill) shape_fill(S, sd, rule, prefilter, spx, spy, c);
+        if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<CPU>(S, sd, prefilter, spx, spy, c);
+    }
+}
+
+DVG_DEV void group_local(const DevScene &S, uint32_t g, float px, float py, float &lx, float &ly) {
+    const float *x = S.group_xform + g * 6u;
+    lx = x[0] * px + x[1] * py + x[2];
+    ly = x[3] * px + x[4] * py + x[5];
+}
+
+DVG_DEV void blend(float4 &o, float4 c, float a) {
+    float inv = 1.f - a;
+    o.x = c.x * a + o.x * inv;
+    o.y = c.y * a + o.y * inv;
+    o.z = c.z * a + o.z * inv;
+    o.w = a + o.w * inv;
+}
+
+// eval_scene_tiled, forward.rs:1133-1290: premultiplied RGBA at (px, py).
+DVG_DEV float4 eval_forward(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, float4 bg) {
+    float4 out = make_float4(bg.x * bg.w, bg.y * bg.w, bg.z * bg.w, bg.w);
+    bool prefilter = S.prefilter != 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        uint32_t g = list[e];
+        const float *gd = S.group_data + g * 16u;
+        uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+        if (fk == DVG_PAINT_NONE && sk == DVG_PAINT_NONE) continue;
+        float lx, ly;
+        group_local(S, g, px, py, lx, ly);
+        Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
+        group_cov<false>(S, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
+        float inv_scale = S.group_inv_scale[g];
+        float gscale = inv_scale > 0.f ? 1.f / inv_scale : 1.f;
+        // blend_group, forward.rs:339-404: stroke first, then fill
+        if (sk != DVG_PAINT_NONE) {
+            float cov = 0.f;
+            if (prefilter) {
+                float smin = c.stroke_min * gscale, sr = c.stroke_r * gscale;
+                if (smin < BIG && sr > 0.f) cov = smoothstep_unit(smin + sr) - smoothstep_unit(smin - sr);
+            } else if (c.stroke_hit > 0.f) {
+                cov = 1.f;
+            }
+            if (cov > 0.f) {
+                float4 col = paint_fwd(S, sk, (uint32_t)gd[5], gd + 12, px, py);
+                blend(out, col, cov * col.w);
+            }
+        }
+        if (fk != DVG_PAINT_NONE) {
+            bool inside = (rule == 1u) ? c.crossings > 0.f : c.winding != 0.f;
+            float cov = 0.f;
+            if (prefilter) {
+                float fmin = c.fill_min * gscale;
+                if (fmin < BIG) cov = smoothstep_unit(inside ? fmin : -fmin);
+            } else if (inside) {
+                cov = 1.f;
+            }
+            if (cov > 0.f) {
+                float4 col = paint_fwd(S, fk, (uint32_t)gd[3], gd + 8, px, py);
+                blend(out, col, cov * col.w);
+            }
+        }
+    }
+    return out;
+}
+
+DVG_DEV float4 background_at(const DevScene &S, uint32_t x, uint32_t y) {
+    if (S.bg_image) {
+        const float *b = S.bg_image + ((size_t)y * S.W + x) * 4u;
+        return make_float4(b[0], b[1], b[2], b[3]);
+    }
+    return make_float4(S.bg[0], S.bg[1], S.bg[2], S.bg[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Filter weight sum of every pixel, gathered from the samples of its
+// neighbourhood (the same sum rasterize_weights_f32 scatters).
+template <bool CPU>
+__global__ void k_weights(DevScene S, float *__restrict__ weight) {
+    uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= S.W || y >= S.H) return;
+    float xc = (float)x + 0.5f, yc = (float)y + 0.5f;
+    float sum = 0.f;
+    int ri = S.ri;
+    for (int dy = -ri; dy <= ri; ++dy)
+        for (int dx = -ri; dx <= ri; ++dx) {
+            int sx_ = (int)x - dx, sy_ = (int)y - dy;
+            if (sx_ < 0 || sy_ < 0 || sx_ >= (int)S.W || sy_ >= (int)S.H) continue;
+            for (uint32_t sy = 0; sy < S.nsy; ++sy)
+                for (uint32_t sx = 0; sx < S.nsx; ++sx) {
+                    float px, py;
+                    sample_point<CPU>(S, (uint32_t)sx_, (uint32_t)sy_, sx, sy, px, py);
+                    float w = filter_weight(S.filter_type, xc - px, yc - py, S.filter_radius);
+                    if (w > 0.f) sum += w;
+                }
+        }
+    weight[(size_t)y * S.W + x] = sum;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_forward(DevScene S, const float *__restrict__ weight,
+                                                   float *__restrict__ accum) {
+    uint32_t tile = blockIdx.x;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    const uint32_t *list = S.tile_entries + lb;
+    uint32_t n = le - lb;
+    uint32_t total = TILE * TILE * S.spp;
+    int ri = S.ri;
+    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+        uint32_t x, y, sx, sy;
+        tile_sample(S, tile, s, x, y, sx, sy);
+        if (x >= S.W || y >= S.H) continue;
+        float px, py;
+        sample_point<false>(S, x, y, sx, sy, px, py);
+        float4 col = eval_forward(S, list, n, px, py, background_at(S, x, y));
+        for (int dy = -ri; dy <= ri; ++dy)
+            for (int dx = -ri; dx <= ri; ++dx) {
+                int xx = (int)x + dx, yy = (int)y + dy;
+                if (xx < 0 || yy < 0 || xx >= (int)S.W || yy >= (int)S.H) continue;
+                size_t base = (size_t)yy * S.W + xx;
+                float ws = weight[base];
+                if (ws > 0.f) {
+                    float w = filter_weight(S.filter_type, ((float)xx + 0.5f) - px, ((float)yy + 0.5f) - py,
+                                            S.filter_radius);
+                    if (w > 0.f) {
+                        float wn = w / ws;
+                        float *a = accum + base * 4u;
+                        unsafeAtomicAdd(a + 0, col.x * wn);
+                        unsafeAtomicAdd(a + 1, col.y * wn);
+                        unsafeAtomicAdd(a + 2, col.z * wn);
+                        unsafeAtomicAdd(a + 3, col.w * wn);
+                    }
+                }
+            }
+    }
+}
+
+// resolve_splat_f32 with the premultiplied background of renderer.rs:499-515.
+__global__ void k_resolve(DevScene S, const float *__restrict__ weight, const float *__restrict__ accum,
+                          float *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.W * S.H) return;
+    float a = S.bg[3];
+    float r = S.bg[0] * a, g = S.bg[1] * a, b = S.bg[2] * a;
+    if (weight[i] > 0.f) {
+        float4 c = reinterpret_cast<const float4 *>(accum)[i];
+        r = c.x; g = c.y; b = c.z; a = c.w;
+    }
+    if (a > 1.0e-8f) { r = r / a; g = g / a; b = b / a; }
+    reinterpret_cast<float4 *>(out)[i] = make_float4(r, g, b, a);
+}
+
+// ---------------------------------------------------------------------------
+// Backward (src/backward/*.rs semantics).
+
+// compute_filter_weight (backward/filters.rs:159-188)
+DVG_DEV float fw_cpu(uint32_t type, float radius, float dx, float dy) {
+    if (radius <= 0.f) return 1.f;
+    if (fabsf(dx) > radius || fabsf(dy) > radius) return 0.f;
+    if (type == DVG_FILTER_BOX) { float d = 2.0f * radius; return 1.0f / (d * d); }
+    if (type == DVG_FILTER_TENT) {
+        float fx = radius - fabsf(dx), fy = radius - fabsf(dy);
+        float norm = 1.0f / (radius * radius);
+        return fx * fy * norm * norm;
+    }
+    if (type == DVG_FILTER_RADIAL_PARABOLIC) {
+        float rx = 1.0f - (dx / radius) * (dx / radius), ry = 1.0f - (dy / radius) * (dy / radius);
+        return (4.0f / 3.0f) * rx * (4.0f / 3.0f) * ry;
+    }
+    const float PI = 3.14159274101257324219f;  // core::f32::consts::PI
+    float ndx = (dx / (2.0f * radius)) + 0.5f, ndy = (dy / (2.0f * radius)) + 0.5f;
+    float fx = 0.5f * (1.0f - cosf(2.0f * PI * ndx)), fy = 0.5f * (1.0f - cosf(2.0f * PI * ndy));
+    float norm = 1.0f / (radius * radius);
+    return fx * fy * norm;
+}
+
+// d_compute_filter_weight (backward/filters.rs:190-239)
+DVG_DEV float d_fw_cpu(uint32_t type, float radius, float dx, float dy, float d_ret) {
+    if (type == DVG_FILTER_BOX) {
+        float denom = 2.0f * radius;
+        return denom != 0.f ? d_ret * (-2.0f) * denom / ((denom * denom) * denom) : 0.f;
+    }
+    if (type == DVG_FILTER_TENT) {
+        float fx = radius - fabsf(dx), fy = radius - fabsf(dy);
+        float norm = 1.0f / (radius * radius);
+        float d_fx = d_ret * fy * norm, d_fy = d_ret * fx * norm, d_norm = d_ret * fx * fy;
+        float r2 = radius * radius;
+        return radius != 0.f ? d_fx + d_fy + (-4.0f) * d_norm / ((r2 * r2) * radius) : 0.f;
+    }
+    if (type == DVG_FILTER_RADIAL_PARABOLIC) {
+        float r3 = radius * radius * radius;
+        return r3 != 0.f ? -(2.0f * dx * dx + 2.0f * dy * dy) / r3 : 0.f;
+    }
+    const float PI = 3.14159274101257324219f;
+    float ndx = (dx / (2.0f * radius)) + 0.5f, ndy = (dy / (2.0f * radius)) + 0.5f;
+    float fx = 0.5f * (1.0f - cosf(2.0f * PI * ndx)), fy = 0.5f * (1.0f - cosf(2.0f * PI * ndy));
+    float norm = 1.0f / (radius * radius);
+    float d_fx = d_ret * fy * norm, d_fy = d_ret * fx * norm, d_norm = d_ret * fx * fy;
+    float d_ndx = d_fx * 0.5f * sinf(2.0f * PI * ndx) * (2.0f * PI);
+    float d_ndy = d_fy * 0.5f * sinf(2.0f * PI * ndy) * (2.0f * PI);
+    float d2 = 2.0f * radius;
+    return d_ndx * (-2.0f * dx / (d2 * d2)) + d_ndy * (-2.0f * dy / (d2 * d2)) +
+           (-2.0f) * d_norm / ((radius * radius) * radius);
+}
+
+// gather_d_color (backward/filters.rs:70-103)
+DVG_DEV float4 gather_d_color(const DevScene &S, const float *weight, const float *d_render, float ptx, float pty) {
+    int x = (int)floorf(ptx), y = (int)floorf(pty);
+    int ri = (int)ceilf(S.filter_radius);
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int dy = -ri; dy <= ri; ++dy)
+        for (int dx = -ri; dx <= ri; ++dx) {
+            int xx = x + dx, yy = y + dy;
+            if (xx < 0 || yy < 0 || xx >= (int)S.W || yy >= (int)S.H) continue;
+            float w = fw_cpu(S.filter_type, S.filter_radius, ((float)xx + 0.5f) - ptx, ((float)yy + 0.5f) - pty);
+            size_t b = (size_t)yy * S.W + xx;
+            float ws = weight[b];
+            if (ws > 0.f) {
+                float4 d = reinterpret_cast<const float4 *>(d_render)[b];
+                o.x += (w / ws) * d.x;
+                o.y += (w / ws) * d.y;
+                o.z += (w / ws) * d.z;
+                o.w += (w / ws) * d.w;
+            }
+        }
+    return o;
+}
+
+// accumulate_filter_gradient (backward/filters.rs:106-156), returning d_radius.
+DVG_DEV float filter_grad(const DevScene &S, const float *weight, const float *d_render, float4 col, float ptx,
+                          float pty) {
+    int x = (int)floorf(ptx), y = (int)floorf(pty);
+    int ri = (int)ceilf(S.filter_radius);
+    float acc = 0.f;
+    for (int dy = -ri; dy <= ri; ++dy)
+        for (int dx = -ri; dx <= ri; ++dx) {
+            int xx = x + dx, yy = y + dy;
+            if (xx < 0 || yy < 0 || xx >= (int)S.W || yy >= (int)S.H) continue;
+            size_t b = (size_t)yy * S.W + xx;
+            float ws = weight[b];
+            if (ws <= 0.f) continue;
+            float xc = (float)xx + 0.5f, yc = (float)yy + 0.5f;
+            float w = fw_cpu(S.filter_type, S.filter_radius, xc - ptx, yc - pty);
+            if (w <= 0.f) continue;
+            float4 d = reinterpret_cast<const float4 *>(d_render)[b];
+            float dot = d.x * col.x + d.y * col.y + d.z * col.z + d.w * col.w;
+            float denom = ws * ws;
+            float d_w = denom > 0.f ? (dot * ws - w * dot * (ws - w)) / denom : 0.f;
+            if (d_w != 0.f) acc += d_fw_cpu(S.filter_type, S.filter_radius, xc - ptx, yc - pty, d_w);
+        }
+    return acc;
+}
+
+// Non-prefiltered fragment tests of one group (sampling.rs:95-118 with the
+// CPU queries): stroke hit, then fill inside.
+DVG_DEV void group_frags(const DevScene &S, uint32_t g, float px, float py, bool &stroke, bool &fill) {
+    const float *gd = S.group_data + g * 16u;
+    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+    stroke = fill = false;
+    if (fk == DVG_PAINT_NONE && sk == DVG_PAINT_NONE) return;
+    float lx, ly;
+    group_local(S, g, px, py, lx, ly);
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
+    group_cov<true>(S, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, false, c);
+    stroke = sk != DVG_PAINT_NONE && c.stroke_hit > 0.f;
+    fill = fk != DVG_PAINT_NONE && ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f);
+}
+
+DVG_DEV float4 frag_color(const DevScene &S, uint32_t g, bool stroke, float px, float py) {
+    const float *gd = S.group_data + g * 16u;
+    return stroke ? paint_cpu(S, (uint32_t)gd[4], (uint32_t)gd[5], gd + 12, px, py)
+                  : paint_cpu(S, (uint32_t)gd[2], (uint32_t)gd[3], gd + 8, px, py);
+}
+
+// CPU blend of sampling.rs:155-160 on premultiplied colours.
+DVG_DEV void blend_cpu(float4 &p, float4 c) {
+    float a = c.w;
+    p = make_float4(p.x * (1.0f - a) + c.x * a, p.y * (1.0f - a) + c.y * a, p.z * (1.0f - a) + c.z * a,
+                    p.w * (1.0f - a) + a);
+}
+
+// One pass over the tile's groups: counts fragments, composites them all into
+// `fin`, composites the first `start` into `pre`, and records the list
+// positions of fragments [start, start + FRAG_WINDOW) as (pos << 1 | stroke).
+DVG_DEV uint32_t walk_window(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t start,
+                             uint32_t *ids, float4 &pre, float4 &fin) {
+    uint32_t count = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        uint32_t g = list[e];
+        bool st, fi;
+        group_frags(S, g, px, py, st, fi);
+        for (int k = 0; k < 2; ++k) {
+            bool is_stroke = k == 0;
+            if (!(is_stroke ? st : fi)) continue;
+            float4 c = frag_color(S, g, is_stroke, px, py);
+            blend_cpu(fin, c);
+            if (count < start) blend_cpu(pre, c);
+            else if (count < start + FRAG_WINDOW) ids[count - start] = (e << 1) | (is_stroke ? 1u : 0u);
+            ++count;
+        }
+    }
+    return count;
+}
+
+// d_sample_paint for gradient paints (paint.rs:77-280), into global memory.
+DVG_DEV void d_gradient_paint(const DevScene &S, const DevGrads &G, uint32_t gidx, float px, float py, float4 d,
+                              float &dtx, float &dty) {
+    const float *gd = S.gradient_data + gidx * 8u;
+    uint32_t gtype = (uint32_t)gd[0];
+    float p0 = gd[1], p1 = gd[2], p2 = gd[3], p3 = gd[4];
+    uint32_t so = (uint32_t)gd[5], sc = (uint32_t)gd[6];
+    if (sc == 0) return;
+    const float *offs = S.stop_offsets + so;
+    const float *cols = S.stop_colors + so * 4u;
+    float t, denom = 0.f, offx = 0.f, offy = 0.f, nx = 0.f, ny = 0.f;
+    if (gtype == 0) {
+        float ex = p2 - p0, ey = p3 - p1;
+        denom = fmaxf(ex * ex + ey * ey, 1.0e-3f);
+        t = ((px - p0) * ex + (py - p1) * ey) / denom;
+    } else {
+        offx = px - p0;
+        offy = py - p1;
+        nx = offx / p2;
+        ny = offy / p3;
+        t = sqrtf(nx * nx + ny * ny);
+    }
+    float *dsc = G.d_stop_colors + so * 4u;
+    if (t < offs[0]) {
+        unsafeAtomicAdd(dsc + 0, d.x); unsafeAtomicAdd(dsc + 1, d.y);
+        unsafeAtomicAdd(dsc + 2, d.z); unsafeAtomicAdd(dsc + 3, d.w);
+        return;
+    }
+    for (uint32_t i = 0; i + 1 < sc; ++i) {
+        float cu = offs[i], nxo = offs[i + 1];
+        if (t >= cu && t < nxo) {
+            float tt = (t - cu) / (nxo - cu);
+            const float *c0 = cols + i * 4u, *c1 = c0 + 4;
+            float *d0 = dsc + i * 4u, *d1 = d0 + 4;
+            unsafeAtomicAdd(d0 + 0, d.x * (1.0f - tt)); unsafeAtomicAdd(d0 + 1, d.y * (1.0f - tt));
+            unsafeAtomicAdd(d0 + 2, d.z * (1.0f - tt)); unsafeAtomicAdd(d0 + 3, d.w * (1.0f - tt));
+            unsafeAtomicAdd(d1 + 0, d.x * tt); unsafeAtomicAdd(d1 + 1, d.y * tt);
+            unsafeAtomicAdd(d1 + 2, d.z * tt); unsafeAtomicAdd(d1 + 3, d.w * tt);
+            float d_tt = d.x * (c1[0] - c0[0]) + d.y * (c1[1] - c0[1]) + d.z * (c1[2] - c0[2]) + d.w * (c1[3] - c0[3]);
+            float dof = nxo - cu;
+            if (fabsf(dof) > 0.f) {
+                unsafeAtomicAdd(G.d_stop_offsets + so + i, d_tt * (tt - 1.0f) / dof);
+                unsafeAtomicAdd(G.d_stop_offsets + so + i + 1, -d_tt * tt / dof);
+            }
+            float d_t = d_tt / dof;
+            float *dg = G.d_gradient_data + gidx * 8u;
+            if (gtype == 0) {
+                float dbx = d_t * (-(px - p0) - (p2 - p0)) / denom, dby = d_t * (-(py - p1) - (p3 - p1)) / denom;
+                float dex = d_t * (px - p0) / denom, dey = d_t * (py - p1) / denom;
+                float d_l = -d_t * t / denom;
+                float l2 = (p2 - p0) * (p2 - p0) + (p3 - p1) * (p3 - p1);
+                if (l2 > 1.0e-3f) {
+                    unsafeAtomicAdd(dg + 1, dbx + 2.0f * d_l * (p0 - p2));
+                    unsafeAtomicAdd(dg + 2, dby + 2.0f * d_l * (p1 - p3));
+                    unsafeAtomicAdd(dg + 3, dex + 2.0f * d_l * (p2 - p0));
+                    unsafeAtomicAdd(dg + 4, dey + 2.0f * d_l * (p3 - p1));
+                } else {
+                    unsafeAtomicAdd(dg + 1, dbx); unsafeAtomicAdd(dg + 2, dby);
+                    unsafeAtomicAdd(dg + 3, dex); unsafeAtomicAdd(dg + 4, dey);
+                }
+                dtx += dbx + dex;
+                dty += dby + dey;
+            } else {
+                // d_length (backward/math.rs:33-41)
+                float l = sqrtf(nx * nx + ny * ny);
+                float dnx = 0.f, dny = 0.f;
+                if (l != 0.f) {
+                    float dls = 0.5f * d_t / l;
+                    dnx = nx * (2.0f * dls);
+                    dny = ny * (2.0f * dls);
+                }
+                float dox = dnx / p2, doy = dny / p3;
+                unsafeAtomicAdd(dg + 1, -dox);
+                unsafeAtomicAdd(dg + 2, -doy);
+                unsafeAtomicAdd(dg + 3, -dnx * offx / (p2 * p2));
+                unsafeAtomicAdd(dg + 4, -dny * offy / (p3 * p3));
+                dtx += -dox;
+                dty += -doy;
+            }
+            return;
+        }
+    }
+    float *dl = dsc + (sc - 1) * 4u;
+    unsafeAtomicAdd(dl + 0, d.x); unsafeAtomicAdd(dl + 1, d.y);
+    unsafeAtomicAdd(dl + 2, d.z); unsafeAtomicAdd(dl + 3, d.w);
+}
+
+DVG_DEV float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_backward_interior(DevScene S, const float *__restrict__ weight,
+                                                             const float *__restrict__ d_render, DevGrads G) {
+    __shared__ float s_paint[LDS_GROUP_CAP * 8];
+    __shared__ float s_red[4][6];
+    uint32_t tile = blockIdx.x;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    const uint32_t *list = S.tile_entries + lb;
+    uint32_t n = le - lb;
+    uint32_t nlds = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
+    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) s_paint[i] = 0.f;
+    __syncthreads();
+    float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
+    uint32_t total = TILE * TILE * S.spp;
+    float Wf = (float)S.W, Hf = (float)S.H;
+    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+        uint32_t x, y, sx, sy;
+        tile_sample(S, tile, s, x, y, sx, sy);
+        if (x >= S.W || y >= S.H) continue;
+        float px0, py0;
+        sample_point<true>(S, x, y, sx, sy, px0, py0);
+        // render.rs:110-111 then sampling.rs:89: the point goes through the
+        // normalized [0, 1] canvas and back.
+        float px = (px0 / Wf) * Wf, py = (py0 / Hf) * Hf;
+        size_t pix = (size_t)y * S.W + x;
+        float4 bgs = background_at(S, x, y);
+        float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
+        float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
+        uint32_t ids[FRAG_WINDOW];
+        float4 pre = bg, fin = bg;
+        uint32_t nf = walk_window(S, list, n, px, py, 0, ids, pre, fin);
+        float4 col;
+        float dtx = 0.f, dty = 0.f;
+        if (nf == 0) {
+            col = bg;
+            acc_bg[0] += dcol.x; acc_bg[1] += dcol.y; acc_bg[2] += dcol.z; acc_bg[3] += dcol.w;
+            if (S.bg_image && G.d_background_image) {
+                float *db = G.d_background_image + pix * 4u;
+                unsafeAtomicAdd(db + 0, dcol.x); unsafeAtomicAdd(db + 1, dcol.y);
+                unsafeAtomicAdd(db + 2, dcol.z); unsafeAtomicAdd(db + 3, dcol.w);
+            }
+        } else {
+            float fa = fin.w;
+            float fr = fin.x, fg = fin.y, fb = fin.z;
+            if (fa > 1.0e-6f) {
+                float sc = 1.0f / fa;
+                fr = fr * sc; fg = fg * sc; fb = fb * sc;
+            }
+            col = make_float4(fr, fg, fb, fa);
+            float dr = dcol.x, dg = dcol.y, db = dcol.z, da = dcol.w;
+            if (fa > 1.0e-6f) {
+                float sc = 1.0f / fa;
+                float dot = dr * fr + dg * fg + db * fb;
+                dr = dr * sc; dg = dg * sc; db = db * sc;
+                da -= dot / fa;
+            }
+            uint32_t nwin = (nf + FRAG_WINDOW - 1) / FRAG_WINDOW;
+            for (int w = (int)nwin - 1; w >= 0; --w) {
+                uint32_t wstart = (uint32_t)w * FRAG_WINDOW;
+                float4 wpre = bg;
+                if (nwin > 1) {
+                    float4 f2 = bg;
+                    wpre = bg;
+                    walk_window(S, list, n, px, py, wstart, ids, wpre, f2);
+                }
+                uint32_t m = min((uint32_t)FRAG_WINDOW, nf - wstart);
+                for (int i = (int)m - 1; i >= 0; --i) {
+                    // prefix colour before fragment i of this window
+                    float4 pc = wpre;
+                    for (int j = 0; j < i; ++j) {
+                        uint32_t id = ids[j];
+                        blend_cpu(pc, frag_color(S, list[id >> 1], (id & 1u) != 0, px, py));
+                    }
+                    uint32_t id = ids[i];
+                    uint32_t e = id >> 1;
+                    bool is_stroke = (id & 1u) != 0;
+                    uint32_t g = list[e];
+                    float4 c = frag_color(S, g, is_stroke, px, py);
+                    float a = c.w;
+                    float d_prev_a = da * (1.0f - a);
+                    float d_alpha = da * (1.0f - pc.w) + (dr * (c.x - pc.x) + dg * (c.y - pc.y) + db * (c.z - pc.z));
+                    float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha);
+                    const float *gd = S.group_data + g * 16u;
+                    uint32_t kind = (uint32_t)gd[is_stroke ? 4 : 2];
+                    if (kind == DVG_PAINT_SOLID) {
+                        if (e < LDS_GROUP_CAP) {
+                            float *sp = s_paint + e * 8u + (is_stroke ? 4u : 0u);
+                            atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
+                            atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
+                        } else {
+                            float *dgp = G.d_group_data + g * 16u + (is_stroke ? 12u : 8u);
+                            unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
+                            unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
+                        }
+                    } else {
+                        d_gradient_paint(S, G, (uint32_t)gd[is_stroke ? 5 : 3], px, py, dv, dtx, dty);
+                    }
+                    dr = dr * (1.0f - a); dg = dg * (1.0f - a); db = db * (1.0f - a); da = d_prev_a;
+                }
+            }
+            acc_bg[0] += dr; acc_bg[1] += dg; acc_bg[2] += db; acc_bg[3] += da;
+            if (S.bg_image && G.d_background_image) {
+                float *dbp = G.d_background_image + pix * 4u;
+                unsafeAtomicAdd(dbp + 0, dr); unsafeAtomicAdd(dbp + 1, dg);
+                unsafeAtomicAdd(dbp + 2, db); unsafeAtomicAdd(dbp + 3, da);
+            }
+        }
+        if (G.d_translation && (dtx != 0.f || dty != 0.f)) {
+            unsafeAtomicAdd(G.d_translation + pix * 2u, dtx);
+            unsafeAtomicAdd(G.d_translation + pix * 2u + 1, dty);
+        }
+        acc_fr += filter_grad(S, weight, d_render, col, px0, py0);
+    }
+    // scalar gradients: wave reduction, then one atomic per workgroup
+    float v[5] = {acc_bg[0], acc_bg[1], acc_bg[2], acc_bg[3], acc_fr};
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 5; ++k) {
+        float r = wave_sum(v[k]);
+        if (lane == 0) s_red[wave][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        float r = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
+        if (threadIdx.x < 4) {
+            if (!S.bg_image) unsafeAtomicAdd(G.d_background + threadIdx.x, r);
+        } else {
+            unsafeAtomicAdd(G.d_filter_radius, r);
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) {
+        float v2 = s_paint[i];
+        if (v2 != 0.f) {
+            uint32_t e = i >> 3, c = i & 7u;
+            uint32_t g = list[e];
+            unsafeAtomicAdd(G.d_group_data + g * 16u + 8u + c, v2);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary sampling (backward/boundary.rs:113-1148).
+
+// sample_cdf (boundary.rs:454-469): first entry with u <= cdf[i], by binary
+// search on the non-decreasing table.
+DVG_DEV uint32_t cdf_search(const float *cdf, uint32_t n, float u, float &t_out) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u > cdf[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t idx = lo >= n ? n - 1 : lo;
+    float prev = idx == 0 ? 0.f : cdf[idx - 1];
+    float denom = fmaxf(cdf[idx] - prev, 1.0e-6f);
+    float t = (u - prev) / denom;
+    t_out = t < 0.f ? 0.f : (t > 1.f ? 1.f : t);
+    return idx;
+}
+
+DVG_DEV float2 normalize2(float x, float y) {
+    float len = sqrtf(x * x + y * y);
+    if (len == 0.f) return make_float2(0.f, 0.f);
+    return make_float2(x / len, y / len);
+}
+
+struct BData {
+    uint32_t base_point_id, point_id;
+    float t;
+    bool is_stroke;
+};
+
+DVG_DEV bool path_pt(const DevScene &S, uint32_t si, uint32_t idx, float default_r, float &x, float &y, float &r) {
+    uint32_t np = S.sp_cnt[si];
+    if (np == 0) return false;
+    bool closed = S.s_closed[si] != 0;
+    uint32_t i;
+    if (closed) i = idx % np;
+    else {
+        if (idx >= np) return false;
+        i = idx;
+    }
+    const float *p = S.path_points + (size_t)(S.sp_off[si] + i) * 2u;
+    x = p[0];
+    y = p[1];
+    uint32_t nt = S.st_cnt[si];
+    r = (nt > 0 && i < nt) ? S.path_thickness[S.st_off[si] + i] * 1.0f : default_r;
+    return true;
+}
+
+// sample_boundary_path (boundary.rs:586-843)
+DVG_DEV bool sample_boundary_path(const DevScene &S, uint32_t si, float path_length, float t, float dir, float sw,
+                                  float &ox, float &oy, float &nx, float &ny, float &pdf, BData &bd) {
+    const float PI = 3.14159274101257324219f;
+    uint32_t np = S.sp_cnt[si], nb = S.sc_cnt[si];
+    if (np == 0 || nb == 0) return false;
+    bool closed = S.s_closed[si] != 0;
+    uint32_t nt = S.st_cnt[si];
+    const float *th = S.path_thickness + S.st_off[si];
+    const float *pts = S.path_points + (size_t)S.sp_off[si] * 2u;
+    const uint32_t *ctrl = S.path_ctrl + S.sc_off[si];
+    if (dir != 0.f && !closed) {
+        float cap_length = 0.f;
+        if (nt > 0) cap_length = PI * (th[0] + th[nt - 1]);
+        else cap_length = 2.0f * PI * sw;
+        float denom = cap_length + path_length;
+        if (denom > 0.f) {
+            float cap_prob = cap_length / denom;
+            if (t < cap_prob) {
+                t /= cap_prob;
+                pdf *= cap_prob;
+                float r0 = sw, r1 = sw;
+                if (nt > 0) { r0 = th[0]; r1 = th[nt - 1]; }
+                float angle = 2.0f * PI * t;
+                if (dir < 0.f) {
+                    float offx = r0 * cosf(angle), offy = r0 * sinf(angle);
+                    float2 nn = normalize2(offx, offy);
+                    nx = nn.x; ny = nn.y;
+                    pdf /= 2.0f * PI * r0;
+                    bd.base_point_id = 0; bd.point_id = 0; bd.t = 0.f;
+                    ox = pts[0] + offx; oy = pts[1] + offy;
+                    return true;
+                }
+                float offx = r1 * cosf(angle), offy = r1 * sinf(angle);
+                float2 nn = normalize2(offx, offy);
+                nx = nn.x; ny = nn.y;
+                pdf /= 2.0f * PI * r1;
+                bd.base_point_id = nb - 1;
+                uint32_t sub = 2u + ctrl[nb - 1];
+                bd.point_id = np >= sub ? np - sub : 0u;
+                bd.t = 1.f;
+                ox = pts[(np - 1) * 2] + offx; oy = pts[(np - 1) * 2 + 1] + offy;
+                return true;
+            }
+            t = (t - cap_prob) / (1.0f - cap_prob);
+            pdf *= 1.0f - cap_prob;
+        }
+    }
+    uint32_t ccnt = S.path_cdf_cnt[si];
+    if (ccnt == 0) return false;
+    float lt;
+    uint32_t sid = cdf_search(S.path_cdf + S.path_cdf_off[si], ccnt, t, lt);
+    uint32_t pid = S.path_point_ids[S.path_point_off[si] + sid];
+    bd.base_point_id = sid;
+    bd.point_id = pid;
+    bd.t = lt;
+    if (lt < -1.0e-3f || lt > 1.0f + 1.0e-3f) return false;
+    uint32_t nc = sid < nb ? ctrl[sid] : 0u;
+    float pmf = S.path_pmf[S.path_cdf_off[si] + sid];
+    auto next_index = [&](uint32_t idx) -> uint32_t { return closed ? idx % np : (idx < np - 1 ? idx : np - 1); };
+    float x0, y0, r0, x1, y1, r1, x2, y2, r2, x3, y3, r3;
+    float ex, ey, tx, ty, rr;
+    float tt = 1.0f - lt;
+    if (nc == 0) {
+        if (!path_pt(S, si, pid, sw, x0, y0, r0) || !path_pt(S, si, next_index(pid + 1), sw, x1, y1, r1)) return false;
+        tx = x1 - x0; ty = y1 - y0;
+        ex = x0 + tx * lt; ey = y0 + ty * lt;
+        rr = r0 + lt * (r1 - r0);
+    } else if (nc == 1) {
+        if (!path_pt(S, si, pid, sw, x0, y0, r0) || !path_pt(S, si, pid + 1, sw, x1, y1, r1) ||
+            !path_pt(S, si, next_index(pid + 2), sw, x2, y2, r2)) return false;
+        ex = ((x0 * (tt * tt)) + (x1 * (2.0f * tt * lt))) + (x2 * (lt * lt));
+        ey = ((y0 * (tt * tt)) + (y1 * (2.0f * tt * lt))) + (y2 * (lt * lt));
+        tx = ((x1 - x0) * (2.0f * tt)) + ((x2 - x1) * (2.0f * lt));
+        ty = ((y1 - y0) * (2.0f * tt)) + ((y2 - y1) * (2.0f * lt));
+        rr = ((r0 * (tt * tt)) + (r1 * (2.0f * tt * lt))) + (r2 * (lt * lt));
+    } else if (nc == 2) {
+        if (!path_pt(S, si, pid, sw, x0, y0, r0) || !path_pt(S, si, pid + 1, sw, x1, y1, r1) ||
+            !path_pt(S, si, pid + 2, sw, x2, y2, r2) || !path_pt(S, si, next_index(pid + 3), sw, x3, y3, r3))
+            return false;
+        ex = (((x0 * (tt * tt * tt)) + (x1 * (3.0f * tt * tt * lt))) + (x2 * (3.0f * tt * lt * lt))) + (x3 * (lt * lt * lt));
+        ey = (((y0 * (tt * tt * tt)) + (y1 * (3.0f * tt * tt * lt))) + (y2 * (3.0f * tt * lt * lt))) + (y3 * (lt * lt * lt));
+        tx = (((x1 - x0) * (3.0f * tt * tt)) + ((x2 - x1) * (6.0f * tt * lt))) + ((x3 - x2) * (3.0f * lt * lt));
+        ty = (((y1 - y0) * (3.0f * tt * tt)) + ((y2 - y1) * (6.0f * tt * lt))) + ((y3 - y2) * (3.0f * lt * lt));
+        rr = (((r0 * (tt * tt * tt)) + (r1 * (3.0f * tt * tt * lt))) + (r2 * (3.0f * tt * lt * lt))) + (r3 * (lt * lt * lt));
+    } else {
+        return false;
+    }
+    float tl = sqrtf(tx * tx + ty * ty);
+    if (tl == 0.f) return false;
+    nx = -ty / tl;
+    ny = tx / tl;
+    pdf *= pmf / tl;
+    ox = ex;
+    oy = ey;
+    if (dir != 0.f) {
+        ox += nx * (dir * rr);
+        oy += ny * (dir * rr);
+        if (dir < 0.f) { nx = nx * -1.0f; ny = ny * -1.0f; }
+    }
+    return true;
+}
+
+// Colour of one boundary query point with the EdgeQuery hit tracking of
+// sampling.rs:76-224 (no adjoint).  (qx, qy) is the normalized point.
+DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint32_t eq_group, uint32_t eq_prep,
+                          bool &hit) {
+    float px = qx * (float)S.W, py = qy * (float)S.H;
+    int tx = (int)floorf(px / (float)TILE), ty = (int)floorf(py / (float)TILE);
+    tx = tx < 0 ? 0 : (tx >= (int)S.tiles_x ? (int)S.tiles_x - 1 : tx);
+    ty = ty < 0 ? 0 : (ty >= (int)S.tiles_y ? (int)S.tiles_y - 1 : ty);
+    uint32_t tile = (uint32_t)ty * S.tiles_x + (uint32_t)tx;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    float4 acc = bg;
+    uint32_t nfr = 0;
+    bool edge_test = false;  // hit set while collecting fragments
+    bool h0 = false, h1 = true;  // compositing-loop hit for an initial false / true
+    for (uint32_t e = lb; e < le; ++e) {
+        uint32_t g = S.tile_entries[e];
+        bool st, fi;
+        group_frags(S, g, px, py, st, fi);
+        if (g == eq_group) {
+            // within_distance_edge / is_inside_edge (sampling.rs:485-549) on
+            // the sampled shape alone
+            const float *gd = S.group_data + g * 16u;
+            uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+            float lx, ly;
+            group_local(S, g, px, py, lx, ly);
+            const float *xf = S.shape_xform + eq_prep * 6u;
+            float spx = xf[0] * lx + xf[1] * ly + xf[2];
+            float spy = xf[3] * lx + xf[4] * ly + xf[5];
+            const float *sd = S.shape_data + eq_prep * 15u;
+            Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
+            if (sk != DVG_PAINT_NONE) shape_stroke<true>(S, sd, false, spx, spy, c);
+            if (fk != DVG_PAINT_NONE) shape_fill(S, sd, rule, false, spx, spy, c);
+            if (c.stroke_hit > 0.f) edge_test = true;
+            if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
+                if (fk != DVG_PAINT_NONE) edge_test = true;
+            }
+        }
+        for (int k = 0; k < 2; ++k) {
+            bool is_stroke = k == 0;
+            if (!(is_stroke ? st : fi)) continue;
+            float4 c = frag_color(S, g, is_stroke, px, py);
+            if (c.w >= 1.0f) { h0 = false; h1 = false; }
+            if (g == eq_group) { h0 = true; h1 = true; }
+            blend_cpu(acc, c);
+            ++nfr;
+        }
+    }
+    hit = edge_test ? h1 : h0;
+    if (nfr == 0) return bg;
+    float fa = acc.w;
+    if (fa > 1.0e-6f) {
+        float sc = 1.0f / fa;
+        return make_float4(acc.x * sc, acc.y * sc, acc.z * sc, fa);
+    }
+    return acc;
+}
+
+// 3x3 helpers on affine matrices given by their first two rows.
+struct M3 {
+    float m[3][3];
+};
+DVG_DEV M3 m3_affine(const float *r) {
+    M3 o;
+    o.m[0][0] = r[0]; o.m[0][1] = r[1]; o.m[0][2] = r[2];
+    o.m[1][0] = r[3]; o.m[1][1] = r[4]; o.m[1][2] = r[5];
+    o.m[2][0] = 0.f; o.m[2][1] = 0.f; o.m[2][2] = 1.f;
+    return o;
+}
+DVG_DEV M3 m3_mul(const M3 &a, const M3 &b) {  // math.rs:185-195
+    M3 o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+    return o;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__restrict__ weight,
+                                                    const float *__restrict__ d_render, DevGrads G) {
+    const float PI = 3.14159274101257324219f;
+    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t total = (size_t)S.W * S.H * S.spp;
+    if (idx >= total || S.n_cdf == 0) return;
+    Pcg32 rng((uint64_t)idx, (uint64_t)S.seed);
+    float u = rng.next_f32();
+    float t = rng.next_f32();
+    float unused;
+    uint32_t sid = cdf_search(S.shape_cdf, S.n_cdf, u, unused);
+    uint32_t si = S.shape_ids[sid], gi = S.group_ids[sid];
+    float spmf = S.shape_pmf[sid];
+    if (spmf <= 0.f) return;
+    uint32_t prep = sid;  // prepared instances follow the CDF order
+    float slen = S.shape_lengths[si];
+    const float *gd = S.group_data + gi * 16u;
+    bool has_fill = (uint32_t)gd[2] != DVG_PAINT_NONE, has_stroke = (uint32_t)gd[4] != DVG_PAINT_NONE;
+    // sample_boundary_point (boundary.rs:472-583)
+    if (!has_fill && !has_stroke) return;
+    float pdf = 1.0f, lt = t;
+    bool stroke = false;
+    if (has_fill && has_stroke) {
+        if (lt < 0.5f) { lt *= 2.0f; pdf = 0.5f; }
+        else { stroke = true; lt = 2.0f * (lt - 0.5f); pdf = 0.5f; }
+    } else if (has_stroke) {
+        stroke = true;
+    }
+    float dir = 0.f;
+    if (stroke) {
+        if (lt < 0.5f) { dir = -1.0f; lt *= 2.0f; pdf *= 0.5f; }
+        else { dir = 1.0f; lt = 2.0f * (lt - 0.5f); pdf *= 0.5f; }
+    }
+    BData bd = {0u, 0u, 0.f, stroke};
+    const float *sd = S.shape_data + prep * 15u;
+    uint32_t kind = (uint32_t)sd[0];
+    float sw = sd[3];
+    float lx = 0.f, ly = 0.f, nx = 0.f, ny = 0.f;
+    if (kind == DVG_SHAPE_CIRCLE) {
+        float r = sd[6];
+        if (r <= 0.f) return;
+        float angle = 2.0f * PI * lt;
+        float offx = r * cosf(angle), offy = r * sinf(angle);
+        float2 nn = normalize2(offx, offy);
+        nx = nn.x; ny = nn.y;
+        pdf /= 2.0f * PI * r;
+        lx = sd[4] + offx; ly = sd[5] + offy;
+        if (dir != 0.f) {
+            lx += nx * (dir * sw); ly += ny * (dir * sw);
+            if (dir < 0.f) { nx = -nx; ny = -ny; }
+        }
+    } else if (kind == DVG_SHAPE_ELLIPSE) {
+        float rx = sd[6], ry = sd[7];
+        if (rx <= 0.f || ry <= 0.f) return;
+        float angle = 2.0f * PI * lt;
+        float s_ = sinf(angle), c_ = cosf(angle);
+        float offx = rx * c_, offy = ry * s_;
+        float dxdt = -rx * s_ * 2.0f * PI, dydt = ry * c_ * 2.0f * PI;
+        float2 nn = normalize2(dydt, -dxdt);
+        nx = nn.x; ny = nn.y;
+        pdf /= sqrtf(dxdt * dxdt + dydt * dydt);
+        lx = sd[4] + offx; ly = sd[5] + offy;
+        if (dir != 0.f) {
+            lx += nx * (dir * sw); ly += ny * (dir * sw);
+            if (dir < 0.f) { nx = -nx; ny = -ny; }
+        }
+    } else if (kind == DVG_SHAPE_RECT) {
+        float mnx = sd[4], mny = sd[5], mxx = sd[6], mxy = sd[7];
+        float w = mxx - mnx, h = mxy - mny;
+        if (w == 0.f && h == 0.f) return;
+        pdf /= 2.0f * (w + h);
+        if (lt <= w / (w + h)) {
+            lt *= (w + h) / w;
+            if (lt < 0.5f) { nx = 0.f; ny = -1.f; lx = mnx + 2.0f * lt * (mxx - mnx); ly = mny + 0.f; }
+            else { nx = 0.f; ny = 1.f; lx = mnx + 2.0f * (lt - 0.5f) * (mxx - mnx); ly = mxy + 0.f; }
+        } else {
+            lt = (lt - w / (w + h)) * (w + h) / h;
+            if (lt < 0.5f) { nx = -1.f; ny = 0.f; lx = mnx + 0.f; ly = mny + 2.0f * lt * (mxy - mny); }
+            else { nx = 1.f; ny = 0.f; lx = mxx + 0.f; ly = mny + 2.0f * (lt - 0.5f) * (mxy - mny); }
+        }
+        if (dir != 0.f) {
+            lx += nx * (dir * sw); ly += ny * (dir * sw);
+            if (dir < 0.f) { nx = nx * -1.0f; ny = ny * -1.0f; }
+        }
+    } else {
+        if (S.path_cdf_cnt[si] == 0 || !(slen > 0.f)) return;
+        if (!sample_boundary_path(S, si, slen, lt, dir, sw, lx, ly, nx, ny, pdf, bd)) return;
+    }
+    if (!isfinite(pdf) || pdf <= 0.f) return;
+    pdf = spmf * pdf;
+    // boundary.rs:168-174: local -> canvas, normal by the inverse transpose
+    M3 Gs = m3_affine(S.group_shape_xform + gi * 6u);
+    M3 T = m3_affine(S.shape_transform + si * 6u);
+    M3 s2c = m3_mul(Gs, T);
+    float bx = s2c.m[0][0] * lx + s2c.m[0][1] * ly + s2c.m[0][2];
+    float by = s2c.m[1][0] * lx + s2c.m[1][1] * ly + s2c.m[1][2];
+    M3 Si = m3_affine(S.shape_xform + prep * 6u);
+    M3 Gi = m3_affine(S.group_xform + gi * 6u);
+    M3 ci = m3_mul(Si, Gi);
+    float2 nc = normalize2(ci.m[0][0] * nx + ci.m[1][0] * ny, ci.m[0][1] * nx + ci.m[1][1] * ny);
+    float Wf = (float)S.W, Hf = (float)S.H;
+    float qx = bx / Wf, qy = by / Hf;
+    // render_edge_sample (boundary.rs:845-967)
+    int pxi = (int)(qx * Wf), pyi = (int)(qy * Hf);
+    if (pxi < 0 || pxi >= (int)S.W || pyi < 0 || pyi >= (int)S.H) return;
+    size_t pix = (size_t)pyi * S.W + (size_t)pxi;
+    float4 bgs = background_at(S, (uint32_t)pxi, (uint32_t)pyi);
+    float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
+    bool ih, oh;
+    float4 cin = edge_color(S, qx - nc.x * 1.0e-4f, qy - nc.y * 1.0e-4f, bg, gi, prep, ih);
+    float4 cout = edge_color(S, qx + nc.x * 1.0e-4f, qy + nc.y * 1.0e-4f, bg, gi, prep, oh);
+    if (!ih && !oh) return;
+    float nnx = nc.x, nny = nc.y;
+    if (!ih) {
+        nnx = nnx * -1.0f; nny = nny * -1.0f;
+        float4 tmp = cin; cin = cout; cout = tmp;
+    }
+    float4 dc = gather_d_color(S, weight, d_render, qx * Wf, qy * Hf);
+    float norm = Wf * Hf;
+    if (norm > 0.f) { dc.x /= norm; dc.y /= norm; dc.z /= norm; dc.w /= norm; }
+    float contrib = ((cin.x - cout.x) * dc.x + (cin.y - cout.y) * dc.y + (cin.z - cout.z) * dc.z +
+                     (cin.w - cout.w) * dc.w) / pdf;
+    if (!isfinite(contrib)) return;
+    // accumulate_boundary_gradient (boundary.rs:970-1148)
+    if (bd.is_stroke) {
+        uint32_t nt = S.st_cnt[si];
+        if (kind == DVG_SHAPE_PATH && nt > 0) {
+            float *dth = G.d_shape_thickness + S.st_off[si];
+            uint32_t np = S.sp_cnt[si];
+            bool closed = S.s_closed[si] != 0;
+            uint32_t nc_ = bd.base_point_id < S.sc_cnt[si] ? S.path_ctrl[S.sc_off[si] + bd.base_point_id] : 0u;
+            uint32_t pid = bd.point_id;
+            float tv = bd.t, tt = 1.0f - tv;
+            auto add = [&](uint32_t i, float v) { if (i < nt) unsafeAtomicAdd(dth + i, v); };
+            if (nc_ == 0) {
+                add(pid, (1.0f - tv) * contrib);
+                add(closed ? (pid + 1) % np : pid + 1, tv * contrib);
+            } else if (nc_ == 1) {
+                add(pid, tt * tt * contrib);
+                add(pid + 1, 2.0f * tt * tv * contrib);
+                add(closed ? (pid + 2) % np : pid + 2, tv * tv * contrib);
+            } else if (nc_ == 2) {
+                add(pid, tt * tt * tt * contrib);
+                add(pid + 1, 3.0f * tt * tt * tv * contrib);
+                add(pid + 2, 3.0f * tt * tv * tv * contrib);
+                add(closed ? (pid + 3) % np : pid + 3, tv * tv * tv * contrib);
+            }
+        } else {
+            unsafeAtomicAdd(G.d_shape_stroke_width + si, contrib);
+        }
+    }
+    float *dp = G.d_shape_params + si * 8u;
+    if (kind == DVG_SHAPE_CIRCLE) {
+        unsafeAtomicAdd(dp + 0, nnx * contrib);
+        unsafeAtomicAdd(dp + 1, nny * contrib);
+        unsafeAtomicAdd(dp + 2, contrib);
+    } else if (kind == DVG_SHAPE_ELLIPSE) {
+        unsafeAtomicAdd(dp + 0, nnx * contrib);
+        unsafeAtomicAdd(dp + 1, nny * contrib);
+        float angle = 2.0f * PI * t;
+        unsafeAtomicAdd(dp + 2, cosf(angle) * nnx * contrib);
+        unsafeAtomicAdd(dp + 3, sinf(angle) * nny * contrib);
+    } else if (kind == DVG_SHAPE_PATH) {
+        uint32_t np = S.sp_cnt[si];
+        bool closed = S.s_closed[si] != 0;
+        uint32_t nc_ = bd.base_point_id < S.sc_cnt[si] ? S.path_ctrl[S.sc_off[si] + bd.base_point_id] : 0u;
+        uint32_t pid = bd.point_id;
+        float tv = bd.t, tt = 1.0f - tv;
+        float *dpt = G.d_shape_points + (size_t)S.sp_off[si] * 2u;
+        auto add = [&](uint32_t i, float w) {
+            uint32_t index = closed ? i % np : i;
+            if (index < np) {
+                unsafeAtomicAdd(dpt + index * 2u, nnx * (w * contrib));
+                unsafeAtomicAdd(dpt + index * 2u + 1, nny * (w * contrib));
+            }
+        };
+        if (nc_ == 0) {
+            add(pid, 1.0f - tv);
+            add(pid + 1, tv);
+        } else if (nc_ == 1) {
+            add(pid, tt * tt);
+            add(pid + 1, 2.0f * tt * tv);
+            add(pid + 2, tv * tv);
+        } else if (nc_ == 2) {
+            add(pid, tt * tt * tt);
+            add(pid + 1, 3.0f * tt * tt * tv);
+            add(pid + 2, 3.0f * tt * tv * tv);
+            add(pid + 3, tv * tv * tv);
+        }
+    } else if (kind == DVG_SHAPE_RECT) {
+        if (nnx == -1.f && nny == 0.f) unsafeAtomicAdd(dp + 0, -contrib);
+        else if (nnx == 1.f && nny == 0.f) unsafeAtomicAdd(dp + 2, contrib);
+        else if (nnx == 0.f && nny == -1.f) unsafeAtomicAdd(dp + 1, -contrib);
+        else if (nnx == 0.f && nny == 1.f) unsafeAtomicAdd(dp + 3, contrib);
+    }
+    // d_xform_pt (backward/math.rs:101-120) on shape_to_canvas = G * T, then
+    // d_G += dM * T^T and d_T += G^T * dM.
+    float t0 = s2c.m[0][0] * lx + s2c.m[0][1] * ly + s2c.m[0][2];
+    float t1 = s2c.m[1][0] * lx + s2c.m[1][1] * ly + s2c.m[1][2];
+    float t2 = s2c.m[2][0] * lx + s2c.m[2][1] * ly + s2c.m[2][2];
+    float dox = nnx * contrib, doy = nny * contrib;
+    float ox = t0 / t2, oy = t1 / t2;
+    float dt0 = dox / t2, dt1 = doy / t2, dt2 = -(dox * ox + doy * oy) / t2;
+    M3 dm;
+    dm.m[0][0] = dt0 * lx; dm.m[0][1] = dt0 * ly; dm.m[0][2] = dt0;
+    dm.m[1][0] = dt1 * lx; dm.m[1][1] = dt1 * ly; dm.m[1][2] = dt1;
+    dm.m[2][0] = dt2 * lx; dm.m[2][1] = dt2 * ly; dm.m[2][2] = dt2;
+    float *dgt = G.d_group_transform + gi * 9u;
+    float *dst = G.d_shape_transform + si * 9u;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float a = dm.m[i][0] * T.m[j][0] + dm.m[i][1] * T.m[j][1] + dm.m[i][2] * T.m[j][2];
+            float b = Gs.m[0][i] * dm.m[0][j] + Gs.m[1][i] * dm.m[1][j] + Gs.m[2][i] * dm.m[2][j];
+            unsafeAtomicAdd(dgt + i * 3 + j, a);
+            unsafeAtomicAdd(dst + i * 3 + j, b);
+        }
+    if (G.d_translation) {
+        unsafeAtomicAdd(G.d_translation + pix * 2u, nnx * contrib);
+        unsafeAtomicAdd(G.d_translation + pix * 2u + 1, nny * contrib);
+    }
+}
+
+// Zero up to 14 float arrays in one launch (used instead of memset nodes so
+// that the calls stay capturable in a HIP graph).
+struct ZeroList {
+    float *p[14];
+    uint32_t n[14];
+};
+__global__ void k_zero(ZeroList z) {
+    uint32_t k = blockIdx.y;
+    float *p = z.p[k];
+    if (!p) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < z.n[k]; i += gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream) {
+    ZeroList z;
+    uint32_t mx = 0;
+    for (int k = 0; k < 14; ++k) {
+        z.p[k] = k < count ? ptrs[k] : nullptr;
+        z.n[k] = k < count ? lens[k] : 0u;
+        if (z.p[k] && z.n[k] > mx) mx = z.n[k];
+    }
+    if (mx == 0) return;
+    uint32_t bx = cdiv(mx, 256);
+    if (bx > 1024) bx = 1024;
+    hipLaunchKernelGGL(k_zero, dim3(bx, 14), dim3(256), 0, stream, z);
+}
+
+void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
+                      hipStream_t stream) {
+    uint32_t nt = tiles_x * tiles_y;
+    hipLaunchKernelGGL(k_bin_count, dim3(cdiv(nt, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const int4 *>(group_aabb), G, tiles_x, tiles_y, counts);
+}
+void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, counts, n, offsets);
+}
+void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
+                      const uint32_t *offsets, uint32_t *entries, hipStream_t stream) {
+    uint32_t nt = tiles_x * tiles_y;
+    hipLaunchKernelGGL(k_bin_write, dim3(cdiv(nt, 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const int4 *>(group_aabb), G, tiles_x, tiles_y, offsets, entries);
+}
+void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream) {
+    if (cpu) hipLaunchKernelGGL(k_weights<true>, dim3(cdiv(S.W, 16), cdiv(S.H, 16)), dim3(256), 0, stream, S, weight);
+    else hipLaunchKernelGGL(k_weights<false>, dim3(cdiv(S.W, 16), cdiv(S.H, 16)), dim3(256), 0, stream, S, weight);
+}
+void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream) {
+    hipLaunchKernelGGL(k_forward, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight, accum);
+}
+void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)S.W * S.H, 256)), dim3(256), 0, stream, S, weight, accum, out);
+}
+void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                              hipStream_t stream) {
+    hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
+                       d_render, G);
+}
+void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                     hipStream_t stream) {
+    uint64_t total = (uint64_t)S.W * S.H * S.spp;
+    hipLaunchKernelGGL(k_boundary, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G);
+}
+
+}  // namespace dvgk

@@ -1,0 +1,63 @@
+// Oracle (test infrastructure only): restatement of the reference CPU backward
+// (src/backward/*.rs over src/distance/*.rs).
+#pragma once
+#include "oracle_common.h"
+
+namespace oracle {
+
+// Flat gradient arrays in the layout of src/renderer/backward_gpu.rs:599-801
+// (scene shape / group order).
+struct Grads {
+    std::vector<float> d_shape_params, d_shape_points, d_shape_thickness, d_shape_stroke_width, d_shape_transform,
+        d_group_transform, d_group_data, d_gradient_data, d_stop_offsets, d_stop_colors, d_background_image,
+        d_translation;
+    float d_filter_radius[1] = {0.f};
+    float d_background[4] = {0.f, 0.f, 0.f, 0.f};
+};
+
+struct Ctx {
+    const Scene *sc = nullptr;
+    std::vector<std::vector<Seg>> segs;  // per scene shape, default radius = stroke_width
+    std::vector<float> group_bounds;     // 4 per group: union of CPU shape bounds
+    std::vector<uint8_t> group_has_bounds;
+};
+
+namespace cpu {
+bool solve_quadratic(double a, double b, double c, double t[2]);
+int solve_cubic(double a, double b, double c, double d, double t[3]);
+V2 transform_point_inverse(const M3 &m, V2 pt);
+void distance_to_segment(V2 pt, V2 a, V2 b, float &dist, V2 &cp, float &t);
+void closest_point_quadratic(V2 pt, V2 p0, V2 p1, V2 p2, bool approx, V2 &cp, float &t, float &dist);
+void closest_point_cubic(V2 pt, V2 p0, V2 p1, V2 p2, V2 p3, bool approx, V2 &cp, float &t, float &dist);
+void closest_on_seg(const Seg &s, V2 pt, bool approx, V2 &cp, float &t, float &dist);
+V2 closest_point_circle(V2 c, float r, V2 pt);
+V2 closest_point_ellipse(V2 c, V2 radius, V2 pt);
+V2 closest_point_rect(V2 mn, V2 mx, V2 pt);
+float rect_signed_distance(V2 pt, V2 mn, V2 mx);
+float ellipse_signed_distance(V2 pt, V2 c, V2 radius);
+int winding_number_segment(uint32_t kind, V2 p0, V2 p1, V2 p2, V2 p3, V2 pt);
+int winding_number_shape(const Ctx &cx, uint32_t si, V2 pt);
+bool within_distance_shape(const Ctx &cx, uint32_t si, V2 pt, float r);
+bool within_distance_group(const Ctx &cx, uint32_t gi, V2 pt);
+bool is_inside_group(const Ctx &cx, uint32_t gi, V2 pt);
+V4 paint_color(const Paint &p, V2 pt);
+void d_sample_paint(const Paint &p, Grads &gr, float *d_paint_solid, V2 pt, V4 d, V2 &d_trans);
+float compute_filter_weight(uint32_t type, float radius, float dx, float dy);
+V4 sample_background(const Scene &sc, size_t pix);
+void build_group_bounds(Ctx &cx);
+void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, const float *d_render,
+                       const std::vector<float> &wimg, Grads &gr, bool compute_translation, size_t i0, size_t i1);
+
+}  // namespace cpu
+
+// render_backward (backward/render.rs:35-210) with d_render present.
+// Gradients are finalized (background.rs:29-61).
+int render_backward(const Scene &sc, const dvg_render_options &opt, bool compute_translation, const float *d_render,
+                    Grads &gr, const char **err);
+
+void grads_zero(const Scene &sc, bool with_translation, Grads &gr);
+void make_ctx(const Scene &sc, Ctx &cx);
+std::vector<float> weight_image(const Scene &sc, const dvg_render_options &opt);
+void backward_samples(const Ctx &cx, const dvg_render_options &opt, bool compute_translation, const float *d_render,
+                      const std::vector<float> &wimg, Grads &gr, size_t i0, size_t i1);
+}  // namespace oracle

@@ -1,0 +1,159 @@
+"""ctypes binding of liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module.  It is the checker; nothing in the product path loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        from pydvg import abi
+        L = C.CDLL(LIB)
+        f32p, u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+        L.oracle_render.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p, f32p]
+        L.oracle_render.restype = C.c_int
+        L.oracle_render_backward.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions),
+                                             C.POINTER(abi.BackwardOptions), f32p, C.POINTER(abi.GradsC)]
+        L.oracle_render_backward.restype = C.c_int
+        L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, f32p]
+        L.oracle_solve_cubic_f64.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
+        L.oracle_solve_cubic_f64.restype = C.c_int
+        L.oracle_solve_cubic_f32.argtypes = [C.c_float] * 4 + [f32p]
+        L.oracle_solve_cubic_f32.restype = C.c_int
+        L.oracle_closest_f64.argtypes = [C.c_uint32, f32p, C.c_float, C.c_float, C.c_int, f32p]
+        L.oracle_closest_f32.argtypes = [C.c_uint32, f32p, C.c_float, C.c_float, C.c_int, f32p]
+        L.oracle_winding_f64.argtypes = [C.c_uint32, f32p, C.c_float, C.c_float]
+        L.oracle_winding_f64.restype = C.c_int
+        L.oracle_winding_f32.argtypes = [C.c_uint32, f32p, C.c_float, C.c_float]
+        L.oracle_winding_f32.restype = C.c_float
+        L.oracle_within_distance.argtypes = [C.POINTER(abi.SceneDescC), C.c_uint32, C.c_float, C.c_float]
+        L.oracle_is_inside.argtypes = [C.POINTER(abi.SceneDescC), C.c_uint32, C.c_float, C.c_float]
+        L.oracle_filter_weight.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_float]
+        L.oracle_filter_weight.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def render(desc, opts, want_samples=False):
+    """Forward image (W*H*4) with the reference GPU-forward semantics."""
+    sc = desc.scene
+    out = np.zeros(sc.width * sc.height * 4, np.float32)
+    spp = opts.samples_x * opts.samples_y
+    samples = np.zeros(sc.width * sc.height * spp * 4, np.float32) if want_samples else None
+    st = lib().oracle_render(desc.ref(), C.byref(opts.to_c()), _fp(out),
+                             _fp(samples) if want_samples else None)
+    if st != 0:
+        raise RuntimeError(f"oracle_render status {st}")
+    return (out, samples) if want_samples else out
+
+
+def render_backward(desc, opts, d_render, translation=False):
+    """Reference CPU render_backward gradients as pydvg.scene.FlatGrads."""
+    from pydvg.scene import BackwardOptions, FlatGrads
+    g = FlatGrads(desc, translation)
+    d_render = np.ascontiguousarray(d_render, np.float32)
+    st = lib().oracle_render_backward(desc.ref(), C.byref(opts.to_c()),
+                                      C.byref(BackwardOptions(translation).to_c()), _fp(d_render), g.ref())
+    if st != 0:
+        raise RuntimeError(f"oracle_render_backward status {st}")
+    return g
+
+
+def pcg32(idx, seed, n):
+    u = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.float32)
+    lib().oracle_pcg32(idx, seed, n, u.ctypes.data_as(C.POINTER(C.c_uint32)), _fp(f))
+    return u, f
+
+
+def solve_cubic_f64(a, b, c, d):
+    r = (C.c_double * 3)()
+    n = lib().oracle_solve_cubic_f64(a, b, c, d, r)
+    return sorted(r[i] for i in range(n))
+
+
+def solve_cubic_f32(a, b, c, d):
+    r = np.zeros(3, np.float32)
+    n = lib().oracle_solve_cubic_f32(a, b, c, d, _fp(r))
+    return sorted(float(x) for x in r[:n])
+
+
+def closest(kind, pts, p, approx=False, f32=False):
+    pts = np.ascontiguousarray(np.pad(np.asarray(pts, np.float32).reshape(-1), (0, 8 - np.asarray(pts).size)),
+                               np.float32)
+    if f32:
+        out = np.zeros(2, np.float32)
+        lib().oracle_closest_f32(kind, _fp(pts), p[0], p[1], int(approx), _fp(out))
+        return float(out[0]), float(out[1])
+    out = np.zeros(4, np.float32)
+    lib().oracle_closest_f64(kind, _fp(pts), p[0], p[1], int(approx), _fp(out))
+    return (float(out[0]), float(out[1])), float(out[2]), float(out[3])
+
+
+def winding(kind, pts, p, f32=False):
+    pts = np.ascontiguousarray(np.pad(np.asarray(pts, np.float32).reshape(-1), (0, 8 - np.asarray(pts).size)), np.float32)
+    if f32:
+        return lib().oracle_winding_f32(kind, _fp(pts), p[0], p[1])
+    return lib().oracle_winding_f64(kind, _fp(pts), p[0], p[1])
+
+
+def within_distance(desc, group, p):
+    return bool(lib().oracle_within_distance(desc.ref(), group, p[0], p[1]))
+
+
+def is_inside(desc, group, p):
+    return bool(lib().oracle_is_inside(desc.ref(), group, p[0], p[1]))
+
+
+def filter_weight(ftype, radius, dx, dy):
+    return float(lib().oracle_filter_weight(ftype, radius, dx, dy))
+
+
+def time_slice(desc, opts, d_render, seconds=15.0):
+    """Time the oracle's per-sample forward + backward + boundary work on a
+    contiguous slice of sample indices of the full workload, growing the
+    slice until it takes about `seconds`; returns pixel-samples/s."""
+    import time as _t
+    L = lib()
+    L.oracle_time_slice.argtypes = [C.POINTER(__import__("pydvg").abi.SceneDescC),
+                                    C.POINTER(__import__("pydvg").abi.RenderOptions), C.POINTER(C.c_float),
+                                    C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]
+    L.oracle_time_slice.restype = C.c_double
+    sc = desc.scene
+    total = sc.width * sc.height * opts.samples_x * opts.samples_y
+    d_render = np.ascontiguousarray(d_render, np.float32)
+    copts = opts.to_c()
+    # start the slice at the image centre rows so it sees typical coverage
+    n = 4096
+    ws = C.c_double(0.0)
+    while True:
+        i0 = max(0, total // 2 - n // 2)
+        t = L.oracle_time_slice(desc.ref(), C.byref(copts), _fp(d_render), i0, i0 + n, C.byref(ws))
+        if t >= seconds * 0.5 or n >= total:
+            break
+        n = min(total, int(n * max(2.0, seconds / max(t, 1e-3))))
+    per_sample = t / n
+    return {"samples_per_s": 1.0 / per_sample, "seconds": t, "samples": n, "weight_image_seconds": ws.value,
+            "sample": f"{n} consecutive pixel-samples (indices {i0}..{i0 + n}) of the full workload, "
+                      f"forward + interior backward + boundary sampling, 1 thread; the weight image "
+                      f"({ws.value:.2f} s for all samples) is excluded"}
