@@ -52,7 +52,9 @@ def main():
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    # one stream (default priority) shared by torch and libdvg
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     r = Renderer(0)
     r.set_stream(stream.cuda_stream)
 

@@ -25,6 +25,9 @@ struct dvg_ctx {
     size_t weight_cap = 0;
     float *accum = nullptr;
     size_t accum_cap = 0;
+    // boundary-pass work buffers (u32), grown on demand
+    uint32_t *bwork = nullptr;
+    size_t bwork_cap = 0;
 };
 
 struct dvg_device_scene {
@@ -107,6 +110,7 @@ void dvg_ctx_destroy(dvg_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->weight) (void)hipFree(ctx->weight);
     if (ctx->accum) (void)hipFree(ctx->accum);
+    if (ctx->bwork) (void)hipFree(ctx->bwork);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -115,7 +119,7 @@ const char *dvg_last_error(const dvg_ctx *ctx) { return ctx ? ctx->last_error : 
 
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *s) {
     if (!ctx) return DVG_DEVICE_ERROR;
-    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    ctx->stream = (hipStream_t)s;
     return DVG_OK;
 }
 
@@ -429,7 +433,24 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
     dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
     dvgk::launch_weights(S, ctx->weight, true, ctx->stream);
     dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
-    if (S.n_cdf > 0) dvgk::launch_boundary(S, ctx->weight, d_render, G, ctx->stream);
+    if (S.n_cdf > 0) {
+        size_t ns = npix * S.spp, nt = (size_t)S.tiles_x * S.tiles_y;
+        size_t need = 2 * ns + 3 * nt + 3;
+        if (ctx->bwork_cap < need) {
+            if (ctx->bwork) (void)hipFree(ctx->bwork);
+            ctx->bwork = nullptr;
+            ctx->bwork_cap = 0;
+            HIP_TRY(ctx, hipMalloc(&ctx->bwork, need * sizeof(uint32_t)));
+            ctx->bwork_cap = need;
+        }
+        dvgk::BoundaryWork bw;
+        bw.keys = ctx->bwork;
+        bw.order = bw.keys + ns;
+        bw.counts = bw.order + ns;
+        bw.offsets = bw.counts + nt + 1;
+        bw.cursor = bw.offsets + nt + 2;
+        dvgk::launch_boundary(S, ctx->weight, d_render, G, bw, ctx->stream);
+    }
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }

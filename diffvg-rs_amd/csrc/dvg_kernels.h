@@ -60,8 +60,13 @@ void launch_forward(const DevScene &S, const float *weight, float *accum, hipStr
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                               hipStream_t stream);
+// Work buffers of the boundary pass: keys/order hold one u32 per sample,
+// counts ntiles+1, offsets ntiles+2, cursor ntiles.
+struct BoundaryWork {
+    uint32_t *keys, *order, *counts, *offsets, *cursor;
+};
 void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                     hipStream_t stream);
+                     const BoundaryWork &bw, hipStream_t stream);
 // Zero `count` (<= 14) device float arrays of the given lengths.
 void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
 

@@ -5,7 +5,9 @@
 //   k_forward                            per-sample scene evaluation + splat (forward.rs:1568-1735)
 //   k_resolve                            splat resolve (forward.rs:295-334)
 //   k_backward_interior                  colour adjoint per sample (backward/sampling.rs:76-224)
-//   k_boundary                           edge sampling (backward/boundary.rs:113-1148)
+//   k_bnd_key / k_bnd_scatter / k_bnd_eval
+//                                        edge sampling (backward/boundary.rs:113-1148),
+//                                        counting-sorted by tile before evaluation
 //
 // One workgroup of 256 lanes (4 wave64) owns one 16x16 tile; its samples are
 // visited in 4x4-pixel blocks so that a wave covers a compact footprint and
@@ -1109,12 +1111,16 @@ DVG_DEV M3 m3_mul(const M3 &a, const M3 &b) {  // math.rs:185-195
     return o;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__restrict__ weight,
-                                                    const float *__restrict__ d_render, DevGrads G) {
+// One boundary sample of boundary_sampling (boundary.rs:138-197): shape by
+// length CDF, point and normal on it, canvas position and pixel.
+struct BPoint {
+    float t, pdf, lx, ly, ncx, ncy, qx, qy;
+    uint32_t si, gi, prep, kind, pxi, pyi;
+    BData bd;
+};
+
+DVG_DEV bool boundary_point(const DevScene &S, size_t idx, BPoint &P) {
     const float PI = 3.14159274101257324219f;
-    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    size_t total = (size_t)S.W * S.H * S.spp;
-    if (idx >= total || S.n_cdf == 0) return;
     Pcg32 rng((uint64_t)idx, (uint64_t)S.seed);
     float u = rng.next_f32();
     float t = rng.next_f32();
@@ -1122,13 +1128,13 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
     uint32_t sid = cdf_search(S.shape_cdf, S.n_cdf, u, unused);
     uint32_t si = S.shape_ids[sid], gi = S.group_ids[sid];
     float spmf = S.shape_pmf[sid];
-    if (spmf <= 0.f) return;
+    if (spmf <= 0.f) return false;
     uint32_t prep = sid;  // prepared instances follow the CDF order
     float slen = S.shape_lengths[si];
     const float *gd = S.group_data + gi * 16u;
     bool has_fill = (uint32_t)gd[2] != DVG_PAINT_NONE, has_stroke = (uint32_t)gd[4] != DVG_PAINT_NONE;
     // sample_boundary_point (boundary.rs:472-583)
-    if (!has_fill && !has_stroke) return;
+    if (!has_fill && !has_stroke) return false;
     float pdf = 1.0f, lt = t;
     bool stroke = false;
     if (has_fill && has_stroke) {
@@ -1149,7 +1155,7 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
     float lx = 0.f, ly = 0.f, nx = 0.f, ny = 0.f;
     if (kind == DVG_SHAPE_CIRCLE) {
         float r = sd[6];
-        if (r <= 0.f) return;
+        if (r <= 0.f) return false;
         float angle = 2.0f * PI * lt;
         float offx = r * cosf(angle), offy = r * sinf(angle);
         float2 nn = normalize2(offx, offy);
@@ -1162,7 +1168,7 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
         }
     } else if (kind == DVG_SHAPE_ELLIPSE) {
         float rx = sd[6], ry = sd[7];
-        if (rx <= 0.f || ry <= 0.f) return;
+        if (rx <= 0.f || ry <= 0.f) return false;
         float angle = 2.0f * PI * lt;
         float s_ = sinf(angle), c_ = cosf(angle);
         float offx = rx * c_, offy = ry * s_;
@@ -1178,7 +1184,7 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
     } else if (kind == DVG_SHAPE_RECT) {
         float mnx = sd[4], mny = sd[5], mxx = sd[6], mxy = sd[7];
         float w = mxx - mnx, h = mxy - mny;
-        if (w == 0.f && h == 0.f) return;
+        if (w == 0.f && h == 0.f) return false;
         pdf /= 2.0f * (w + h);
         if (lt <= w / (w + h)) {
             lt *= (w + h) / w;
@@ -1194,10 +1200,10 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
             if (dir < 0.f) { nx = nx * -1.0f; ny = ny * -1.0f; }
         }
     } else {
-        if (S.path_cdf_cnt[si] == 0 || !(slen > 0.f)) return;
-        if (!sample_boundary_path(S, si, slen, lt, dir, sw, lx, ly, nx, ny, pdf, bd)) return;
+        if (S.path_cdf_cnt[si] == 0 || !(slen > 0.f)) return false;
+        if (!sample_boundary_path(S, si, slen, lt, dir, sw, lx, ly, nx, ny, pdf, bd)) return false;
     }
-    if (!isfinite(pdf) || pdf <= 0.f) return;
+    if (!isfinite(pdf) || pdf <= 0.f) return false;
     pdf = spmf * pdf;
     // boundary.rs:168-174: local -> canvas, normal by the inverse transpose
     M3 Gs = m3_affine(S.group_shape_xform + gi * 6u);
@@ -1211,123 +1217,228 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(DevScene S, const float *__r
     float2 nc = normalize2(ci.m[0][0] * nx + ci.m[1][0] * ny, ci.m[0][1] * nx + ci.m[1][1] * ny);
     float Wf = (float)S.W, Hf = (float)S.H;
     float qx = bx / Wf, qy = by / Hf;
-    // render_edge_sample (boundary.rs:845-967)
+    // render_edge_sample (boundary.rs:845-867): the pixel under the point
     int pxi = (int)(qx * Wf), pyi = (int)(qy * Hf);
-    if (pxi < 0 || pxi >= (int)S.W || pyi < 0 || pyi >= (int)S.H) return;
-    size_t pix = (size_t)pyi * S.W + (size_t)pxi;
-    float4 bgs = background_at(S, (uint32_t)pxi, (uint32_t)pyi);
-    float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
-    bool ih, oh;
-    float4 cin = edge_color(S, qx - nc.x * 1.0e-4f, qy - nc.y * 1.0e-4f, bg, gi, prep, ih);
-    float4 cout = edge_color(S, qx + nc.x * 1.0e-4f, qy + nc.y * 1.0e-4f, bg, gi, prep, oh);
-    if (!ih && !oh) return;
-    float nnx = nc.x, nny = nc.y;
-    if (!ih) {
-        nnx = nnx * -1.0f; nny = nny * -1.0f;
-        float4 tmp = cin; cin = cout; cout = tmp;
-    }
-    float4 dc = gather_d_color(S, weight, d_render, qx * Wf, qy * Hf);
-    float norm = Wf * Hf;
-    if (norm > 0.f) { dc.x /= norm; dc.y /= norm; dc.z /= norm; dc.w /= norm; }
-    float contrib = ((cin.x - cout.x) * dc.x + (cin.y - cout.y) * dc.y + (cin.z - cout.z) * dc.z +
-                     (cin.w - cout.w) * dc.w) / pdf;
-    if (!isfinite(contrib)) return;
-    // accumulate_boundary_gradient (boundary.rs:970-1148)
-    if (bd.is_stroke) {
-        uint32_t nt = S.st_cnt[si];
-        if (kind == DVG_SHAPE_PATH && nt > 0) {
-            float *dth = G.d_shape_thickness + S.st_off[si];
-            uint32_t np = S.sp_cnt[si];
-            bool closed = S.s_closed[si] != 0;
-            uint32_t nc_ = bd.base_point_id < S.sc_cnt[si] ? S.path_ctrl[S.sc_off[si] + bd.base_point_id] : 0u;
-            uint32_t pid = bd.point_id;
-            float tv = bd.t, tt = 1.0f - tv;
-            auto add = [&](uint32_t i, float v) { if (i < nt) unsafeAtomicAdd(dth + i, v); };
-            if (nc_ == 0) {
-                add(pid, (1.0f - tv) * contrib);
-                add(closed ? (pid + 1) % np : pid + 1, tv * contrib);
-            } else if (nc_ == 1) {
-                add(pid, tt * tt * contrib);
-                add(pid + 1, 2.0f * tt * tv * contrib);
-                add(closed ? (pid + 2) % np : pid + 2, tv * tv * contrib);
-            } else if (nc_ == 2) {
-                add(pid, tt * tt * tt * contrib);
-                add(pid + 1, 3.0f * tt * tt * tv * contrib);
-                add(pid + 2, 3.0f * tt * tv * tv * contrib);
-                add(closed ? (pid + 3) % np : pid + 3, tv * tv * tv * contrib);
+    if (pxi < 0 || pxi >= (int)S.W || pyi < 0 || pyi >= (int)S.H) return false;
+    P.t = t; P.pdf = pdf; P.lx = lx; P.ly = ly; P.ncx = nc.x; P.ncy = nc.y; P.qx = qx; P.qy = qy;
+    P.si = si; P.gi = gi; P.prep = prep; P.kind = kind; P.pxi = (uint32_t)pxi; P.pyi = (uint32_t)pyi;
+    P.bd = bd;
+    return true;
+}
+
+DVG_DEV uint32_t tile_of(const DevScene &S, uint32_t x, uint32_t y) {
+    return (y / TILE) * S.tiles_x + x / TILE;
+}
+
+// Pass 1: tile key of every boundary sample (ntiles when it contributes
+// nothing) and the per-tile histogram.
+__global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ counts) {
+    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t total = (size_t)S.W * S.H * S.spp;
+    if (idx >= total) return;
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    BPoint P;
+    uint32_t key = ntiles;
+    if (S.n_cdf > 0 && boundary_point(S, idx, P)) key = tile_of(S, P.pxi, P.pyi);
+    keys[idx] = key;
+    atomicAdd(counts + key, 1u);
+}
+
+// Pass 2: counting-sort scatter of sample indices by tile.
+__global__ __launch_bounds__(BLOCK) void k_bnd_scatter(DevScene S, const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    size_t total = (size_t)S.W * S.H * S.spp;
+    if (idx >= total) return;
+    uint32_t key = keys[idx];
+    if (key >= S.tiles_x * S.tiles_y) return;
+    uint32_t pos = offsets[key] + atomicAdd(cursor + key, 1u);
+    order[pos] = (uint32_t)idx;
+}
+
+// Sum over the lanes of the wave whose `mine` is set; every lane returns it.
+DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0.f); }
+
+// Pass 3: boundary samples in tile order.  Consecutive lanes sit on the same
+// few shapes of one tile, so the tile's group list is shared by the wave and
+// the shape gradients are reduced per shape instance before the atomics.
+__global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__restrict__ weight,
+                                                    const float *__restrict__ d_render, DevGrads G,
+                                                    const uint32_t *__restrict__ offsets,
+                                                    const uint32_t *__restrict__ order) {
+    const float PI = 3.14159274101257324219f;
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    uint32_t n_valid = offsets[ntiles];
+    uint32_t pos = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t lane = threadIdx.x & 63u;
+    if (blockIdx.x * BLOCK >= n_valid) return;  // whole workgroup past the end
+    BPoint P;
+    bool valid = false;
+    float contrib = 0.f, nnx = 0.f, nny = 0.f;
+    size_t pix = 0;
+    if (pos < n_valid) {
+        size_t idx = order[pos];
+        valid = boundary_point(S, idx, P);
+        if (valid) {
+            pix = (size_t)P.pyi * S.W + P.pxi;
+            float4 bgs = background_at(S, P.pxi, P.pyi);
+            float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
+            bool ih, oh;
+            float4 cin = edge_color(S, P.qx - P.ncx * 1.0e-4f, P.qy - P.ncy * 1.0e-4f, bg, P.gi, P.prep, ih);
+            float4 cout = edge_color(S, P.qx + P.ncx * 1.0e-4f, P.qy + P.ncy * 1.0e-4f, bg, P.gi, P.prep, oh);
+            valid = ih || oh;
+            nnx = P.ncx;
+            nny = P.ncy;
+            if (valid && !ih) {
+                nnx = nnx * -1.0f; nny = nny * -1.0f;
+                float4 tmp = cin; cin = cout; cout = tmp;
             }
-        } else {
-            unsafeAtomicAdd(G.d_shape_stroke_width + si, contrib);
-        }
-    }
-    float *dp = G.d_shape_params + si * 8u;
-    if (kind == DVG_SHAPE_CIRCLE) {
-        unsafeAtomicAdd(dp + 0, nnx * contrib);
-        unsafeAtomicAdd(dp + 1, nny * contrib);
-        unsafeAtomicAdd(dp + 2, contrib);
-    } else if (kind == DVG_SHAPE_ELLIPSE) {
-        unsafeAtomicAdd(dp + 0, nnx * contrib);
-        unsafeAtomicAdd(dp + 1, nny * contrib);
-        float angle = 2.0f * PI * t;
-        unsafeAtomicAdd(dp + 2, cosf(angle) * nnx * contrib);
-        unsafeAtomicAdd(dp + 3, sinf(angle) * nny * contrib);
-    } else if (kind == DVG_SHAPE_PATH) {
-        uint32_t np = S.sp_cnt[si];
-        bool closed = S.s_closed[si] != 0;
-        uint32_t nc_ = bd.base_point_id < S.sc_cnt[si] ? S.path_ctrl[S.sc_off[si] + bd.base_point_id] : 0u;
-        uint32_t pid = bd.point_id;
-        float tv = bd.t, tt = 1.0f - tv;
-        float *dpt = G.d_shape_points + (size_t)S.sp_off[si] * 2u;
-        auto add = [&](uint32_t i, float w) {
-            uint32_t index = closed ? i % np : i;
-            if (index < np) {
-                unsafeAtomicAdd(dpt + index * 2u, nnx * (w * contrib));
-                unsafeAtomicAdd(dpt + index * 2u + 1, nny * (w * contrib));
+            if (valid) {
+                float Wf = (float)S.W, Hf = (float)S.H;
+                float4 dc = gather_d_color(S, weight, d_render, P.qx * Wf, P.qy * Hf);
+                float norm = Wf * Hf;
+                if (norm > 0.f) { dc.x /= norm; dc.y /= norm; dc.z /= norm; dc.w /= norm; }
+                contrib = ((cin.x - cout.x) * dc.x + (cin.y - cout.y) * dc.y + (cin.z - cout.z) * dc.z +
+                           (cin.w - cout.w) * dc.w) / P.pdf;
+                valid = isfinite(contrib);
             }
-        };
-        if (nc_ == 0) {
-            add(pid, 1.0f - tv);
-            add(pid + 1, tv);
-        } else if (nc_ == 1) {
-            add(pid, tt * tt);
-            add(pid + 1, 2.0f * tt * tv);
-            add(pid + 2, tv * tv);
-        } else if (nc_ == 2) {
-            add(pid, tt * tt * tt);
-            add(pid + 1, 3.0f * tt * tt * tv);
-            add(pid + 2, 3.0f * tt * tv * tv);
-            add(pid + 3, tv * tv * tv);
         }
-    } else if (kind == DVG_SHAPE_RECT) {
-        if (nnx == -1.f && nny == 0.f) unsafeAtomicAdd(dp + 0, -contrib);
-        else if (nnx == 1.f && nny == 0.f) unsafeAtomicAdd(dp + 2, contrib);
-        else if (nnx == 0.f && nny == -1.f) unsafeAtomicAdd(dp + 1, -contrib);
-        else if (nnx == 0.f && nny == 1.f) unsafeAtomicAdd(dp + 3, contrib);
     }
-    // d_xform_pt (backward/math.rs:101-120) on shape_to_canvas = G * T, then
-    // d_G += dM * T^T and d_T += G^T * dM.
-    float t0 = s2c.m[0][0] * lx + s2c.m[0][1] * ly + s2c.m[0][2];
-    float t1 = s2c.m[1][0] * lx + s2c.m[1][1] * ly + s2c.m[1][2];
-    float t2 = s2c.m[2][0] * lx + s2c.m[2][1] * ly + s2c.m[2][2];
-    float dox = nnx * contrib, doy = nny * contrib;
-    float ox = t0 / t2, oy = t1 / t2;
-    float dt0 = dox / t2, dt1 = doy / t2, dt2 = -(dox * ox + doy * oy) / t2;
-    M3 dm;
-    dm.m[0][0] = dt0 * lx; dm.m[0][1] = dt0 * ly; dm.m[0][2] = dt0;
-    dm.m[1][0] = dt1 * lx; dm.m[1][1] = dt1 * ly; dm.m[1][2] = dt1;
-    dm.m[2][0] = dt2 * lx; dm.m[2][1] = dt2 * ly; dm.m[2][2] = dt2;
-    float *dgt = G.d_group_transform + gi * 9u;
-    float *dst = G.d_shape_transform + si * 9u;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            float a = dm.m[i][0] * T.m[j][0] + dm.m[i][1] * T.m[j][1] + dm.m[i][2] * T.m[j][2];
-            float b = Gs.m[0][i] * dm.m[0][j] + Gs.m[1][i] * dm.m[1][j] + Gs.m[2][i] * dm.m[2][j];
-            unsafeAtomicAdd(dgt + i * 3 + j, a);
-            unsafeAtomicAdd(dst + i * 3 + j, b);
+    // ---- accumulate_boundary_gradient (boundary.rs:970-1148) ----
+    // per instance: d(shape_to_canvas) (9), stroke width (1), shape params (4)
+    float v[14];
+    for (int k = 0; k < 14; ++k) v[k] = 0.f;
+    uint32_t key = 0xffffffffu;
+    if (valid) {
+        key = P.prep;
+        M3 Gs = m3_affine(S.group_shape_xform + P.gi * 6u);
+        M3 T = m3_affine(S.shape_transform + P.si * 6u);
+        M3 s2c = m3_mul(Gs, T);
+        float t0 = s2c.m[0][0] * P.lx + s2c.m[0][1] * P.ly + s2c.m[0][2];
+        float t1 = s2c.m[1][0] * P.lx + s2c.m[1][1] * P.ly + s2c.m[1][2];
+        float t2 = s2c.m[2][0] * P.lx + s2c.m[2][1] * P.ly + s2c.m[2][2];
+        float dox = nnx * contrib, doy = nny * contrib;
+        float ox = t0 / t2, oy = t1 / t2;
+        float dt0 = dox / t2, dt1 = doy / t2, dt2 = -(dox * ox + doy * oy) / t2;
+        v[0] = dt0 * P.lx; v[1] = dt0 * P.ly; v[2] = dt0;
+        v[3] = dt1 * P.lx; v[4] = dt1 * P.ly; v[5] = dt1;
+        v[6] = dt2 * P.lx; v[7] = dt2 * P.ly; v[8] = dt2;
+        if (P.bd.is_stroke) {
+            uint32_t nt = S.st_cnt[P.si];
+            if (P.kind == DVG_SHAPE_PATH && nt > 0) {
+                float *dth = G.d_shape_thickness + S.st_off[P.si];
+                uint32_t np = S.sp_cnt[P.si];
+                bool closed = S.s_closed[P.si] != 0;
+                uint32_t nc_ = P.bd.base_point_id < S.sc_cnt[P.si] ? S.path_ctrl[S.sc_off[P.si] + P.bd.base_point_id] : 0u;
+                uint32_t pid = P.bd.point_id;
+                float tv = P.bd.t, tt = 1.0f - tv;
+                auto add = [&](uint32_t i, float x) { if (i < nt) unsafeAtomicAdd(dth + i, x); };
+                if (nc_ == 0) {
+                    add(pid, (1.0f - tv) * contrib);
+                    add(closed ? (pid + 1) % np : pid + 1, tv * contrib);
+                } else if (nc_ == 1) {
+                    add(pid, tt * tt * contrib);
+                    add(pid + 1, 2.0f * tt * tv * contrib);
+                    add(closed ? (pid + 2) % np : pid + 2, tv * tv * contrib);
+                } else if (nc_ == 2) {
+                    add(pid, tt * tt * tt * contrib);
+                    add(pid + 1, 3.0f * tt * tt * tv * contrib);
+                    add(pid + 2, 3.0f * tt * tv * tv * contrib);
+                    add(closed ? (pid + 3) % np : pid + 3, tv * tv * tv * contrib);
+                }
+            } else {
+                v[9] = contrib;
+            }
         }
-    if (G.d_translation) {
-        unsafeAtomicAdd(G.d_translation + pix * 2u, nnx * contrib);
-        unsafeAtomicAdd(G.d_translation + pix * 2u + 1, nny * contrib);
+        if (P.kind == DVG_SHAPE_CIRCLE) {
+            v[10] = nnx * contrib; v[11] = nny * contrib; v[12] = contrib;
+        } else if (P.kind == DVG_SHAPE_ELLIPSE) {
+            v[10] = nnx * contrib; v[11] = nny * contrib;
+            float angle = 2.0f * PI * P.t;
+            v[12] = cosf(angle) * nnx * contrib;
+            v[13] = sinf(angle) * nny * contrib;
+        } else if (P.kind == DVG_SHAPE_RECT) {
+            if (nnx == -1.f && nny == 0.f) v[10] = -contrib;
+            else if (nnx == 1.f && nny == 0.f) v[12] = contrib;
+            else if (nnx == 0.f && nny == -1.f) v[11] = -contrib;
+            else if (nnx == 0.f && nny == 1.f) v[13] = contrib;
+        }
+        if (G.d_translation) {
+            unsafeAtomicAdd(G.d_translation + pix * 2u, nnx * contrib);
+            unsafeAtomicAdd(G.d_translation + pix * 2u + 1, nny * contrib);
+        }
+    }
+    // reduce per instance across the wave; the leader applies
+    // d_G += dM T^T and d_T += G^T dM (the products are linear in dM)
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        uint32_t lkey = __shfl(key, leader, 64);
+        bool mine = valid && key == lkey;
+        float r[14];
+        for (int k = 0; k < 14; ++k) r[k] = masked_wave_sum(mine, v[k]);
+        if ((int)lane == leader) {
+            M3 Gs = m3_affine(S.group_shape_xform + P.gi * 6u);
+            M3 T = m3_affine(S.shape_transform + P.si * 6u);
+            float *dgt = G.d_group_transform + P.gi * 9u;
+            float *dst = G.d_shape_transform + P.si * 9u;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    float a = r[i * 3] * T.m[j][0] + r[i * 3 + 1] * T.m[j][1] + r[i * 3 + 2] * T.m[j][2];
+                    float b = Gs.m[0][i] * r[j] + Gs.m[1][i] * r[3 + j] + Gs.m[2][i] * r[6 + j];
+                    unsafeAtomicAdd(dgt + i * 3 + j, a);
+                    unsafeAtomicAdd(dst + i * 3 + j, b);
+                }
+            if (r[9] != 0.f) unsafeAtomicAdd(G.d_shape_stroke_width + P.si, r[9]);
+            float *dp = G.d_shape_params + P.si * 8u;
+            for (int k = 0; k < 4; ++k)
+                if (r[10 + k] != 0.f) unsafeAtomicAdd(dp + k, r[10 + k]);
+        }
+        pending &= ~__ballot(mine);
+    }
+    // path points, reduced per segment (same shape and first point)
+    uint32_t pkey = 0xffffffffu;
+    float pv[8];
+    for (int k = 0; k < 8; ++k) pv[k] = 0.f;
+    uint32_t nc_ = 0;
+    bool pvalid = valid && P.kind == DVG_SHAPE_PATH;
+    if (pvalid) {
+        nc_ = P.bd.base_point_id < S.sc_cnt[P.si] ? S.path_ctrl[S.sc_off[P.si] + P.bd.base_point_id] : 0u;
+        pvalid = nc_ <= 2;
+        float tv = P.bd.t, tt = 1.0f - tv;
+        float w[4] = {0.f, 0.f, 0.f, 0.f};
+        if (nc_ == 0) { w[0] = 1.0f - tv; w[1] = tv; }
+        else if (nc_ == 1) { w[0] = tt * tt; w[1] = 2.0f * tt * tv; w[2] = tv * tv; }
+        else { w[0] = tt * tt * tt; w[1] = 3.0f * tt * tt * tv; w[2] = 3.0f * tt * tv * tv; w[3] = tv * tv * tv; }
+        for (int k = 0; k < 4; ++k) {
+            pv[2 * k] = nnx * (w[k] * contrib);
+            pv[2 * k + 1] = nny * (w[k] * contrib);
+        }
+        pkey = S.sp_off[P.si] + P.bd.point_id;
+    }
+    pending = __ballot(pvalid);
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        uint32_t lkey = __shfl(pkey, leader, 64);
+        bool mine = pvalid && pkey == lkey;
+        float r[8];
+        for (int k = 0; k < 8; ++k) r[k] = masked_wave_sum(mine, pv[k]);
+        if ((int)lane == leader) {
+            uint32_t np = S.sp_cnt[P.si];
+            bool closed = S.s_closed[P.si] != 0;
+            float *dpt = G.d_shape_points + (size_t)S.sp_off[P.si] * 2u;
+            for (uint32_t k = 0; k <= nc_ + 1; ++k) {
+                uint32_t i = P.bd.point_id + k;
+                uint32_t index = closed ? i % np : i;
+                if (index < np) {
+                    unsafeAtomicAdd(dpt + index * 2u, r[2 * k]);
+                    unsafeAtomicAdd(dpt + index * 2u + 1, r[2 * k + 1]);
+                }
+            }
+        }
+        pending &= ~__ballot(mine);
     }
 }
 
@@ -1393,9 +1504,18 @@ void launch_backward_interior(const DevScene &S, const float *weight, const floa
                        d_render, G);
 }
 void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                     hipStream_t stream) {
+                     const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
-    hipLaunchKernelGGL(k_boundary, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G);
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    uint32_t nb = cdiv(total, BLOCK);
+    // counts[ntiles + 1] and cursor[ntiles] start at zero
+    float *zp[2] = {reinterpret_cast<float *>(bw.counts), reinterpret_cast<float *>(bw.cursor)};
+    uint32_t zn[2] = {ntiles + 1, ntiles};
+    launch_zero(zp, zn, 2, stream);
+    hipLaunchKernelGGL(k_bnd_key, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, ntiles + 1, bw.offsets);
+    hipLaunchKernelGGL(k_bnd_scatter, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.offsets, bw.cursor, bw.order);
+    hipLaunchKernelGGL(k_bnd_eval, dim3(nb), dim3(BLOCK), 0, stream, S, weight, d_render, G, bw.offsets, bw.order);
 }
 
 }  // namespace dvgk

@@ -320,8 +320,9 @@ dvg_ctx *dvg_ctx_create(int device_ordinal);
 void dvg_ctx_destroy(dvg_ctx *ctx);
 /* Static message of the last non-OK status on this context. */
 const char *dvg_last_error(const dvg_ctx *ctx);
-/* Use a caller stream (hipStream_t passed as void*) for device-resident calls;
- * NULL restores the context's own stream. */
+/* Run all later work of this context on a caller stream (hipStream_t passed
+ * as void*; NULL is the device's null stream).  Until called, the context
+ * uses a non-blocking stream of its own. */
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *hip_stream);
 
 /* ---- scene API ------------------------------------------------------------ */
