@@ -106,6 +106,13 @@ def main():
     value = n_samples / (wall / args.steps)
     log(f"bench: {ms_step:.3f} ms/step (forward {fwd_ms:.3f} ms, backward {bwd_ms:.3f} ms)")
 
+    # a plain copy of known size (64 MiB read + 64 MiB written), the scale
+    # reference for FETCH_SIZE / WRITE_SIZE counter runs
+    cal_src = torch.ones(16 * 1024 * 1024, dtype=torch.float32, device=dev)
+    cal_dst = torch.empty_like(cal_src)
+    cal_dst.copy_(cal_src)
+    torch.cuda.synchronize()
+
     # Adam steps/s: host packing + upload + forward + loss + backward + update
     loop = AdamLoop(r, sc, opts, tgt, lr_points=0.5, lr_colors=1e-2)
     loop.step()
@@ -115,18 +122,41 @@ def main():
     adam_s = (time.perf_counter() - t1) / args.adam_steps
     log(f"bench: Adam {1.0 / adam_s:.2f} steps/s, losses {losses}")
 
+    # per-kernel durations from HIP events recorded on the kernels' stream
+    r.set_timing(True)
+    r.kernel_times(reset=True)
+    n_prof = max(3, min(args.steps, 10))
+    for _ in range(n_prof):
+        step()
+    kt = r.kernel_times(reset=True)
+    r.set_timing(False)
+    per_kernel_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}
+    # one launch of each kernel per step except weights/zero (twice)
+    log("bench: per-kernel ms " + json.dumps({k: round(v, 4) for k, v in per_kernel_ms.items()}))
+
     roofline = None
     flops_path = os.path.join(ROOT, "profiles", "flops_headline.json")
     if os.path.exists(flops_path):
         with open(flops_path) as f:
             fl = json.load(f)
-        # dominant kernel: interior backward + boundary = the backward call
-        per_launch = fl["bwd_flops_per_sample"] * n_samples
-        achieved = per_launch / (bwd_ms * 1e-3) / 1e12
+        alg = {"boundary_eval": fl["bwd_boundary_flops_per_sample"] * n_samples,
+               "backward_interior": fl["bwd_interior_flops_per_sample"] * n_samples,
+               "forward": fl["fwd_flops_per_sample"] * n_samples}
+        dom = max(alg, key=lambda k: per_kernel_ms.get(k, 0.0))
+        ms = per_kernel_ms[dom]
+        achieved = alg[dom] / (ms * 1e-3) / 1e12
+        traffic = None
+        tr_path = os.path.join(ROOT, "profiles", "traffic_headline.json")
+        if os.path.exists(tr_path):
+            with open(tr_path) as f:
+                traffic = json.load(f).get("traffic_bytes", {}).get(dom)
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": fl.get("bwd_hbm_bytes"),
-                    "kernel": "render_backward (k_backward_interior + k_boundary)",
-                    "note": "fp32 VALU peak; algorithmic flops counted by the oracle"}
+                    "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                    "kernel": "k_bnd_eval" if dom == "boundary_eval" else dom, "kernel_ms": ms,
+                    "per_kernel_tflops": {k: round(alg[k] / (per_kernel_ms[k] * 1e-3) / 1e12, 3)
+                                          for k in alg if per_kernel_ms.get(k)},
+                    "note": "peak is MI355X fp32 (vector) 157.3 TFLOP/s: the path is scalar fp32 VALU work, "
+                            "no MFMA; algorithmic flops per sample from profiles/flops_headline.json"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -146,6 +176,7 @@ def main():
                                "box r=0.5, non-prefiltered fwd + interior bwd + boundary sampling",
                    "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu"},
         "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": 1.0 / adam_s,
+        "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -14,8 +14,20 @@
 
 #define DVG_VERSION_STRING "dvg-mi355x 0.1 (gfx950)"
 
+// Per-kernel timing (dvg_ctx_set_timing): event pairs recorded around each
+// launch on the context's stream, read back by dvg_kernel_times.
+struct TimedLaunch {
+    int kernel;
+    hipEvent_t start, stop;
+};
+
 struct dvg_ctx {
     int device = 0;
+    bool timing = false;
+    std::vector<TimedLaunch> pending;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+    double kernel_ms[DVG_NUM_KERNELS] = {};
+    uint32_t kernel_calls[DVG_NUM_KERNELS] = {};
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     const char *last_error = "";
@@ -39,6 +51,31 @@ struct dvg_device_scene {
 };
 
 namespace {
+
+// Brackets one kernel launch with timing events when timing is on.
+struct KernelTimer {
+    dvg_ctx *ctx;
+    int id;
+    TimedLaunch tl;
+    KernelTimer(dvg_ctx *c, int kernel) : ctx(c), id(kernel) {
+        if (!ctx->timing) return;
+        if (ctx->event_pool.empty()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) { ctx->timing = false; return; }
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); ctx->timing = false; return; }
+            ctx->event_pool.push_back({a, b});
+        }
+        auto ev = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        tl = {id, ev.first, ev.second};
+        (void)hipEventRecord(tl.start, ctx->stream);
+    }
+    ~KernelTimer() {
+        if (!ctx->timing) return;
+        (void)hipEventRecord(tl.stop, ctx->stream);
+        ctx->pending.push_back(tl);
+    }
+};
 
 int set_err(dvg_ctx *ctx, int code, const char *msg) {
     if (ctx) ctx->last_error = msg;
@@ -111,6 +148,11 @@ void dvg_ctx_destroy(dvg_ctx *ctx) {
     if (ctx->weight) (void)hipFree(ctx->weight);
     if (ctx->accum) (void)hipFree(ctx->accum);
     if (ctx->bwork) (void)hipFree(ctx->bwork);
+    for (const TimedLaunch &tl : ctx->pending) ctx->event_pool.push_back({tl.start, tl.stop});
+    for (auto &e : ctx->event_pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -120,6 +162,36 @@ const char *dvg_last_error(const dvg_ctx *ctx) { return ctx ? ctx->last_error : 
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *s) {
     if (!ctx) return DVG_DEVICE_ERROR;
     ctx->stream = (hipStream_t)s;
+    return DVG_OK;
+}
+
+int32_t dvg_ctx_set_timing(dvg_ctx *ctx, int enable) {
+    if (!ctx) return DVG_DEVICE_ERROR;
+    ctx->timing = enable != 0;
+    return DVG_OK;
+}
+
+int32_t dvg_kernel_times(dvg_ctx *ctx, double *total_ms, uint32_t *calls, int n, int reset) {
+    if (!ctx) return DVG_DEVICE_ERROR;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (const TimedLaunch &tl : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, tl.start, tl.stop) == hipSuccess) {
+            ctx->kernel_ms[tl.kernel] += ms;
+            ctx->kernel_calls[tl.kernel] += 1;
+        }
+        ctx->event_pool.push_back({tl.start, tl.stop});
+    }
+    ctx->pending.clear();
+    for (int k = 0; k < n && k < DVG_NUM_KERNELS; ++k) {
+        if (total_ms) total_ms[k] = ctx->kernel_ms[k];
+        if (calls) calls[k] = ctx->kernel_calls[k];
+    }
+    if (reset)
+        for (int k = 0; k < DVG_NUM_KERNELS; ++k) {
+            ctx->kernel_ms[k] = 0.0;
+            ctx->kernel_calls[k] = 0;
+        }
     return DVG_OK;
 }
 
@@ -392,11 +464,21 @@ int32_t dvg_device_render(dvg_ctx *ctx, dvg_device_scene *ds, float *out) {
     {
         float *p = ctx->accum;
         uint32_t n = (uint32_t)(npix * 4);
+        KernelTimer kt(ctx, DVG_KERNEL_ZERO);
         dvgk::launch_zero(&p, &n, 1, ctx->stream);
     }
-    dvgk::launch_weights(S, ctx->weight, false, ctx->stream);
-    dvgk::launch_forward(S, ctx->weight, ctx->accum, ctx->stream);
-    dvgk::launch_resolve(S, ctx->weight, ctx->accum, out, ctx->stream);
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_WEIGHTS);
+        dvgk::launch_weights(S, ctx->weight, false, ctx->stream);
+    }
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_FORWARD);
+        dvgk::launch_forward(S, ctx->weight, ctx->accum, ctx->stream);
+    }
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_RESOLVE);
+        dvgk::launch_resolve(S, ctx->weight, ctx->accum, out, ctx->stream);
+    }
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }
@@ -430,9 +512,18 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
     dvgk::DevGrads G;
     grads_to_dev(grads, G);
     // the outputs are accumulations into zeroed arrays
-    dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
-    dvgk::launch_weights(S, ctx->weight, true, ctx->stream);
-    dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_ZERO);
+        dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
+    }
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_WEIGHTS);
+        dvgk::launch_weights(S, ctx->weight, true, ctx->stream);
+    }
+    {
+        KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
+        dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
+    }
     if (S.n_cdf > 0) {
         size_t ns = npix * S.spp, nt = (size_t)S.tiles_x * S.tiles_y;
         size_t need = 2 * ns + 3 * nt + 3;
@@ -449,7 +540,14 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         bw.counts = bw.order + ns;
         bw.offsets = bw.counts + nt + 1;
         bw.cursor = bw.offsets + nt + 2;
-        dvgk::launch_boundary(S, ctx->weight, d_render, G, bw, ctx->stream);
+        {
+            KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_SORT);
+            dvgk::launch_boundary_sort(S, bw, ctx->stream);
+        }
+        {
+            KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_EVAL);
+            dvgk::launch_boundary_eval(S, ctx->weight, d_render, G, bw, ctx->stream);
+        }
     }
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;

@@ -65,8 +65,10 @@ void launch_backward_interior(const DevScene &S, const float *weight, const floa
 struct BoundaryWork {
     uint32_t *keys, *order, *counts, *offsets, *cursor;
 };
-void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                     const BoundaryWork &bw, hipStream_t stream);
+// Boundary samples: tile keys, counting sort; then the tile-ordered evaluation.
+void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream);
+void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                          const BoundaryWork &bw, hipStream_t stream);
 // Zero `count` (<= 14) device float arrays of the given lengths.
 void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
 

@@ -1503,8 +1503,7 @@ void launch_backward_interior(const DevScene &S, const float *weight, const floa
     hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
                        d_render, G);
 }
-void launch_boundary(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                     const BoundaryWork &bw, hipStream_t stream) {
+void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     uint32_t ntiles = S.tiles_x * S.tiles_y;
     uint32_t nb = cdiv(total, BLOCK);
@@ -1515,7 +1514,13 @@ void launch_boundary(const DevScene &S, const float *weight, const float *d_rend
     hipLaunchKernelGGL(k_bnd_key, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, ntiles + 1, bw.offsets);
     hipLaunchKernelGGL(k_bnd_scatter, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.offsets, bw.cursor, bw.order);
-    hipLaunchKernelGGL(k_bnd_eval, dim3(nb), dim3(BLOCK), 0, stream, S, weight, d_render, G, bw.offsets, bw.order);
+}
+
+void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                          const BoundaryWork &bw, hipStream_t stream) {
+    uint64_t total = (uint64_t)S.W * S.H * S.spp;
+    hipLaunchKernelGGL(k_bnd_eval, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
+                       bw.offsets, bw.order);
 }
 
 }  // namespace dvgk

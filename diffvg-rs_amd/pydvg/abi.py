@@ -112,6 +112,8 @@ class RenderParamsC(C.Structure):
                 ("background_image", _f32p), ("compute_translation", C.c_uint32)]
 
 
+KERNEL_NAMES = ["zero", "weights", "forward", "resolve", "backward_interior", "boundary_sort", "boundary_eval"]
+
 # Every symbol include/dvg.h declares, with its ctypes signature.
 _vp = C.c_void_p
 SIGNATURES = {
@@ -136,6 +138,8 @@ SIGNATURES = {
     "dvg_device_render": (C.c_int32, [_vp, _vp, _vp]),
     "dvg_device_render_backward": (C.c_int32, [_vp, _vp, _vp, C.POINTER(GradsC)]),
     "dvg_sync": (C.c_int32, [_vp]),
+    "dvg_ctx_set_timing": (C.c_int32, [_vp, C.c_int]),
+    "dvg_kernel_times": (C.c_int32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int, C.c_int]),
     "dvg_version": (C.c_char_p, []),
 }
 

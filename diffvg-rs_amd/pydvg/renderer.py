@@ -101,6 +101,17 @@ class Renderer:
     def sync(self):
         _check(self.lib, self.ctx, self.lib.dvg_sync(self.ctx))
 
+    def set_timing(self, enable: bool):
+        self.lib.dvg_ctx_set_timing(self.ctx, int(enable))
+
+    def kernel_times(self, reset=True):
+        """{kernel name: (total ms, launches)} since the last reset."""
+        n = len(abi.KERNEL_NAMES)
+        ms = (C.c_double * n)()
+        calls = (C.c_uint32 * n)()
+        _check(self.lib, self.ctx, self.lib.dvg_kernel_times(self.ctx, ms, calls, n, int(reset)))
+        return {abi.KERNEL_NAMES[k]: (ms[k], calls[k]) for k in range(n)}
+
 
 class DeviceScene:
     """A packed scene uploaded once to HBM (dvg_device_scene_create).

@@ -384,6 +384,25 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,
 /* Block until the context's stream is idle. */
 int32_t dvg_sync(dvg_ctx *ctx);
 
+/* ---- kernel timing (no counterpart in the reference) ---------------------- */
+
+/* Kernel ids of dvg_kernel_times. */
+#define DVG_KERNEL_ZERO 0
+#define DVG_KERNEL_WEIGHTS 1
+#define DVG_KERNEL_FORWARD 2
+#define DVG_KERNEL_RESOLVE 3
+#define DVG_KERNEL_INTERIOR 4
+#define DVG_KERNEL_BOUNDARY_SORT 5
+#define DVG_KERNEL_BOUNDARY_EVAL 6
+#define DVG_NUM_KERNELS 7
+
+/* When enabled, every later launch of the context is bracketed by HIP events
+ * on its stream (the tracing hook the reference lacks). */
+int32_t dvg_ctx_set_timing(dvg_ctx *ctx, int enable);
+/* Synchronizes the stream, then reports per kernel id the accumulated
+ * milliseconds and launch counts (arrays of n); reset != 0 clears them. */
+int32_t dvg_kernel_times(dvg_ctx *ctx, double *total_ms, uint32_t *calls, int n, int reset);
+
 /* Version string of the library build. */
 const char *dvg_version(void);
 

@@ -1526,6 +1526,47 @@ void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed,
     }
 }
 
+// Canvas positions of the two colour queries (boundary.rs:880-901) of every
+// contributing boundary sample in [i0, i1); for operation counting.
+void boundary_query_points(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, size_t i0, size_t i1,
+                           std::vector<float> &pts) {
+    const Scene &sc = *cx.sc;
+    pts.clear();
+    if (sc.groups.empty()) return;
+    std::vector<float> L = compute_shape_lengths(cx);
+    ShapeCdf cdf;
+    if (!build_shape_cdf(sc, L, cdf)) return;
+    std::vector<PathCdf> pcs = build_path_cdfs(cx, L);
+    size_t total = (size_t)sc.width * sc.height * nsx * nsy;
+    if (i1 > total) i1 = total;
+    for (size_t idx = i0; idx < i1; ++idx) {
+        Pcg32 rng(idx, seed);
+        float u = rng.next_f32();
+        float t = rng.next_f32();
+        size_t sid;
+        float unused;
+        sample_cdf(cdf.cdf, u, sid, unused);
+        if (sid >= cdf.shape_ids.size()) continue;
+        uint32_t si = cdf.shape_ids[sid], gi = cdf.group_ids[sid];
+        if (cdf.pmf[sid] <= 0.f) continue;
+        V2 lp, nrm;
+        float bpdf;
+        BData bd;
+        if (!sample_boundary_point(sc, si, gi, t, L[si], &pcs[si], lp, nrm, bpdf, bd)) continue;
+        const Group &g = sc.groups[gi];
+        const Shape &sh = sc.shapes[si];
+        V2 bpt = g.shape_to_canvas.mul(sh.transform).transform_point(lp);
+        V2 n = xform_normal(sh.transform.inverse_or_identity().mul(g.canvas_to_shape), nrm);
+        float qx = bpt.x / (float)sc.width, qy = bpt.y / (float)sc.height;
+        int bx = (int)(qx * (float)sc.width), by = (int)(qy * (float)sc.height);
+        if (bx < 0 || by < 0 || bx >= (int)sc.width || by >= (int)sc.height) continue;
+        for (float sgn : {-1.0f, 1.0f}) {
+            pts.push_back((qx + sgn * n.x * 1.0e-4f) * (float)sc.width);
+            pts.push_back((qy + sgn * n.y * 1.0e-4f) * (float)sc.height);
+        }
+    }
+}
+
 }  // namespace cpu
 
 // The per-sample loop of render_backward (backward/render.rs:95-189) over

@@ -45,6 +45,8 @@ void d_sample_paint(const Paint &p, Grads &gr, float *d_paint_solid, V2 pt, V4 d
 float compute_filter_weight(uint32_t type, float radius, float dx, float dy);
 V4 sample_background(const Scene &sc, size_t pix);
 void build_group_bounds(Ctx &cx);
+void boundary_query_points(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, size_t i0, size_t i1,
+                           std::vector<float> &pts);
 void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, const float *d_render,
                        const std::vector<float> &wimg, Grads &gr, bool compute_translation, size_t i0, size_t i1);
 

@@ -248,6 +248,10 @@ static inline bool path_point_radius(const Path &p, size_t index, float default_
     return true;
 }
 
+struct OpCount {
+    uint64_t flops = 0, sfu = 0, frags = 0;
+};
+
 // One path segment with control points and per-point radii
 // (src/renderer/path.rs:100-173, src/distance/bvh.rs:262-380).
 struct Seg {

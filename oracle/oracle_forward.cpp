@@ -11,6 +11,19 @@
 #include "oracle_forward.h"
 
 namespace oracle {
+
+// Algorithmic operation tally (count mode only): +, -, *, /, compare,
+// min/max and sqrt count 1 flop; acos/cos/sin/atan2/pow count 1 sfu.
+OpCount g_ops;
+bool g_counting = false;
+#define OPS(f, s_)                    \
+    do {                              \
+        if (g_counting) {             \
+            g_ops.flops += (f);       \
+            g_ops.sfu += (s_);        \
+        }                             \
+    } while (0)
+
 namespace gpu {
 
 // ---- math.rs ---------------------------------------------------------------
@@ -38,6 +51,7 @@ static inline float dist_pt(float px, float py, float ax, float ay) {  // curves
 
 void distance_to_segment_with_t(float px, float py, float ax, float ay, float bx, float by, float &dist,
                                 float &t) {  // curves.rs:39-66
+    OPS(18, 0);
     float vx = bx - ax, vy = by - ay, wx = px - ax, wy = py - ay;
     float c1 = vx * wx + vy * wy;
     float c2 = vx * vx + vy * vy;
@@ -70,7 +84,9 @@ float distance_to_segment(float px, float py, float ax, float ay, float bx, floa
 
 int solve_quadratic(float a, float b, float c, float out[2]) {  // curves.rs:865-893
     float disc = b * b - 4.0f * a * c;
+    OPS(5, 0);
     if (disc < 0.f) { out[0] = out[1] = 0.f; return 0; }
+    OPS(8, 0);
     float root = sqrt_f(disc);
     float q = (b < 0.f) ? -0.5f * (b - root) : -0.5f * (b + root);
     float t0 = q / a, t1 = c / q;
@@ -89,6 +105,7 @@ static float cbrt_gpu(float x) {  // curves.rs:949-959
 
 int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.rs:897-946
     const float eps = 1.0e-6f;
+    OPS(2, 0);
     if (abs_f(a) < eps) {
         float r[2];
         int n = solve_quadratic(b, c, d, r);
@@ -100,7 +117,9 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
     float q = (bb * bb - 3.0f * cc) / 9.0f;
     float r = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
     float r2 = r * r, q3 = q * q * q;
+    OPS(21, 0);
     if (r2 < q3) {
+        OPS(22, 4);
         float theta = acos_f(r / sqrt_f(q3));
         float sqrt_q = sqrt_f(q);
         const float two_pi = 2.0f * 3.14159265f;
@@ -109,6 +128,7 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
         out[2] = -2.0f * sqrt_q * cos_f((theta - two_pi) * third) - bb * third;
         return 3;
     }
+    OPS(11, 1);
     float a_root = (r > 0.f) ? -cbrt_gpu(r + sqrt_f(r2 - q3)) : cbrt_gpu(-r + sqrt_f(r2 - q3));
     float b_root = (abs_f(a_root) > eps) ? q / a_root : 0.f;
     out[0] = (a_root + b_root) - bb * third;
@@ -119,7 +139,9 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
 void winding_line(float px, float py, float x0, float y0, float x1, float y1, uint32_t rule, float &w,
                   float &cr) {  // curves.rs:97-123
     bool a = y0 <= py, b = y1 <= py;
+    OPS(3, 0);
     if (a != b) {
+        OPS(7, 0);
         float t = (py - y0) / (y1 - y0);
         float xi = x0 + t * (x1 - x0);
         if (xi > px) {
@@ -134,11 +156,14 @@ void winding_quadratic(float px, float py, float x0, float y0, float x1, float y
     float ay = y0 - 2.0f * y1 + y2;
     float by = -2.0f * y0 + 2.0f * y1;
     float cy = y0 - py;
+    OPS(8, 0);
     float roots[2];
     int n = solve_quadratic(ay, by, cy, roots);
     for (int i = 0; i < n; ++i) {
         float t = roots[i];
+        OPS(2, 0);
         if (t >= 0.f && t <= 1.f) {
+            OPS(15, 0);
             float tt = 1.f - t;
             float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
             if (x > px) {
@@ -158,11 +183,14 @@ void winding_cubic(float px, float py, float x0, float y0, float x1, float y1, f
     float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
     float c = -3.0f * y0 + 3.0f * y1;
     float d = y0 - py;
+    OPS(15, 0);
     float roots[3];
     int n = solve_cubic(a, b, c, d, roots);
     for (int i = 0; i < n; ++i) {
         float t = roots[i];
+        OPS(2, 0);
         if (t >= 0.f && t <= 1.f) {
+            OPS(25, 0);
             float tt = 1.f - t;
             float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
             if (x > px) {
@@ -213,6 +241,7 @@ void closest_quadratic(float px, float py, float x0, float y0, float x1, float y
                        bool approx, float &dist, float &tout) {  // curves.rs:218-279
     float best_t = 0.f;
     float best = dist_pt(px, py, x0, y0);
+    OPS(10, 0);
     if (approx) {
         float ax, ay, at;
         quadratic_closest_pt_approx(px, py, x0, y0, x1, y1, x2, y2, ax, ay, at);
@@ -228,11 +257,14 @@ void closest_quadratic(float px, float py, float x0, float y0, float x1, float y
         float b = 3.0f * (ax * bx + ay * by);
         float c = 2.0f * (bx * bx + by * by) + (ax * cx + ay * cy);
         float d = bx * cx + by * cy;
+        OPS(28, 0);
         float roots[3];
         int n = solve_cubic(a, b, c, d, roots);
         for (int i = 0; i < n; ++i) {
             float t = roots[i];
+            OPS(2, 0);
             if (t >= 0.f && t <= 1.f) {
+                OPS(22, 0);
                 float tt = 1.f - t;
                 float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
                 float y = (tt * tt) * y0 + (2.0f * tt * t) * y1 + (t * t) * y2;
@@ -285,6 +317,7 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
     float best_t = 0.f;
     float best = dist_pt(px, py, x0, y0);
     float de = dist_pt(px, py, x3, y3);
+    OPS(30, 0);
     if (de < best) { best = de; best_t = 1.f; }
     float ax = -x0 + 3.0f * x1 - 3.0f * x2 + x3;
     float ay = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
@@ -306,6 +339,7 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
         float p1c = (4.0f / 5.0f) * e - (2.0f / 25.0f) * b * d;
         float p1d = f - b * e / 25.0f;
         float q_root = -b / 5.0f;
+        OPS(58, 0);
         float p_roots[3];
         int num_p = solve_cubic(p1a, p1b, p1c, p1d, p_roots);
         float iv[4];
@@ -327,6 +361,7 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
             if (idx < ni && iv[idx] < 0.f) { ++idx; continue; }
             float upper = (idx < ni) ? min_f(iv[idx], 1.f) : 1.f;
             float lb = lower, ub = upper;
+            OPS(24, 0);
             float lbe = eval_quintic(lb, b, c, d, e, f);
             float ube = eval_quintic(ub, b, c, d, e, f);
             if (lbe * ube > 0.f) {
@@ -340,6 +375,7 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
             }
             float t = (lb + ub) * 0.5f;
             for (uint32_t it = 0; it < 20; ++it) {
+                OPS(30, 0);
                 if (t < lb || t > ub) t = (lb + ub) * 0.5f;
                 float value = eval_quintic(t, b, c, d, e, f);
                 if (abs_f(value) < 1.0e-5f || it == 19) break;
@@ -347,7 +383,9 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
                 float der = eval_quintic_deriv(t, b, c, d, e);
                 t = t - value / der;
             }
+            OPS(2, 0);
             if (t >= 0.f && t <= 1.f) {
+                OPS(34, 0);
                 float tt = 1.f - t;
                 float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
                 float y = (tt * tt * tt) * y0 + (3.0f * tt * tt * t) * y1 + (3.0f * tt * t * t) * y2 + (t * t * t) * y3;
@@ -451,6 +489,7 @@ struct PreparedInstance {
     std::vector<Seg> segs; // curve segments in shape space
 };
 struct PreparedGroup {
+    int tile[4] = {1, 1, 0, 0};  // 16-px tile range of the canvas AABB (count mode)
     float xform[6];        // canvas_to_shape rows 0-1
     float inv_scale;
     std::vector<PreparedInstance> shapes;
@@ -523,6 +562,30 @@ static void prepare(const Scene &sc, const dvg_render_options &opt, std::vector<
                 pi.b[0] = 1.f; pi.b[1] = 1.f; pi.b[2] = 0.f; pi.b[3] = 0.f;
             }
             pg.shapes.push_back(pi);
+        }
+        // canvas AABB -> tile range, as bin_tiles_count (binning.rs:30-110)
+        {
+            float u[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
+            bool any = false;
+            for (const PreparedInstance &pi : pg.shapes)
+                if (pi.bounds_valid) {
+                    u[0] = rmin(u[0], pi.b[0]); u[1] = rmin(u[1], pi.b[1]);
+                    u[2] = rmax(u[2], pi.b[2]); u[3] = rmax(u[3], pi.b[3]);
+                    any = true;
+                }
+            if (any) {
+                const M3 &m = g.shape_to_canvas;
+                float xs[4], ys[4];
+                float cx[4] = {u[0], u[2], u[2], u[0]}, cy[4] = {u[1], u[1], u[3], u[3]};
+                for (int k = 0; k < 4; ++k) {
+                    xs[k] = m.m[0][0] * cx[k] + m.m[0][1] * cy[k] + m.m[0][2];
+                    ys[k] = m.m[1][0] * cx[k] + m.m[1][1] * cy[k] + m.m[1][2];
+                }
+                float mnx = std::min({xs[0], xs[1], xs[2], xs[3]}), mxx = std::max({xs[0], xs[1], xs[2], xs[3]});
+                float mny = std::min({ys[0], ys[1], ys[2], ys[3]}), mxy = std::max({ys[0], ys[1], ys[2], ys[3]});
+                pg.tile[0] = (int)std::floor(mnx / 16.f); pg.tile[1] = (int)std::floor(mny / 16.f);
+                pg.tile[2] = (int)std::floor(mxx / 16.f); pg.tile[3] = (int)std::floor(mxy / 16.f);
+            }
         }
         out.push_back(pg);
     }
@@ -653,6 +716,8 @@ static void blend_group(float out[4], const Paint &fill, const Paint &stroke, ui
             cov = 1.f;
         }
         if (cov > 0.f) {
+            OPS(12, 0);
+            if (g_counting) ++g_ops.frags;
             float a = cov * scol.w, inv = 1.f - a;
             out[0] = scol.x * a + out[0] * inv;
             out[1] = scol.y * a + out[1] * inv;
@@ -672,6 +737,8 @@ static void blend_group(float out[4], const Paint &fill, const Paint &stroke, ui
             cov = 1.f;
         }
         if (cov > 0.f) {
+            OPS(12, 0);
+            if (g_counting) ++g_ops.frags;
             float a = cov * fc.w, inv = 1.f - a;
             out[0] = fc.x * a + out[0] * inv;
             out[1] = fc.y * a + out[1] * inv;
@@ -689,9 +756,14 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
     out[1] = bg[1] * bg[3];
     out[2] = bg[2] * bg[3];
     out[3] = bg[3];
+    bool counting = g_counting;
+    int tx = (int)std::floor(px / 16.f), ty = (int)std::floor(py / 16.f);
     for (size_t gi = 0; gi < sc.groups.size(); ++gi) {
         const Group &g = sc.groups[gi];
         const PreparedGroup &pg = pgs[gi];
+        // count only the groups the reference's tile list holds for this tile
+        g_counting = counting && tx >= pg.tile[0] && tx <= pg.tile[2] && ty >= pg.tile[1] && ty <= pg.tile[3];
+        OPS(14, 0);
         const float *x = pg.xform;
         float lx = x[0] * px + x[1] * py + x[2];
         float ly = x[3] * px + x[4] * py + x[5];
@@ -702,9 +774,11 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
         if (g.fill.kind != DVG_PAINT_NONE || g.stroke.kind != DVG_PAINT_NONE) {
             for (const PreparedInstance &pi : pg.shapes) {
                 // accumulate_shape_in_group, forward.rs:1420-1503
+                OPS(6, 0);
                 bool inb = pi.b[0] <= pi.b[2] && pi.b[1] <= pi.b[3];
                 if (inb && (lx < pi.b[0] || lx > pi.b[2] || ly < pi.b[1] || ly > pi.b[3])) inb = false;
                 if (!inb) continue;
+                OPS(8, 0);
                 const float *iv = pi.inv;
                 float spx = iv[0] * lx + iv[1] * ly + iv[2];
                 float spy = iv[3] * lx + iv[4] * ly + iv[5];
@@ -713,9 +787,26 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
                 if (g.stroke.kind != DVG_PAINT_NONE) shape_stroke(sh, pi, prefilter, spx, spy, acc);
             }
         }
+        OPS(10, 0);
         blend_group(out, g.fill, g.stroke, g.fill_rule, fc, scol, acc.fill_min * gscale, acc.winding, acc.crossings,
                     acc.stroke_min * gscale, acc.stroke_radius * gscale, acc.stroke_hit, prefilter);
     }
+    g_counting = counting;
+}
+
+// Count mode: algorithmic operations of the scene evaluation at the points
+// pts (x0, y0, x1, y1, ...), with the reference's tile culling.
+OpCount count_eval_points(const Scene &sc, const dvg_render_options &opt, const float *pts, size_t n) {
+    std::vector<PreparedGroup> pgs;
+    prepare(sc, opt, pgs);
+    bool prefilter = opt.use_prefiltering != 0;
+    float bg[4] = {sc.background.x, sc.background.y, sc.background.z, sc.background.w};
+    g_ops = OpCount();
+    g_counting = true;
+    float col[4];
+    for (size_t i = 0; i < n; ++i) eval_scene(sc, pgs, prefilter, pts[2 * i], pts[2 * i + 1], bg, col);
+    g_counting = false;
+    return g_ops;
 }
 
 // Scene evaluation (no splat) of the samples [i0, i1), for CPU timing.

@@ -28,6 +28,7 @@ void jitter_xy(uint32_t x, uint32_t y, uint32_t sx, uint32_t sy, uint32_t width,
 
 // Full forward render; sample_colors (optional) receives the premultiplied
 // per-sample colours in (pixel, sy, sx) order.
+OpCount count_eval_points(const Scene &sc, const dvg_render_options &opt, const float *pts, size_t n);
 void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, size_t i1);
 int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_img, float *sample_colors,
                    const char **err);

@@ -157,3 +157,15 @@ def time_slice(desc, opts, d_render, seconds=15.0):
             "sample": f"{n} consecutive pixel-samples (indices {i0}..{i0 + n}) of the full workload, "
                       f"forward + interior backward + boundary sampling, 1 thread; the weight image "
                       f"({ws.value:.2f} s for all samples) is excluded"}
+
+
+def count_ops(desc, opts, i0, i1):
+    """Algorithmic op counts over samples [i0, i1) (see oracle_count_ops)."""
+    L = lib()
+    from pydvg import abi as _abi
+    L.oracle_count_ops.argtypes = [C.POINTER(_abi.SceneDescC), C.POINTER(_abi.RenderOptions), C.c_uint64,
+                                   C.c_uint64, C.POINTER(C.c_double)]
+    out = (C.c_double * 6)()
+    L.oracle_count_ops(desc.ref(), C.byref(opts.to_c()), i0, i1, out)
+    return {"fwd_flops": out[0], "fwd_sfu": out[1], "fragments": out[2], "bnd_flops": out[3], "bnd_sfu": out[4],
+            "bnd_samples": out[5], "samples": i1 - i0}
