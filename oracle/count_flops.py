@@ -14,7 +14,7 @@ and gradient scatter) are counted from their formulas:
   filter adjoint   9 taps x 25      = 225
   reverse adjoint  25 per fragment
   boundary point   100; boundary gradient scatter 150; contribution 12
-Usage: python profiles/count_flops.py  (CPU; about a minute)
+Usage: python oracle/count_flops.py  (CPU; about a minute)
 """
 import json
 import os
