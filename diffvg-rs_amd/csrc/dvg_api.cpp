@@ -525,7 +525,7 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
     }
     if (S.n_cdf > 0) {
-        size_t ns = npix * S.spp, nt = (size_t)S.tiles_x * S.tiles_y;
+        size_t ns = npix * S.spp, nt = dvgk::boundary_buckets(S);
         size_t need = 2 * ns + 3 * nt + 3;
         if (ctx->bwork_cap < need) {
             if (ctx->bwork) (void)hipFree(ctx->bwork);

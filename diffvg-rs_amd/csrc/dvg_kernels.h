@@ -60,8 +60,14 @@ void launch_forward(const DevScene &S, const float *weight, float *accum, hipStr
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                               hipStream_t stream);
+// Boundary samples are sorted by the 4x4 pixel block of their pixel, blocks
+// numbered tile-major (16 per tile), so a wave shares one tile group list and
+// a compact footprint.
+constexpr int BND_BLOCK = 4;
+constexpr int BND_BLOCKS_PER_TILE = (TILE / BND_BLOCK) * (TILE / BND_BLOCK);
+inline uint32_t boundary_buckets(const DevScene &S) { return S.tiles_x * S.tiles_y * BND_BLOCKS_PER_TILE; }
 // Work buffers of the boundary pass: keys/order hold one u32 per sample,
-// counts ntiles+1, offsets ntiles+2, cursor ntiles.
+// counts buckets+1, offsets buckets+2, cursor buckets.
 struct BoundaryWork {
     uint32_t *keys, *order, *counts, *offsets, *cursor;
 };

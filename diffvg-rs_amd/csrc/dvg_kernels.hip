@@ -449,14 +449,29 @@ __global__ void k_weights(DevScene S, float *__restrict__ weight) {
     weight[(size_t)y * S.W + x] = sum;
 }
 
+// The splat of a tile's samples lands in the tile plus a halo of ri pixels.
+// For ri <= SPLAT_HALO it is summed in LDS and flushed with one global atomic
+// per touched pixel channel; wider filters splat straight to global memory.
+constexpr int SPLAT_HALO = 4;
+constexpr int SPLAT_EDGE = TILE + 2 * SPLAT_HALO;
+
 __global__ __launch_bounds__(BLOCK) void k_forward(DevScene S, const float *__restrict__ weight,
                                                    float *__restrict__ accum) {
+    __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
     uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
     uint32_t total = TILE * TILE * S.spp;
     int ri = S.ri;
+    bool lds = ri <= SPLAT_HALO;
+    int edge = TILE + 2 * ri;
+    int x0 = (int)((tile % S.tiles_x) * TILE) - ri, y0 = (int)((tile / S.tiles_x) * TILE) - ri;
+    if (lds) {
+        for (int i = threadIdx.x; i < edge * edge; i += BLOCK)
+            for (int c = 0; c < 4; ++c) lacc[c][i] = 0.f;
+        __syncthreads();
+    }
     for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
         uint32_t x, y, sx, sy;
         tile_sample(S, tile, s, x, y, sx, sy);
@@ -475,14 +490,34 @@ __global__ __launch_bounds__(BLOCK) void k_forward(DevScene S, const float *__re
                                             S.filter_radius);
                     if (w > 0.f) {
                         float wn = w / ws;
-                        float *a = accum + base * 4u;
-                        unsafeAtomicAdd(a + 0, col.x * wn);
-                        unsafeAtomicAdd(a + 1, col.y * wn);
-                        unsafeAtomicAdd(a + 2, col.z * wn);
-                        unsafeAtomicAdd(a + 3, col.w * wn);
+                        if (lds) {
+                            int li = (yy - y0) * edge + (xx - x0);
+                            atomicAdd(&lacc[0][li], col.x * wn);
+                            atomicAdd(&lacc[1][li], col.y * wn);
+                            atomicAdd(&lacc[2][li], col.z * wn);
+                            atomicAdd(&lacc[3][li], col.w * wn);
+                        } else {
+                            float *a = accum + base * 4u;
+                            unsafeAtomicAdd(a + 0, col.x * wn);
+                            unsafeAtomicAdd(a + 1, col.y * wn);
+                            unsafeAtomicAdd(a + 2, col.z * wn);
+                            unsafeAtomicAdd(a + 3, col.w * wn);
+                        }
                     }
                 }
             }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < edge * edge; i += BLOCK) {
+            int xx = x0 + i % edge, yy = y0 + i / edge;
+            if (xx < 0 || yy < 0 || xx >= (int)S.W || yy >= (int)S.H) continue;
+            float *a = accum + ((size_t)yy * S.W + xx) * 4u;
+            for (int c = 0; c < 4; ++c) {
+                float v = lacc[c][i];
+                if (v != 0.f) unsafeAtomicAdd(a + c, v);
+            }
+        }
     }
 }
 
@@ -1226,36 +1261,48 @@ DVG_DEV bool boundary_point(const DevScene &S, size_t idx, BPoint &P) {
     return true;
 }
 
-DVG_DEV uint32_t tile_of(const DevScene &S, uint32_t x, uint32_t y) {
-    return (y / TILE) * S.tiles_x + x / TILE;
+DVG_DEV uint32_t bucket_of(const DevScene &S, uint32_t x, uint32_t y) {
+    uint32_t tile = (y / TILE) * S.tiles_x + x / TILE;
+    uint32_t blk = ((y % TILE) / BND_BLOCK) * (TILE / BND_BLOCK) + (x % TILE) / BND_BLOCK;
+    return tile * BND_BLOCKS_PER_TILE + blk;
 }
 
-// Pass 1: tile key of every boundary sample (ntiles when it contributes
-// nothing) and the per-tile histogram.
+// Pass 1: bucket key of every boundary sample (nb when it contributes
+// nothing) and the histogram of the contributing ones.
 __global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ counts) {
+                                                   uint32_t *__restrict__ counts, uint32_t nb) {
     size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
-    uint32_t ntiles = S.tiles_x * S.tiles_y;
     BPoint P;
-    uint32_t key = ntiles;
-    if (S.n_cdf > 0 && boundary_point(S, idx, P)) key = tile_of(S, P.pxi, P.pyi);
+    uint32_t key = nb;
+    if (S.n_cdf > 0 && boundary_point(S, idx, P)) key = bucket_of(S, P.pxi, P.pyi);
     keys[idx] = key;
-    atomicAdd(counts + key, 1u);
+    // samples without a contribution are not counted: a shared counter for
+    // them would serialise every such atomic on one address
+    if (key < nb) atomicAdd(counts + key, 1u);
 }
 
-// Pass 2: counting-sort scatter of sample indices by tile.
+// Pass 2: counting-sort scatter of sample indices by bucket.
 __global__ __launch_bounds__(BLOCK) void k_bnd_scatter(DevScene S, const uint32_t *__restrict__ keys,
                                                        const uint32_t *__restrict__ offsets,
-                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order,
+                                                       uint32_t nb) {
     size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
     uint32_t key = keys[idx];
-    if (key >= S.tiles_x * S.tiles_y) return;
+    if (key >= nb) return;
     uint32_t pos = offsets[key] + atomicAdd(cursor + key, 1u);
     order[pos] = (uint32_t)idx;
+}
+
+DVG_DEV float sel3(uint32_t i, float a, float b, float c) { return i == 0u ? a : (i == 1u ? b : c); }
+DVG_DEV float sel4(uint32_t i, float a, float b, float c, float d) {
+    return i < 2u ? (i == 0u ? a : b) : (i == 2u ? c : d);
+}
+DVG_DEV float sel8(uint32_t i, const float *r) {
+    return i < 4u ? sel4(i, r[0], r[1], r[2], r[3]) : sel4(i - 4u, r[4], r[5], r[6], r[7]);
 }
 
 // Sum over the lanes of the wave whose `mine` is set; every lane returns it.
@@ -1267,10 +1314,9 @@ DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0
 __global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__restrict__ weight,
                                                     const float *__restrict__ d_render, DevGrads G,
                                                     const uint32_t *__restrict__ offsets,
-                                                    const uint32_t *__restrict__ order) {
+                                                    const uint32_t *__restrict__ order, uint32_t nb) {
     const float PI = 3.14159274101257324219f;
-    uint32_t ntiles = S.tiles_x * S.tiles_y;
-    uint32_t n_valid = offsets[ntiles];
+    uint32_t n_valid = offsets[nb];
     uint32_t pos = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t lane = threadIdx.x & 63u;
     if (blockIdx.x * BLOCK >= n_valid) return;  // whole workgroup past the end
@@ -1379,22 +1425,32 @@ __global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__r
         bool mine = valid && key == lkey;
         float r[14];
         for (int k = 0; k < 14; ++k) r[k] = masked_wave_sum(mine, v[k]);
-        if ((int)lane == leader) {
-            M3 Gs = m3_affine(S.group_shape_xform + P.gi * 6u);
-            M3 T = m3_affine(S.shape_transform + P.si * 6u);
-            float *dgt = G.d_group_transform + P.gi * 9u;
-            float *dst = G.d_shape_transform + P.si * 9u;
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) {
-                    float a = r[i * 3] * T.m[j][0] + r[i * 3 + 1] * T.m[j][1] + r[i * 3 + 2] * T.m[j][2];
-                    float b = Gs.m[0][i] * r[j] + Gs.m[1][i] * r[3 + j] + Gs.m[2][i] * r[6 + j];
-                    unsafeAtomicAdd(dgt + i * 3 + j, a);
-                    unsafeAtomicAdd(dst + i * 3 + j, b);
-                }
-            if (r[9] != 0.f) unsafeAtomicAdd(G.d_shape_stroke_width + P.si, r[9]);
-            float *dp = G.d_shape_params + P.si * 8u;
-            for (int k = 0; k < 4; ++k)
-                if (r[10 + k] != 0.f) unsafeAtomicAdd(dp + k, r[10 + k]);
+        // the 23 outputs go out from 23 lanes in one instruction each
+        uint32_t lgi = __shfl(P.gi, leader, 64), lsi = __shfl(P.si, leader, 64);
+        if (lane < 18u) {
+            uint32_t e = lane < 9u ? lane : lane - 9u, i = e / 3u, j = e - 3u * i;
+            float val;
+            float *dst;
+            if (lane < 9u) {  // d_G[i][j] = sum_k dM[i][k] T[j][k]
+                const float *tr = S.shape_transform + lsi * 6u + j * 3u;
+                float t0 = j < 2u ? tr[0] : 0.f, t1 = j < 2u ? tr[1] : 0.f, t2 = j < 2u ? tr[2] : 1.f;
+                float m0 = sel3(i, r[0], r[3], r[6]), m1 = sel3(i, r[1], r[4], r[7]), m2 = sel3(i, r[2], r[5], r[8]);
+                val = m0 * t0 + m1 * t1 + m2 * t2;
+                dst = G.d_group_transform + lgi * 9u + e;
+            } else {  // d_T[i][j] = sum_k G[k][i] dM[k][j]
+                const float *gr = S.group_shape_xform + lgi * 6u;
+                float g0 = gr[i], g1 = gr[3u + i], g2 = i == 2u ? 1.f : 0.f;
+                float m0 = sel3(j, r[0], r[1], r[2]), m1 = sel3(j, r[3], r[4], r[5]), m2 = sel3(j, r[6], r[7], r[8]);
+                val = g0 * m0 + g1 * m1 + g2 * m2;
+                dst = G.d_shape_transform + lsi * 9u + e;
+            }
+            unsafeAtomicAdd(dst, val);
+        } else if (lane == 18u) {
+            if (r[9] != 0.f) unsafeAtomicAdd(G.d_shape_stroke_width + lsi, r[9]);
+        } else if (lane < 23u) {
+            uint32_t k = lane - 19u;
+            float val = sel4(k, r[10], r[11], r[12], r[13]);
+            if (val != 0.f) unsafeAtomicAdd(G.d_shape_params + lsi * 8u + k, val);
         }
         pending &= ~__ballot(mine);
     }
@@ -1425,18 +1481,16 @@ __global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__r
         bool mine = pvalid && pkey == lkey;
         float r[8];
         for (int k = 0; k < 8; ++k) r[k] = masked_wave_sum(mine, pv[k]);
-        if ((int)lane == leader) {
-            uint32_t np = S.sp_cnt[P.si];
-            bool closed = S.s_closed[P.si] != 0;
-            float *dpt = G.d_shape_points + (size_t)S.sp_off[P.si] * 2u;
-            for (uint32_t k = 0; k <= nc_ + 1; ++k) {
-                uint32_t i = P.bd.point_id + k;
-                uint32_t index = closed ? i % np : i;
-                if (index < np) {
-                    unsafeAtomicAdd(dpt + index * 2u, r[2 * k]);
-                    unsafeAtomicAdd(dpt + index * 2u + 1, r[2 * k + 1]);
-                }
-            }
+        uint32_t lsi = __shfl(P.si, leader, 64), lpid = __shfl(P.bd.point_id, leader, 64);
+        uint32_t lnc = __shfl(nc_, leader, 64);
+        uint32_t k = lane >> 1;
+        if (lane < 8u && k <= lnc + 1u) {
+            uint32_t np = S.sp_cnt[lsi];
+            bool closed = S.s_closed[lsi] != 0;
+            uint32_t i = lpid + k;
+            uint32_t index = closed ? i % np : i;
+            float val = sel8(lane, r);
+            if (index < np) unsafeAtomicAdd(G.d_shape_points + ((size_t)S.sp_off[lsi] + index) * 2u + (lane & 1u), val);
         }
         pending &= ~__ballot(mine);
     }
@@ -1505,22 +1559,23 @@ void launch_backward_interior(const DevScene &S, const float *weight, const floa
 }
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
-    uint32_t ntiles = S.tiles_x * S.tiles_y;
-    uint32_t nb = cdiv(total, BLOCK);
-    // counts[ntiles + 1] and cursor[ntiles] start at zero
+    uint32_t nbk = boundary_buckets(S);
+    uint32_t grid = cdiv(total, BLOCK);
+    // counts[nbk + 1] and cursor[nbk] start at zero
     float *zp[2] = {reinterpret_cast<float *>(bw.counts), reinterpret_cast<float *>(bw.cursor)};
-    uint32_t zn[2] = {ntiles + 1, ntiles};
+    uint32_t zn[2] = {nbk + 1, nbk};
     launch_zero(zp, zn, 2, stream);
-    hipLaunchKernelGGL(k_bnd_key, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.counts);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, ntiles + 1, bw.offsets);
-    hipLaunchKernelGGL(k_bnd_scatter, dim3(nb), dim3(BLOCK), 0, stream, S, bw.keys, bw.offsets, bw.cursor, bw.order);
+    hipLaunchKernelGGL(k_bnd_key, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.counts, nbk);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, nbk + 1, bw.offsets);
+    hipLaunchKernelGGL(k_bnd_scatter, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.offsets, bw.cursor, bw.order,
+                       nbk);
 }
 
 void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                           const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     hipLaunchKernelGGL(k_bnd_eval, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
-                       bw.offsets, bw.order);
+                       bw.offsets, bw.order, boundary_buckets(S));
 }
 
 }  // namespace dvgk

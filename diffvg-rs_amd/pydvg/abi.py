@@ -160,6 +160,15 @@ def load_library(path=None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise OSError(f"libdvg.so not built: {p} (run __graft_entry__.build())")
+    # One HIP runtime per process.  torch ships its own libamdhip64 with the
+    # same soname as the system one libdvg.so links; loading torch first makes
+    # the dynamic loader bind libdvg.so to torch's copy, so device pointers and
+    # streams pass between them.  Loaded the other way round, torch would map
+    # a second runtime and find no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
