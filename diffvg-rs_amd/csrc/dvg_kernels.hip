@@ -20,6 +20,21 @@
 
 namespace dvgk {
 
+// Waves per SIMD the heavy kernels are compiled for.  The scene walks are
+// VALU-issue bound with long memory waits between groups, so more resident
+// waves beat an unspilled register file: measured on the 1024x1024 /
+// 1000-path scene, 5 waves (<= 96 VGPRs) is best for the forward and the
+// boundary pass, 4 for the interior adjoint (its LDS allows no more).
+#ifndef OCC_FORWARD
+#define OCC_FORWARD 5
+#endif
+#ifndef OCC_INTERIOR
+#define OCC_INTERIOR 4
+#endif
+#ifndef OCC_BOUNDARY
+#define OCC_BOUNDARY 5
+#endif
+
 // ---------------------------------------------------------------------------
 // Tile binning: group g belongs to tile t iff t lies in g's clamped tile range
 // (binning.rs:30-110).  One lane per tile scans the groups in ascending order,
@@ -455,7 +470,7 @@ __global__ void k_weights(DevScene S, float *__restrict__ weight) {
 constexpr int SPLAT_HALO = 4;
 constexpr int SPLAT_EDGE = TILE + 2 * SPLAT_HALO;
 
-__global__ __launch_bounds__(BLOCK) void k_forward(DevScene S, const float *__restrict__ weight,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_forward(DevScene S, const float *__restrict__ weight,
                                                    float *__restrict__ accum) {
     __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
     uint32_t tile = blockIdx.x;
@@ -667,7 +682,8 @@ DVG_DEV void blend_cpu(float4 &p, float4 c) {
 
 // One pass over the tile's groups: counts fragments, composites them all into
 // `fin`, composites the first `start` into `pre`, and records the list
-// positions of fragments [start, start + FRAG_WINDOW) as (pos << 1 | stroke).
+// positions of fragments [start, start + FRAG_WINDOW) as (pos << 1 | stroke)
+// in ids[k * BLOCK] (the lane's column of an LDS table).
 DVG_DEV uint32_t walk_window(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t start,
                              uint32_t *ids, float4 &pre, float4 &fin) {
     uint32_t count = 0;
@@ -681,7 +697,7 @@ DVG_DEV uint32_t walk_window(const DevScene &S, const uint32_t *list, uint32_t n
             float4 c = frag_color(S, g, is_stroke, px, py);
             blend_cpu(fin, c);
             if (count < start) blend_cpu(pre, c);
-            else if (count < start + FRAG_WINDOW) ids[count - start] = (e << 1) | (is_stroke ? 1u : 0u);
+            else if (count < start + FRAG_WINDOW) ids[(count - start) * BLOCK] = (e << 1) | (is_stroke ? 1u : 0u);
             ++count;
         }
     }
@@ -780,10 +796,12 @@ DVG_DEV float wave_sum(float v) {
     return v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_backward_interior(DevScene S, const float *__restrict__ weight,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(DevScene S, const float *__restrict__ weight,
                                                              const float *__restrict__ d_render, DevGrads G) {
     __shared__ float s_paint[LDS_GROUP_CAP * 8];
     __shared__ float s_red[4][6];
+    __shared__ uint32_t s_ids[FRAG_WINDOW * BLOCK];
+    uint32_t *ids = s_ids + threadIdx.x;
     uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
@@ -807,7 +825,6 @@ __global__ __launch_bounds__(BLOCK) void k_backward_interior(DevScene S, const f
         float4 bgs = background_at(S, x, y);
         float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
         float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
-        uint32_t ids[FRAG_WINDOW];
         float4 pre = bg, fin = bg;
         uint32_t nf = walk_window(S, list, n, px, py, 0, ids, pre, fin);
         float4 col;
@@ -849,10 +866,10 @@ __global__ __launch_bounds__(BLOCK) void k_backward_interior(DevScene S, const f
                     // prefix colour before fragment i of this window
                     float4 pc = wpre;
                     for (int j = 0; j < i; ++j) {
-                        uint32_t id = ids[j];
+                        uint32_t id = ids[j * BLOCK];
                         blend_cpu(pc, frag_color(S, list[id >> 1], (id & 1u) != 0, px, py));
                     }
-                    uint32_t id = ids[i];
+                    uint32_t id = ids[i * BLOCK];
                     uint32_t e = id >> 1;
                     bool is_stroke = (id & 1u) != 0;
                     uint32_t g = list[e];
@@ -1091,21 +1108,26 @@ DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint
         group_frags(S, g, px, py, st, fi);
         if (g == eq_group) {
             // within_distance_edge / is_inside_edge (sampling.rs:485-549) on
-            // the sampled shape alone
+            // the sampled shape alone; for a group of one shape that is the
+            // group's own stroke / fill test above (bounds are conservative)
             const float *gd = S.group_data + g * 16u;
-            uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-            float lx, ly;
-            group_local(S, g, px, py, lx, ly);
-            const float *xf = S.shape_xform + eq_prep * 6u;
-            float spx = xf[0] * lx + xf[1] * ly + xf[2];
-            float spy = xf[3] * lx + xf[4] * ly + xf[5];
-            const float *sd = S.shape_data + eq_prep * 15u;
-            Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-            if (sk != DVG_PAINT_NONE) shape_stroke<true>(S, sd, false, spx, spy, c);
-            if (fk != DVG_PAINT_NONE) shape_fill(S, sd, rule, false, spx, spy, c);
-            if (c.stroke_hit > 0.f) edge_test = true;
-            if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
-                if (fk != DVG_PAINT_NONE) edge_test = true;
+            if ((uint32_t)gd[1] == 1u) {
+                if (st || fi) edge_test = true;
+            } else {
+                uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+                float lx, ly;
+                group_local(S, g, px, py, lx, ly);
+                const float *xf = S.shape_xform + eq_prep * 6u;
+                float spx = xf[0] * lx + xf[1] * ly + xf[2];
+                float spy = xf[3] * lx + xf[4] * ly + xf[5];
+                const float *sd = S.shape_data + eq_prep * 15u;
+                Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
+                if (sk != DVG_PAINT_NONE) shape_stroke<true>(S, sd, false, spx, spy, c);
+                if (fk != DVG_PAINT_NONE) shape_fill(S, sd, rule, false, spx, spy, c);
+                if (c.stroke_hit > 0.f) edge_test = true;
+                if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
+                    if (fk != DVG_PAINT_NONE) edge_test = true;
+                }
             }
         }
         for (int k = 0; k < 2; ++k) {
@@ -1311,11 +1333,12 @@ DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0
 // Pass 3: boundary samples in tile order.  Consecutive lanes sit on the same
 // few shapes of one tile, so the tile's group list is shared by the wave and
 // the shape gradients are reduced per shape instance before the atomics.
-__global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__restrict__ weight,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval(DevScene S, const float *__restrict__ weight,
                                                     const float *__restrict__ d_render, DevGrads G,
                                                     const uint32_t *__restrict__ offsets,
                                                     const uint32_t *__restrict__ order, uint32_t nb) {
     const float PI = 3.14159274101257324219f;
+    __shared__ BPoint s_pt[BLOCK];
     uint32_t n_valid = offsets[nb];
     uint32_t pos = blockIdx.x * BLOCK + threadIdx.x;
     uint32_t lane = threadIdx.x & 63u;
@@ -1332,8 +1355,17 @@ __global__ __launch_bounds__(BLOCK) void k_bnd_eval(DevScene S, const float *__r
             float4 bgs = background_at(S, P.pxi, P.pyi);
             float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
             bool ih, oh;
-            float4 cin = edge_color(S, P.qx - P.ncx * 1.0e-4f, P.qy - P.ncy * 1.0e-4f, bg, P.gi, P.prep, ih);
-            float4 cout = edge_color(S, P.qx + P.ncx * 1.0e-4f, P.qy + P.ncy * 1.0e-4f, bg, P.gi, P.prep, oh);
+            float qix = P.qx - P.ncx * 1.0e-4f, qiy = P.qy - P.ncy * 1.0e-4f;
+            float qox = P.qx + P.ncx * 1.0e-4f, qoy = P.qy + P.ncy * 1.0e-4f;
+            uint32_t gi = P.gi, prep = P.prep;
+            // the point waits in LDS, not in registers, during the two scene
+            // walks (the empty asm is a compiler barrier: P is reloaded)
+            s_pt[threadIdx.x] = P;
+            asm volatile("" ::: "memory");
+            float4 cin = edge_color(S, qix, qiy, bg, gi, prep, ih);
+            float4 cout = edge_color(S, qox, qoy, bg, gi, prep, oh);
+            asm volatile("" ::: "memory");
+            P = s_pt[threadIdx.x];
             valid = ih || oh;
             nnx = P.ncx;
             nny = P.ncy;
