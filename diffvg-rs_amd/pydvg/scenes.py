@@ -150,29 +150,36 @@ def config2(seed=0):
     return sc, opts
 
 
-def config3(seed=0):
+def prefiltered_scene(width, height, n_strokes, n_polygons, seed, step=20.0, radius=(5.0, 80.0),
+                      seg_range=(4, 16)):
+    """Stroked open cubic paths (random walk control points, width U[0.5, 4])
+    and filled polygons of 3-12 vertices on a circle, drawn in that order."""
     rng = np.random.default_rng(seed)
-    W = H = 1024
+    W, H = width, height
     sc = Scene(W, H, background=WHITE, filter=Filter(abi.FILTER_BOX, 0.5))
-    for i in range(2000):
-        n_seg = int(rng.integers(4, 17))
+    for i in range(n_strokes):
+        n_seg = int(rng.integers(seg_range[0], seg_range[1] + 1))
         start = rng.uniform(0, 1, size=2) * (W, H)
-        steps = rng.normal(0.0, 20.0, size=(3 * n_seg, 2))
+        steps = rng.normal(0.0, step, size=(3 * n_seg, 2))
         pts = np.vstack([start, start + np.cumsum(steps, axis=0)]).astype(np.float32)
         sc.shapes.append(Shape(Path(np.full(n_seg, 2, np.uint8), pts, is_closed=False),
                                stroke_width=float(rng.uniform(0.5, 4.0))))
         sc.groups.append(ShapeGroup([i], fill=None, stroke=Solid(_rgba(rng))))
-    for j in range(500):
+    for j in range(n_polygons):
         n = int(rng.integers(3, 13))
         c = rng.uniform(0, 1, size=2) * (W, H)
-        rad = rng.uniform(5, 80)
+        rad = rng.uniform(*radius)
         ang = np.sort(rng.uniform(0, 2 * np.pi, size=n))
         pts = (c + rad * np.stack([np.cos(ang), np.sin(ang)], axis=1)).astype(np.float32)
-        idx = 2000 + j
+        idx = n_strokes + j
         sc.shapes.append(Shape(Path(np.zeros(n, np.uint8), pts, is_closed=True)))
         sc.groups.append(ShapeGroup([idx], fill=Solid(_rgba(rng))))
     opts = RenderOptions(samples_x=1, samples_y=1, seed=0, jitter=False, use_prefiltering=True)
     return sc, opts
+
+
+def config3(seed=0):
+    return prefiltered_scene(1024, 1024, 2000, 500, seed)
 
 
 def config4(seed=0, n_scenes=64):

@@ -180,6 +180,33 @@ void oracle_count_ops(const dvg_scene_desc *desc, const dvg_render_options *opt,
     out[3] = (double)b.flops; out[4] = (double)b.sfu; out[5] = (double)(bp.size() / 4);
 }
 
+// The scalar whose gradient render_backward returns for a prefiltered scene:
+// J = sum over samples of dot(colour, gather_d_color) with the reference CPU
+// prefiltered colour (sampling.rs:227-482, no adjoint).  J is linear in the
+// per-sample colours with weights independent of the geometry, so central
+// differences of J pin the geometry and paint gradients.
+double oracle_prefiltered_objective(const dvg_scene_desc *desc, const dvg_render_options *opt,
+                                    const float *d_render) {
+    Scene sc = read_scene(desc);
+    Ctx cx;
+    make_ctx(sc, cx);
+    std::vector<float> wimg = weight_image(sc, *opt);
+    uint32_t nsx = std::max(opt->samples_x, 1u), nsy = std::max(opt->samples_y, 1u);
+    size_t W = sc.width, H = sc.height;
+    double J = 0.0;
+    for (size_t idx = 0; idx < W * H * nsx * nsy; ++idx) {
+        size_t sx = idx % nsx, sy = (idx / nsx) % nsy, x = (idx / (nsx * nsy)) % W, y = idx / ((size_t)nsx * nsy * W);
+        float px = (float)x + ((float)sx + 0.5f) / (float)nsx;
+        float py = (float)y + ((float)sy + 0.5f) / (float)nsy;
+        V2 npt(px / (float)W, py / (float)H);
+        V4 bg = cpu::sample_background(sc, y * W + x);
+        V4 dc = cpu::gather_d_color(sc, d_render, wimg, V2(px, py));
+        V4 c = cpu::sample_color_prefiltered(cx, npt, &bg, nullptr, nullptr, -1, y * W + x);
+        J += (double)c.x * dc.x + (double)c.y * dc.y + (double)c.z * dc.z + (double)c.w * dc.w;
+    }
+    return J;
+}
+
 float oracle_filter_weight(uint32_t type, float radius, float dx, float dy) {
     return cpu::compute_filter_weight(type, radius, dx, dy);
 }

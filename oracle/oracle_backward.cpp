@@ -476,7 +476,7 @@ bool within_distance_shape(const Ctx &cx, uint32_t si, V2 pt, float r) {
 
 // shape_bounds (distance/shape.rs:9-105): geometry bounds under the shape
 // transform, inflated by the max stroke radius.
-static bool cpu_shape_bounds(const Shape &sh, float b[4]) {
+bool cpu_shape_bounds(const Shape &sh, float b[4]) {
     const M3 &T = sh.transform;
     V2 mn, mx;
     if (sh.kind == DVG_SHAPE_CIRCLE || sh.kind == DVG_SHAPE_ELLIPSE) {
@@ -593,7 +593,7 @@ V4 paint_color(const Paint &p, V2 pt) {  // paint.rs:8-74
     return p.stops[ns - 1].color;
 }
 
-static V2 d_length(V2 v, float d_l) {  // backward/math.rs:33-41
+V2 d_length(V2 v, float d_l) {  // backward/math.rs:33-41
     float l = sqrt_f(v.dot(v));
     if (l == 0.f) return V2();
     float d_l_sq = 0.5f * d_l / l;
@@ -831,7 +831,7 @@ V4 sample_background(const Scene &sc, size_t pix) {  // background.rs:7-26
               sc.background.w);
 }
 
-static void accumulate_background_grad(const Scene &sc, Grads &gr, size_t pix, V4 d) {  // background.rs:64-91
+void accumulate_background_grad(const Scene &sc, Grads &gr, size_t pix, V4 d) {  // background.rs:64-91
     if (sc.has_bg_image) {
         if (!gr.d_background_image.empty()) {
             size_t b = pix * 4;
@@ -861,7 +861,7 @@ void finalize_background_gradients(const Scene &sc, Grads &gr) {  // background.
     }
 }
 
-static void accumulate_translation(Grads &gr, long pix, V2 d) {  // sampling.rs:552-563
+void accumulate_translation(Grads &gr, long pix, V2 d) {  // sampling.rs:552-563
     if (pix < 0 || gr.d_translation.empty()) return;
     gr.d_translation[(size_t)pix * 2] += d.x;
     gr.d_translation[(size_t)pix * 2 + 1] += d.y;
@@ -1118,7 +1118,7 @@ static void sample_cdf(const std::vector<float> &cdf, float u, size_t &idx_out, 
     idx_out = idx;
 }
 
-static V2 normalize(V2 v) {  // backward/math.rs:44-51
+V2 normalize(V2 v) {  // backward/math.rs:44-51
     float len = v.length();
     if (len == 0.f) return V2();
     return v / len;
@@ -1341,7 +1341,7 @@ static bool sample_boundary_point(const Scene &sc, uint32_t si, uint32_t gi, flo
     return true;
 }
 
-static void d_xform_pt(const M3 &m, V2 pt, V2 d_out, M3 &d_m, V2 &d_pt) {  // backward/math.rs:101-120
+void d_xform_pt(const M3 &m, V2 pt, V2 d_out, M3 &d_m, V2 &d_pt) {  // backward/math.rs:101-120
     float t0 = m.m[0][0] * pt.x + m.m[0][1] * pt.y + m.m[0][2];
     float t1 = m.m[1][0] * pt.x + m.m[1][1] * pt.y + m.m[1][2];
     float t2 = m.m[2][0] * pt.x + m.m[2][1] * pt.y + m.m[2][2];
@@ -1596,7 +1596,8 @@ void backward_samples(const Ctx &cx, const dvg_render_options &opt, bool compute
         V4 bg = sample_background(sc, pix);
         V4 dc = gather_d_color(sc, d_render, wimg, pt);
         long tidx = compute_translation ? (long)pix : -1;
-        V4 color = sample_color(cx, npt, &bg, &dc, nullptr, &gr, tidx, pix);
+        V4 color = opt.use_prefiltering ? sample_color_prefiltered(cx, npt, &bg, &dc, &gr, tidx, pix)
+                                        : sample_color(cx, npt, &bg, &dc, nullptr, &gr, tidx, pix);
         accumulate_filter_gradient(sc, color, d_render, wimg, pt, gr.d_filter_radius[0]);
     }
 }
@@ -1624,17 +1625,15 @@ int render_backward(const Scene &sc, const dvg_render_options &opt, bool compute
         return DVG_INVALID_SCENE;
     }
     if (sc.width == 0 || sc.height == 0) return DVG_OK;
-    if (opt.use_prefiltering) {
-        *err = "prefiltered backward is not restated in this oracle";
-        return DVG_INVALID_SCENE;
-    }
     uint32_t nsx = std::max(opt.samples_x, 1u), nsy = std::max(opt.samples_y, 1u);
     size_t total = (size_t)sc.width * sc.height * nsx * nsy;
     Ctx cx;
     make_ctx(sc, cx);
     std::vector<float> wimg = weight_image(sc, opt);
     backward_samples(cx, opt, compute_translation, d_render, wimg, gr, 0, total);
-    boundary_sampling(cx, nsx, nsy, opt.seed, d_render, wimg, gr, compute_translation, 0, total);
+    // boundary sampling only without prefiltering (render.rs:192-206)
+    if (!opt.use_prefiltering)
+        boundary_sampling(cx, nsx, nsy, opt.seed, d_render, wimg, gr, compute_translation, 0, total);
     finalize_background_gradients(sc, gr);
     return DVG_OK;
 }

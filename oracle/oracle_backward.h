@@ -45,6 +45,23 @@ void d_sample_paint(const Paint &p, Grads &gr, float *d_paint_solid, V2 pt, V4 d
 float compute_filter_weight(uint32_t type, float radius, float dx, float dy);
 V4 sample_background(const Scene &sc, size_t pix);
 void build_group_bounds(Ctx &cx);
+bool cpu_shape_bounds(const Shape &sh, float b[4]);
+V2 d_length(V2 v, float d_l);
+V2 normalize(V2 v);
+void d_xform_pt(const M3 &m, V2 pt, V2 d_out, M3 &d_m, V2 &d_pt);
+void accumulate_background_grad(const Scene &sc, Grads &gr, size_t pix, V4 d);
+void accumulate_translation(Grads &gr, long pix, V2 d);
+std::vector<float> build_weight_image(const Scene &sc, uint32_t nsx, uint32_t nsy, uint32_t seed, bool prefilter,
+                                      bool jitter);
+V4 gather_d_color(const Scene &sc, const float *d_render, const std::vector<float> &wimg, V2 pt);
+void accumulate_filter_gradient(const Scene &sc, V4 color, const float *d_render, const std::vector<float> &wimg,
+                                V2 pt, float &d_radius);
+void finalize_background_gradients(const Scene &sc, Grads &gr);
+
+// Prefiltered colour of one sample with its adjoint (sampling.rs:227-482);
+// gr == nullptr skips the adjoint.  Defined in oracle_prefilter.cpp.
+V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 *d_color, Grads *gr, long tidx,
+                            size_t pix);
 void boundary_query_points(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, size_t i0, size_t i1,
                            std::vector<float> &pts);
 void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, const float *d_render,

@@ -46,6 +46,8 @@ def lib():
         L.oracle_is_inside.argtypes = [C.POINTER(abi.SceneDescC), C.c_uint32, C.c_float, C.c_float]
         L.oracle_filter_weight.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_float]
         L.oracle_filter_weight.restype = C.c_float
+        L.oracle_prefiltered_objective.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p]
+        L.oracle_prefiltered_objective.restype = C.c_double
         _lib = L
     return _lib
 
@@ -65,6 +67,13 @@ def render(desc, opts, want_samples=False):
     if st != 0:
         raise RuntimeError(f"oracle_render status {st}")
     return (out, samples) if want_samples else out
+
+
+def prefiltered_objective(desc, opts, d_render):
+    """sum_samples dot(CPU prefiltered colour, gather_d_color): the scalar whose
+    gradient render_backward returns for a prefiltered scene."""
+    d_render = np.ascontiguousarray(d_render, np.float32)
+    return float(lib().oracle_prefiltered_objective(desc.ref(), C.byref(opts.to_c()), _fp(d_render)))
 
 
 def render_backward(desc, opts, d_render, translation=False):
