@@ -223,6 +223,77 @@ void dvg_prepared_pack(const dvg_prepared *p, dvg_scene_pack *pack, dvg_backward
     dvgh::pack_views(*p, pack, bpack, params);
 }
 
+// Root box of the reference CPU group BVH for every group, in group-local
+// space: the union of shape_bounds (distance/shape.rs:10-91) over the group's
+// shapes -- geometry bounds under the shape transform, inflated by the
+// largest stroke radius.  min > max for a group with none.  Without the
+// backward tables (forward-only scenes) every box is "none".
+static std::vector<float> group_cpu_bounds(const dvg_scene_pack *pk, const dvg_backward_pack *bp) {
+    uint32_t G = pk->num_groups;
+    std::vector<float> out((size_t)std::max(G, 1u) * 4);
+    for (uint32_t g = 0; g < std::max(G, 1u); ++g) {
+        out[4 * g] = 1.f; out[4 * g + 1] = 1.f; out[4 * g + 2] = 0.f; out[4 * g + 3] = 0.f;
+    }
+    if (!bp || !bp->prepared_to_scene || !bp->shape_transform) return out;
+    for (uint32_t g = 0; g < G; ++g) {
+        const float *gd = pk->group_data + 16 * (size_t)g;
+        uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+        float u[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
+        bool any = false;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint32_t prep = (uint32_t)pk->group_shapes[off + i];
+            uint32_t si = bp->prepared_to_scene[prep];
+            const float *sd = pk->shape_data + 15 * (size_t)prep;
+            const float *T = bp->shape_transform + 6 * (size_t)si;
+            auto tx = [&](float x, float y) { return T[0] * x + T[1] * y + T[2]; };
+            auto ty = [&](float x, float y) { return T[3] * x + T[4] * y + T[5]; };
+            uint32_t kind = (uint32_t)sd[0];
+            float mnx, mny, mxx, mxy;
+            if (kind == DVG_SHAPE_CIRCLE || kind == DVG_SHAPE_ELLIPSE) {
+                float rx = std::fabs(sd[6]), ry = kind == DVG_SHAPE_CIRCLE ? std::fabs(sd[6]) : std::fabs(sd[7]);
+                float cx = tx(sd[4], sd[5]), cy = ty(sd[4], sd[5]);
+                float ex = std::sqrt((T[0] * rx) * (T[0] * rx) + (T[1] * ry) * (T[1] * ry));
+                float ey = std::sqrt((T[3] * rx) * (T[3] * rx) + (T[4] * ry) * (T[4] * ry));
+                mnx = cx - ex; mny = cy - ey; mxx = cx + ex; mxy = cy + ey;
+            } else if (kind == DVG_SHAPE_RECT) {
+                float xs[4] = {sd[4], sd[6], sd[6], sd[4]}, ys[4] = {sd[5], sd[5], sd[7], sd[7]};
+                mnx = mny = INFINITY;
+                mxx = mxy = -INFINITY;
+                for (int k = 0; k < 4; ++k) {
+                    float a = tx(xs[k], ys[k]), b = ty(xs[k], ys[k]);
+                    mnx = std::fmin(mnx, a); mny = std::fmin(mny, b); mxx = std::fmax(mxx, a); mxy = std::fmax(mxy, b);
+                }
+            } else {
+                uint32_t np = bp->shape_path_point_counts[si];
+                if (np == 0) continue;
+                const float *pts = bp->path_points + 2 * (size_t)bp->shape_path_offsets[si];
+                mnx = mny = INFINITY;
+                mxx = mxy = -INFINITY;
+                for (uint32_t k = 0; k < np; ++k) {
+                    float a = tx(pts[2 * k], pts[2 * k + 1]), b = ty(pts[2 * k], pts[2 * k + 1]);
+                    mnx = std::fmin(mnx, a); mny = std::fmin(mny, b); mxx = std::fmax(mxx, a); mxy = std::fmax(mxy, b);
+                }
+            }
+            // max_stroke_radius (distance/shape.rs:62-79)
+            float scale = std::fmax(std::fmax(std::sqrt(T[0] * T[0] + T[3] * T[3]), std::sqrt(T[1] * T[1] + T[4] * T[4])), 0.f);
+            float r = std::fabs(sd[3]) * scale;
+            if (kind == DVG_SHAPE_PATH && bp->shape_path_thickness_counts[si] > 0) {
+                const float *th = bp->path_thickness + bp->shape_path_thickness_offsets[si];
+                float mt = 0.f;
+                for (uint32_t k = 0; k < bp->shape_path_thickness_counts[si]; ++k) mt = std::fmax(mt, std::fabs(th[k]));
+                r = std::fmax(r, mt * scale);
+            }
+            if (r > 0.f && (uint32_t)sd[9] == DVG_JOIN_MITER) r *= std::fmax(sd[11], 1.0f);
+            float pad = std::fmax(r, 0.f);
+            u[0] = std::fmin(u[0], mnx - pad); u[1] = std::fmin(u[1], mny - pad);
+            u[2] = std::fmax(u[2], mxx + pad); u[3] = std::fmax(u[3], mxy + pad);
+            any = true;
+        }
+        if (any) std::memcpy(&out[4 * (size_t)g], u, sizeof(u));
+    }
+    return out;
+}
+
 int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backward_pack *bp,
                                 const dvg_render_params *prm, dvg_device_scene **out) {
     if (!ctx || !pk || !prm || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
@@ -255,6 +326,11 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         float mxx = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
         float mny = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
         float mxy = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
+        if (prm->use_prefiltering) {
+            // the prefiltered adjoint reaches one canvas pixel past a boundary
+            // (smoothstep band); pad in canvas space, whatever the scaling
+            mnx -= 1.0f; mny -= 1.0f; mxx += 1.0f; mxy += 1.0f;
+        }
         if (mxx < 0.f || mxy < 0.f || mnx >= (float)W || mny >= (float)H) continue;
         auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
         int tx0 = clampi((int)std::floor(mnx / (float)dvgk::TILE), (int)tiles_x - 1);
@@ -271,6 +347,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     std::vector<uint32_t> s2p(std::max(S, 1u), 0);
     if (bp && bp->prepared_to_scene)
         for (uint32_t i = SP; i-- > 0;) s2p[bp->prepared_to_scene[i]] = i;
+    std::vector<float> cpu_bounds = group_cpu_bounds(pk, bp);
 
     Arena A;
     auto addv = [&](const void *p, size_t bytes) { return A.add(p, p ? bytes : 0); };
@@ -331,6 +408,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         i_ppo = addv(bp->path_point_offsets, n_cdf ? (size_t)S * 4 : 0);
         i_s2p = addv(s2p.data(), (size_t)std::max(S, 1u) * 4);
     }
+    size_t i_gcb = addv(cpu_bounds.data(), cpu_bounds.size() * 4);
 
     dvg_device_scene *ds = new dvg_device_scene();
     hipError_t e = hipMalloc(&ds->arena, A.total);
@@ -409,6 +487,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         D.path_point_off = (const uint32_t *)at(i_ppo);
         D.scene_to_prepared = (const uint32_t *)at(i_s2p);
     }
+    D.group_cpu_bounds = (const float *)at(i_gcb);
     ds->num_groups = G;
     ds->num_shapes = S;
     {
@@ -504,7 +583,6 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
                                    const dvg_grads *grads) {
     if (!ctx || !ds || !d_render || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
     const dvgk::DevScene &S = ds->S;
-    if (S.prefilter) return set_err(ctx, DVG_INVALID_SCENE, "prefiltered backward is not available in this build");
     size_t npix = (size_t)S.W * S.H;
     if (npix == 0 || S.num_groups == 0) return DVG_OK;
     int st = ensure(ctx, ctx->weight, ctx->weight_cap, npix);
@@ -519,6 +597,12 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
     {
         KernelTimer kt(ctx, DVG_KERNEL_WEIGHTS);
         dvgk::launch_weights(S, ctx->weight, true, ctx->stream);
+    }
+    if (S.prefilter) {
+        // analytic coverage: no boundary term (backward/render.rs:192-206)
+        KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
+        dvgk::launch_backward_prefiltered(S, ctx->weight, d_render, G, ctx->stream);
+        return DVG_OK;
     }
     {
         KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
@@ -612,8 +696,6 @@ int render_backward_host(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backw
         if (dst && L.len[k]) std::memset(dst, 0, L.len[k] * sizeof(float));
     }
     if (npix == 0 || pk->num_groups == 0) return DVG_OK;
-    if (prm->use_prefiltering)
-        return set_err(ctx, DVG_INVALID_SCENE, "prefiltered backward is not available in this build");
     dvg_device_scene *ds = nullptr;
     int st = dvg_device_scene_create(ctx, pk, bp, prm, &ds);
     if (st) return st;

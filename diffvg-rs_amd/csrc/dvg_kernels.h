@@ -40,6 +40,9 @@ struct DevScene {
     const float *path_cdf, *path_pmf;
     const uint32_t *path_point_ids, *path_cdf_off, *path_cdf_cnt, *path_point_off;
     const uint32_t *scene_to_prepared;  // S: first prepared instance of each scene shape
+    // 4 per group, group-local: the root box of the reference's CPU group BVH
+    // (union of distance/shape.rs shape_bounds); min > max when it has none
+    const float *group_cpu_bounds;
 };
 
 struct DevGrads {
@@ -60,6 +63,9 @@ void launch_forward(const DevScene &S, const float *weight, float *accum, hipStr
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                               hipStream_t stream);
+// Prefiltered (analytic coverage) colour adjoint with its distance adjoint.
+void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                                 hipStream_t stream);
 // Boundary samples are sorted by the 4x4 pixel block of their pixel, blocks
 // numbered tile-major (16 per tile), so a wave shares one tile group list and
 // a compact footprint.

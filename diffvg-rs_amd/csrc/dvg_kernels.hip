@@ -351,10 +351,28 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
 // accumulate_group_shapes + accumulate_shape_in_group (forward.rs:1295-1503):
 // every shape instance of group g whose padded bounds hold the group-local
 // point contributes; (lx, ly) is the group-local point.
+//
+// With CPU = true the fill follows is_inside_bvh (distance/api.rs:259-309):
+// the group's root box gates the test and then every shape's winding counts
+// (an open path can wind outside its own box).
 template <bool CPU>
-DVG_DEV void group_cov(const DevScene &S, const float *gd, float lx, float ly, bool do_fill, bool do_stroke,
-                       uint32_t rule, bool prefilter, Cov &c) {
+DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx, float ly, bool do_fill,
+                       bool do_stroke, uint32_t rule, bool prefilter, Cov &c) {
     uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    if (CPU && do_fill) {
+        const float *rb = S.group_cpu_bounds + g * 4u;
+        bool has = rb[0] <= rb[2] && rb[1] <= rb[3];
+        if (!has || (lx >= rb[0] && lx <= rb[2] && ly >= rb[1] && ly <= rb[3])) {
+            for (uint32_t i = 0; i < cnt; ++i) {
+                uint32_t si = (uint32_t)S.group_shapes[off + i];
+                const float *xf = S.shape_xform + si * 6u;
+                float spx = xf[0] * lx + xf[1] * ly + xf[2];
+                float spy = xf[3] * lx + xf[4] * ly + xf[5];
+                shape_fill(S, S.shape_data + si * 15u, rule, false, spx, spy, c);
+            }
+        }
+        do_fill = false;
+    }
     for (uint32_t i = 0; i < cnt; ++i) {
         uint32_t si = (uint32_t)S.group_shapes[off + i];
         const float *b = S.shape_bounds + si * 4u;
@@ -396,7 +414,7 @@ DVG_DEV float4 eval_forward(const DevScene &S, const uint32_t *list, uint32_t n,
         float lx, ly;
         group_local(S, g, px, py, lx, ly);
         Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-        group_cov<false>(S, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
+        group_cov<false>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
         float inv_scale = S.group_inv_scale[g];
         float gscale = inv_scale > 0.f ? 1.f / inv_scale : 1.f;
         // blend_group, forward.rs:339-404: stroke first, then fill
@@ -662,7 +680,7 @@ DVG_DEV void group_frags(const DevScene &S, uint32_t g, float px, float py, bool
     float lx, ly;
     group_local(S, g, px, py, lx, ly);
     Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-    group_cov<true>(S, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, false, c);
+    group_cov<true>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, false, c);
     stroke = sk != DVG_PAINT_NONE && c.stroke_hit > 0.f;
     fill = fk != DVG_PAINT_NONE && ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f);
 }
@@ -935,6 +953,830 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
     }
 }
 
+// 3x3 helpers on affine matrices given by their first two rows.
+struct M3 {
+    float m[3][3];
+};
+DVG_DEV M3 m3_affine(const float *r) {
+    M3 o;
+    o.m[0][0] = r[0]; o.m[0][1] = r[1]; o.m[0][2] = r[2];
+    o.m[1][0] = r[3]; o.m[1][1] = r[4]; o.m[1][2] = r[5];
+    o.m[2][0] = 0.f; o.m[2][1] = 0.f; o.m[2][2] = 1.f;
+    return o;
+}
+DVG_DEV M3 m3_mul(const M3 &a, const M3 &b) {  // math.rs:185-195
+    M3 o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Prefiltered backward: the reference CPU sample_color_prefiltered
+// (backward/sampling.rs:227-482) with its distance adjoint
+// (backward/distance.rs:50-561).  Fragment coverage comes from the distance to
+// the closest boundary point of each group (distance/api.rs:138-232).
+//
+// Closest-point queries skip shapes and path segments whose bounds lie
+// farther than the best distance so far (or than the query radius): bounds
+// contain the geometry, so the result is the one of the full scan.  A stroke
+// fragment needs |d| < stroke width + 1 and a fill fragment |d| <= 1 or an
+// inside point, so a group whose padded bounds miss the point contributes
+// nothing and the tile lists (padded by one pixel for this mode) cull exactly.
+
+constexpr int PF_WINDOW = 16;  // fragments recorded per lane before a re-walk
+
+DVG_DEV float box_dist(float x0, float y0, float x1, float y1, float px, float py) {  // distance/bvh.rs:104-108
+    float dx = maxf_(maxf_(x0 - px, 0.f), px - x1);
+    float dy = maxf_(maxf_(y0 - py, 0.f), py - y1);
+    return sqrtf(dx * dx + dy * dy);
+}
+
+// Smallest singular value of the 2x2 matrix [a b; c d].
+DVG_DEV float min_scale2(float a, float b, float c, float d) {
+    float q = a * a + b * b + c * c + d * d, det = a * d - b * c;
+    float disc = sqrtf(fmaxf(q * q - 4.0f * det * det, 0.f));
+    // a little under the exact value, so the pruning stays conservative
+    return sqrtf(fmaxf(0.5f * (q - disc), 0.f)) * 0.999f;
+}
+
+struct PHit {
+    float cx, cy, dist, t;  // canvas closest point, canvas distance, path parameter
+    uint32_t prep, seg;     // prepared shape instance, segment index
+    bool path;
+};
+
+// Closest point of prepared shape `prep` to the shape-local point (px, py),
+// shape-local (closest_point_shape, closest.rs:364-456; closest_point_path_bvh,
+// closest.rs:100-221).  `cut` bounds the useful distance (shape-local) for
+// the segment pruning; the returned point is the true closest point whenever
+// it lies within `cut`.
+DVG_DEV bool closest_shape(const DevScene &S, uint32_t prep, float px, float py, float cut, float &ox, float &oy,
+                           uint32_t &seg, float &tt, bool &is_path) {
+    const float *sd = S.shape_data + prep * 15u;
+    uint32_t kind = (uint32_t)sd[0];
+    is_path = false;
+    seg = 0;
+    tt = 0.f;
+    if (kind == DVG_SHAPE_CIRCLE) {  // shape.rs:107-114
+        float cx = sd[4], cy = sd[5], r = sd[6];
+        float dx = px - cx, dy = py - cy;
+        float len = sqrtf(dx * dx + dy * dy);
+        if (len <= 1.0e-8f) { ox = cx + r; oy = cy; }
+        else { ox = cx + dx * (r / len); oy = cy + dy * (r / len); }
+        return true;
+    }
+    if (kind == DVG_SHAPE_ELLIPSE) {
+        ellipse_closest(sd[4], sd[5], sd[6], sd[7], px, py, ox, oy);
+        return true;
+    }
+    if (kind == DVG_SHAPE_RECT) {  // shape.rs:167-185
+        float x0 = sd[4], y0 = sd[5], x1 = sd[6], y1 = sd[7];
+        bool inside = px >= x0 && px <= x1 && py >= y0 && py <= y1;
+        if (!inside) {  // f32::clamp
+            ox = px;
+            if (ox < x0) ox = x0;
+            if (ox > x1) ox = x1;
+            oy = py;
+            if (oy < y0) oy = y0;
+            if (oy > y1) oy = y1;
+            return true;
+        }
+        float dl = px - x0, dr = x1 - px, db = py - y0, dt = y1 - py;
+        if (dl <= dr && dl <= db && dl <= dt) { ox = x0; oy = py; }
+        else if (dr <= db && dr <= dt) { ox = x1; oy = py; }
+        else if (db <= dt) { ox = px; oy = y0; }
+        else { ox = px; oy = y1; }
+        return true;
+    }
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    bool approx = sd[14] > 0.5f;
+    float best = BIG, bt = 0.f;
+    uint32_t bs = 0;
+    bool any = false;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = S.curve_data + (off + s) * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)), bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)), by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+        float lim = any ? best : cut;
+        if (box_dist(bx0, by0, bx1, by1, px, py) > lim) continue;
+        float d, t;
+        if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
+        else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+        else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        if (!any || d < best) {
+            best = d;
+            bt = t;
+            bs = s;
+            any = true;
+        }
+    }
+    if (!any) return false;
+    const float *cv = S.curve_data + (off + bs) * 13u;
+    uint32_t k = (uint32_t)cv[0];
+    float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+    float t = bt, u = 1.0f - t;
+    if (k == 0) {
+        ox = x0 + (x1 - x0) * t;
+        oy = y0 + (y1 - y0) * t;
+    } else if (k == 1) {
+        ox = ((x0 * (u * u)) + (x1 * (2.0f * u * t))) + (x2 * (t * t));
+        oy = ((y0 * (u * u)) + (y1 * (2.0f * u * t))) + (y2 * (t * t));
+    } else {
+        ox = (((x0 * (u * u * u)) + (x1 * (3.0f * u * u * t))) + (x2 * (3.0f * u * t * t))) + (x3 * (t * t * t));
+        oy = (((y0 * (u * u * u)) + (y1 * (3.0f * u * u * t))) + (y2 * (3.0f * u * t * t))) + (y3 * (t * t * t));
+    }
+    is_path = true;
+    seg = bs;
+    tt = t;
+    return true;
+}
+
+// compute_distance_bvh of group g at canvas point (px, py) within max_r:
+// the closest point over the group's shapes, the last one on a tie.
+DVG_DEV bool group_closest(const DevScene &S, uint32_t g, float px, float py, float max_r, PHit &h) {
+    const float *gd = S.group_data + g * 16u;
+    float lx, ly;
+    group_local(S, g, px, py, lx, ly);
+    const float *gs = S.group_shape_xform + g * 6u;
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    // the reference's root node: group-local box distance against the query
+    // radius (distance/api.rs:166-170)
+    const float *rb = S.group_cpu_bounds + g * 4u;
+    if (rb[0] <= rb[2] && rb[1] <= rb[3] && box_dist(rb[0], rb[1], rb[2], rb[3], lx, ly) > max_r) return false;
+    // canvas distances are at least smin times group-local ones
+    float smin = min_scale2(gs[0], gs[1], gs[3], gs[4]);
+    float best = max_r;
+    bool found = false;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        uint32_t prep = (uint32_t)S.group_shapes[off + i];
+        const float *b = S.shape_bounds + prep * 4u;
+        // a shape whose bounds lie beyond the current best cannot win
+        if (b[0] <= b[2] && b[1] <= b[3] && box_dist(b[0], b[1], b[2], b[3], lx, ly) * smin > best) continue;
+        uint32_t si = S.prepared_to_scene[prep];
+        const float *xf = S.shape_xform + prep * 6u;
+        float spx = xf[0] * lx + xf[1] * ly + xf[2];
+        float spy = xf[3] * lx + xf[4] * ly + xf[5];
+        float ox, oy, t;
+        uint32_t seg;
+        bool is_path;
+        if (!closest_shape(S, prep, spx, spy, BIG, ox, oy, seg, t, is_path)) continue;
+        const float *T = S.shape_transform + si * 6u;
+        float gx = T[0] * ox + T[1] * oy + T[2], gy = T[3] * ox + T[4] * oy + T[5];
+        float cx = gs[0] * gx + gs[1] * gy + gs[2], cy = gs[3] * gx + gs[4] * gy + gs[5];
+        float d = dist_pt(cx, cy, px, py);
+        if (d <= best) {
+            best = d;
+            h.cx = cx; h.cy = cy; h.dist = d; h.t = t;
+            h.prep = prep; h.seg = seg; h.path = is_path;
+            found = true;
+        }
+    }
+    return found;
+}
+
+// is_inside_bvh (api.rs:259-309) through the winding tests of the group
+DVG_DEV bool group_inside(const DevScene &S, uint32_t g, float px, float py) {
+    const float *gd = S.group_data + g * 16u;
+    uint32_t rule = (uint32_t)gd[7];
+    float lx, ly;
+    group_local(S, g, px, py, lx, ly);
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
+    group_cov<true>(S, g, gd, lx, ly, true, false, rule, false, c);
+    return (rule == 1u) ? c.crossings > 0.f : c.winding != 0.f;
+}
+
+// Stroke fragment of group g: (signed) distance and coverage.
+DVG_DEV bool pf_stroke(const DevScene &S, uint32_t g, float px, float py, PHit &h, float &w) {
+    if (!group_closest(S, g, px, py, BIG, h)) return false;
+    float sw = S.shape_data[h.prep * 15u + 3u];
+    float d = h.dist;
+    w = smoothstep_unit(absf_(d) + sw) - smoothstep_unit(absf_(d) - sw);
+    return w > 0.f;
+}
+
+// Fill fragment of group g; hit = a boundary point within one pixel.
+DVG_DEV bool pf_fill(const DevScene &S, uint32_t g, float px, float py, PHit &h, bool &hit, float &d, float &w) {
+    hit = group_closest(S, g, px, py, 1.0f, h);
+    bool inside = group_inside(S, g, px, py);
+    if (!hit && !inside) return false;
+    d = hit ? h.dist : 1.0f;
+    if (!inside) d = -d;
+    w = smoothstep_unit(d);
+    return w > 0.f;
+}
+
+// Colour (paint * coverage) of a recorded fragment: `covw` is the stroke
+// coverage itself, or the signed fill distance.
+DVG_DEV float4 pf_color(const DevScene &S, uint32_t g, bool stroke, float covw, float px, float py) {
+    float4 c = frag_color(S, g, stroke, px, py);
+    c.w *= stroke ? covw : smoothstep_unit(covw);
+    return c;
+}
+
+// One pass over the tile's groups: composites every prefiltered fragment into
+// `fin`, the first `start` into `pre`, and records fragments
+// [start, start + PF_WINDOW) as (list pos << 2 | within << 1 | stroke) and a
+// coverage word (the stroke coverage w, or the signed fill distance d) in the
+// lane's LDS columns.
+DVG_DEV uint32_t pf_walk(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t start,
+                         uint32_t *ids, float *cov, float4 &pre, float4 &fin) {
+    uint32_t count = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        uint32_t g = list[e];
+        const float *gd = S.group_data + g * 16u;
+        uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4];
+        for (int k = 0; k < 2; ++k) {
+            bool is_stroke = k == 0;
+            if ((is_stroke ? sk : fk) == DVG_PAINT_NONE) continue;
+            PHit h;
+            float w, covw;
+            bool within = true;
+            if (is_stroke) {
+                if (!pf_stroke(S, g, px, py, h, w)) continue;
+                covw = w;
+            } else {
+                float d;
+                if (!pf_fill(S, g, px, py, h, within, d, w)) continue;
+                covw = d;
+            }
+            float4 c = frag_color(S, g, is_stroke, px, py);
+            c.w *= w;
+            blend_cpu(fin, c);
+            if (count < start) blend_cpu(pre, c);
+            else if (count < start + PF_WINDOW) {
+                uint32_t slot = (count - start) * BLOCK;
+                ids[slot] = (e << 2) | (within ? 2u : 0u) | (is_stroke ? 1u : 0u);
+                cov[slot] = covw;
+            }
+            ++count;
+        }
+    }
+    return count;
+}
+
+// 3x3 helpers for the transform adjoints (row-major, full matrices).
+DVG_DEV void m3_zero(M3 &m) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) m.m[i][j] = 0.f;
+}
+DVG_DEV M3 m3_transpose(const M3 &a) {
+    M3 o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[j][i];
+    return o;
+}
+
+// d_xform_pt (backward/math.rs:101-120)
+DVG_DEV void d_xform_pt(const M3 &m, float ptx, float pty, float dox, float doy, M3 &dm, float &dpx, float &dpy) {
+    float t0 = m.m[0][0] * ptx + m.m[0][1] * pty + m.m[0][2];
+    float t1 = m.m[1][0] * ptx + m.m[1][1] * pty + m.m[1][2];
+    float t2 = m.m[2][0] * ptx + m.m[2][1] * pty + m.m[2][2];
+    float ox = t0 / t2, oy = t1 / t2;
+    float d0 = dox / t2, d1 = doy / t2;
+    float d2 = -(dox * ox + doy * oy) / t2;
+    dm.m[0][0] += d0 * ptx; dm.m[0][1] += d0 * pty; dm.m[0][2] += d0;
+    dm.m[1][0] += d1 * ptx; dm.m[1][1] += d1 * pty; dm.m[1][2] += d1;
+    dm.m[2][0] += d2 * ptx; dm.m[2][1] += d2 * pty; dm.m[2][2] += d2;
+    dpx += d0 * m.m[0][0] + d1 * m.m[1][0] + d2 * m.m[2][0];
+    dpy += d0 * m.m[0][1] + d1 * m.m[1][1] + d2 * m.m[2][1];
+}
+
+// d_length (backward/math.rs:33-41)
+DVG_DEV void d_length2(float vx, float vy, float d_l, float &ox, float &oy) {
+    float l = sqrtf(vx * vx + vy * vy);
+    ox = oy = 0.f;
+    if (l == 0.f) return;
+    float d_l_sq = 0.5f * d_l / l;
+    ox = vx * (2.0f * d_l_sq);
+    oy = vy * (2.0f * d_l_sq);
+}
+
+// d_closest_point_path (backward/distance.rs:188-404): point gradients of the
+// closest point on segment `base` of scene path si at parameter t.
+DVG_DEV void d_closest_path(const DevScene &S, const DevGrads &G, uint32_t si, uint32_t base, float t, float px,
+                            float py, float dcx, float dcy, float &dpx, float &dpy) {
+    uint32_t n = S.sp_cnt[si];
+    if (n == 0) return;
+    bool closed = S.s_closed[si] != 0;
+    const uint32_t *ctrl = S.path_ctrl + S.sc_off[si];
+    uint32_t nseg = S.sc_cnt[si];
+    uint32_t pid = 0;
+    uint32_t lim = base < nseg ? base : nseg;
+    for (uint32_t i = 0; i < lim; ++i) {
+        uint32_t c = ctrl[i];
+        pid += c == 0u ? 1u : (c == 1u ? 2u : (c == 2u ? 3u : 0u));
+    }
+    const float *pts = S.path_points + (size_t)S.sp_off[si] * 2u;
+    float *dpt = G.d_shape_points + (size_t)S.sp_off[si] * 2u;
+    auto gx = [&](uint32_t i) { return closed ? pts[(i % n) * 2u] : pts[(i < n - 1 ? i : n - 1) * 2u]; };
+    auto gy = [&](uint32_t i) { return closed ? pts[(i % n) * 2u + 1u] : pts[(i < n - 1 ? i : n - 1) * 2u + 1u]; };
+    auto add = [&](uint32_t i, float vx, float vy) {
+        uint32_t index = closed ? i % n : i;
+        if (index < n) {
+            unsafeAtomicAdd(dpt + index * 2u, vx);
+            unsafeAtomicAdd(dpt + index * 2u + 1u, vy);
+        }
+    };
+    uint32_t nc = base < nseg ? ctrl[base] : 0u;
+    if (nc == 0u) {
+        uint32_t i0 = pid, i1 = (pid + 1u) % n;
+        float p0x = gx(i0), p0y = gy(i0), p1x = gx(i1), p1y = gy(i1);
+        float ex = p1x - p0x, ey = p1y - p0y;
+        float tl = ((px - p0x) * ex + (py - p0y) * ey) / (ex * ex + ey * ey);
+        if (tl < 0.f) add(i0, dcx, dcy);
+        else if (tl > 1.f) add(i1, dcx, dcy);
+        else {
+            add(i0, dcx * (1.0f - tl), dcy * (1.0f - tl));
+            add(i1, dcx * tl, dcy * tl);
+        }
+    } else if (nc == 1u) {
+        uint32_t i0 = pid, i1 = pid + 1u, i2 = (pid + 2u) % n;
+        float p0x = gx(i0), p0y = gy(i0), p1x = gx(i1), p1y = gy(i1), p2x = gx(i2), p2y = gy(i2);
+        float d0x = 0.f, d0y = 0.f, d1x = 0.f, d1y = 0.f, d2x = 0.f, d2y = 0.f;
+        if (t == 0.f) { d0x += dcx; d0y += dcy; }
+        else if (t == 1.f) { d2x += dcx; d2y += dcy; }
+        else {
+            float ax = (p0x - p1x * 2.0f) + p2x, ay = (p0y - p1y * 2.0f) + p2y;
+            float bx = p1x - p0x, by = p1y - p0y;
+            float qx = p0x - px, qy = p0y - py;
+            float A = ax * ax + ay * ay;
+            float B = 3.0f * (ax * bx + ay * by);
+            float C = 2.0f * (bx * bx + by * by) + (ax * qx + ay * qy);
+            float u = 1.0f - t;
+            float d_tt = 2.0f * u * (dcx * p0x + dcy * p0y) + 2.0f * t * (dcx * p1x + dcy * p1y);
+            float d_t = -d_tt + 2.0f * u * (dcx * p1x + dcy * p1y) + 2.0f * t * (dcx * p2x + dcy * p2y);
+            d0x += dcx * (u * u); d0y += dcy * (u * u);
+            d1x += dcx * (2.0f * u * t); d1y += dcy * (2.0f * u * t);
+            d2x += dcx * (t * t); d2y += dcy * (t * t);
+            float pd = 3.0f * A * t * t + 2.0f * B * t + C;
+            if (absf_(pd) > 1.0e-6f) {
+                float dA = -(d_t / pd) * t * t * t, dB = -(d_t / pd) * t * t, dC = -(d_t / pd) * t, dD = -(d_t / pd);
+                d0x += (((ax * (2.0f * dA) + (bx - ax) * (3.0f * dB)) + (bx * -4.0f) * dC) + (qx + ax) * dC) + (bx - qx) * (2.0f * dD);
+                d0y += (((ay * (2.0f * dA) + (by - ay) * (3.0f * dB)) + (by * -4.0f) * dC) + (qy + ay) * dC) + (by - qy) * (2.0f * dD);
+                d1x += (((ax * (-4.0f * dA) + (ax + bx * -2.0f) * (3.0f * dB)) + (bx * 4.0f) * dC) + qx * (-2.0f * dC)) + qx * dD;
+                d1y += (((ay * (-4.0f * dA) + (ay + by * -2.0f) * (3.0f * dB)) + (by * 4.0f) * dC) + qy * (-2.0f * dC)) + qy * dD;
+                d2x += (ax * (2.0f * dA) + bx * (3.0f * dB)) + qx * dC;
+                d2y += (ay * (2.0f * dA) + by * (3.0f * dB)) + qy * dC;
+                dpx += ax * (-dC) + bx * dD;
+                dpy += ay * (-dC) + by * dD;
+            }
+        }
+        add(i0, d0x, d0y);
+        add(i1, d1x, d1y);
+        add(i2, d2x, d2y);
+    } else if (nc == 2u) {
+        uint32_t i0 = pid, i1 = pid + 1u, i2 = pid + 2u, i3 = (pid + 3u) % n;
+        float p0x = gx(i0), p0y = gy(i0), p1x = gx(i1), p1y = gy(i1);
+        float p2x = gx(i2), p2y = gy(i2), p3x = gx(i3), p3y = gy(i3);
+        float d0x = 0.f, d0y = 0.f, d1x = 0.f, d1y = 0.f, d2x = 0.f, d2y = 0.f, d3x = 0.f, d3y = 0.f;
+        if (t == 0.f) { d0x += dcx; d0y += dcy; }
+        else if (t == 1.f) { d3x += dcx; d3y += dcy; }
+        else {
+            float ax = ((p0x * -1.0f + p1x * 3.0f) + p2x * -3.0f) + p3x, ay = ((p0y * -1.0f + p1y * 3.0f) + p2y * -3.0f) + p3y;
+            float bx = (p0x * 3.0f + p1x * -6.0f) + p2x * 3.0f, by = (p0y * 3.0f + p1y * -6.0f) + p2y * 3.0f;
+            float cx = p0x * -3.0f + p1x * 3.0f, cy = p0y * -3.0f + p1y * 3.0f;
+            float A = 3.0f * (ax * ax + ay * ay);
+            if (absf_(A) < 1.0e-10f) return;
+            float qx = p0x - px, qy = p0y - py;
+            float B = 5.0f * (ax * bx + ay * by);
+            float C = 4.0f * (ax * cx + ay * cy) + 2.0f * (bx * bx + by * by);
+            float D = 3.0f * ((bx * cx + by * cy) + (ax * qx + ay * qy));
+            float E = (cx * cx + cy * cy) + 2.0f * (qx * bx + qy * by);
+            float F = qx * cx + qy * cy;
+            B = B / A; C = C / A; D = D / A; E = E / A; F = F / A;
+            float u = 1.0f - t;
+            float dp0 = dcx * p0x + dcy * p0y, dp1 = dcx * p1x + dcy * p1y, dp2 = dcx * p2x + dcy * p2y,
+                  dp3 = dcx * p3x + dcy * p3y;
+            float d_tt = 3.0f * u * u * dp0 + 6.0f * u * t * dp1 + 3.0f * t * t * dp2;
+            float d_t = -d_tt + 3.0f * u * u * dp1 + 6.0f * u * t * dp2 + 3.0f * t * t * dp3;
+            d0x += dcx * (u * u * u); d0y += dcy * (u * u * u);
+            d1x += dcx * (3.0f * u * u * t); d1y += dcy * (3.0f * u * u * t);
+            d2x += dcx * (3.0f * u * t * t); d2y += dcy * (3.0f * u * t * t);
+            d3x += dcx * (t * t * t); d3y += dcy * (t * t * t);
+            float pd = 5.0f * t * t * t * t + 4.0f * B * t * t * t + 3.0f * C * t * t + 2.0f * D * t + E;
+            if (absf_(pd) > 1.0e-10f) {
+                float dB = -(d_t / pd) * t * t * t * t, dC = -(d_t / pd) * t * t * t, dD = -(d_t / pd) * t * t;
+                float dE = -(d_t / pd) * t, dF = -(d_t / pd);
+                float dA = -dB * B / A - dC * C / A - dD * D / A - dE * E / A - dF * F / A;
+                dB /= A; dC /= A; dD /= A; dE /= A; dF /= A;
+                d0x += ax * (3.0f * -1.0f * 2.0f * dA); d0y += ay * (3.0f * -1.0f * 2.0f * dA);
+                d1x += ax * (3.0f * 3.0f * 2.0f * dA); d1y += ay * (3.0f * 3.0f * 2.0f * dA);
+                d2x += ax * (3.0f * -3.0f * 2.0f * dA); d2y += ay * (3.0f * -3.0f * 2.0f * dA);
+                d3x += ax * (3.0f * 1.0f * 2.0f * dA); d3y += ay * (3.0f * 1.0f * 2.0f * dA);
+                d0x += (bx * -1.0f + ax * 3.0f) * (5.0f * dB); d0y += (by * -1.0f + ay * 3.0f) * (5.0f * dB);
+                d1x += (bx * 3.0f + ax * -6.0f) * (5.0f * dB); d1y += (by * 3.0f + ay * -6.0f) * (5.0f * dB);
+                d2x += (bx * -3.0f + ax * 3.0f) * (5.0f * dB); d2y += (by * -3.0f + ay * 3.0f) * (5.0f * dB);
+                d3x += bx * (5.0f * dB); d3y += by * (5.0f * dB);
+                d0x += (cx * -1.0f + ax * -3.0f) * (4.0f * dC) + bx * (3.0f * 2.0f * dC);
+                d0y += (cy * -1.0f + ay * -3.0f) * (4.0f * dC) + by * (3.0f * 2.0f * dC);
+                d1x += (cx * 3.0f + ax * 3.0f) * (4.0f * dC) + bx * (-6.0f * 2.0f * dC);
+                d1y += (cy * 3.0f + ay * 3.0f) * (4.0f * dC) + by * (-6.0f * 2.0f * dC);
+                d2x += cx * (-3.0f * dC * 4.0f) + bx * (3.0f * 2.0f * dC);
+                d2y += cy * (-3.0f * dC * 4.0f) + by * (3.0f * 2.0f * dC);
+                d3x += cx * (4.0f * dC); d3y += cy * (4.0f * dC);
+                d0x += (cx * 3.0f + bx * -3.0f) * (3.0f * dD) + (ax - qx) * (3.0f * dD);
+                d0y += (cy * 3.0f + by * -3.0f) * (3.0f * dD) + (ay - qy) * (3.0f * dD);
+                d1x += (cx * -6.0f + bx * 3.0f) * (3.0f * dD) + qx * (3.0f * 3.0f * dD);
+                d1y += (cy * -6.0f + by * 3.0f) * (3.0f * dD) + qy * (3.0f * 3.0f * dD);
+                d2x += cx * (3.0f * 3.0f * dD) + qx * (-3.0f * 3.0f * dD);
+                d2y += cy * (3.0f * 3.0f * dD) + qy * (-3.0f * 3.0f * dD);
+                dpx += ax * (-1.0f * 3.0f * dD); dpy += ay * (-1.0f * 3.0f * dD);
+                d0x += cx * (-3.0f * 2.0f * dE) + (bx + qx * 3.0f) * (2.0f * dE);
+                d0y += cy * (-3.0f * 2.0f * dE) + (by + qy * 3.0f) * (2.0f * dE);
+                d1x += cx * (3.0f * 2.0f * dE) + qx * (-6.0f * 2.0f * dE);
+                d1y += cy * (3.0f * 2.0f * dE) + qy * (-6.0f * 2.0f * dE);
+                d2x += qx * (3.0f * 2.0f * dE); d2y += qy * (3.0f * 2.0f * dE);
+                dpx += bx * (-1.0f * 2.0f * dE); dpy += by * (-1.0f * 2.0f * dE);
+                d0x += cx * dF + qx * (-3.0f * dF); d0y += cy * dF + qy * (-3.0f * dF);
+                d1x += qx * (3.0f * dF); d1y += qy * (3.0f * dF);
+                dpx += cx * (-1.0f * dF); dpy += cy * (-1.0f * dF);
+            }
+        }
+        add(i0, d0x, d0y);
+        add(i1, d1x, d1y);
+        add(i2, d2x, d2y);
+        add(i3, d3x, d3y);
+    }
+}
+
+// d_closest_point_ellipse (backward/distance.rs:489-561), including its
+// omission of the direct d(closest)/d(center) term.
+DVG_DEV void d_closest_ellipse(float cx, float cy, float rx_, float ry_, float px, float py, float dcx, float dcy,
+                               float *dp, float &dpx, float &dpy) {
+    float rx = absf_(rx_), ry = absf_(ry_);
+    const float eps = 1.0e-6f;
+    float lx = px - cx, ly = py - cy;
+    float dcen_x = 0.f, dcen_y = 0.f, drx = 0.f, dry = 0.f;
+    if (rx < eps && ry < eps) {
+        dcen_x += dcx; dcen_y += dcy;
+    } else if (rx < eps) {
+        float hit = (ly >= -ry && ly <= ry) ? 1.0f : 0.0f;
+        dcen_x += dcx; dcen_y += dcy;
+        dpy += dcy * hit;
+        if (ly > ry) dry += dcy;
+        else if (ly < -ry) dry -= dcy;
+    } else if (ry < eps) {
+        float hit = (lx >= -rx && lx <= rx) ? 1.0f : 0.0f;
+        dcen_x += dcx; dcen_y += dcy;
+        dpx += dcx * hit;
+        if (lx > rx) drx += dcx;
+        else if (lx < -rx) drx -= dcx;
+    } else {
+        float sx = lx < 0.f ? -1.0f : 1.0f, sy = ly < 0.f ? -1.0f : 1.0f;
+        float x = absf_(lx), y = absf_(ly);
+        float t = atan2f(y * rx, x * ry);
+        float g_t = 0.f;
+        for (int i = 0; i < 20; ++i) {
+            float s = sinf(t), c = cosf(t);
+            float g = rx * x * s - ry * y * c + (ry * ry - rx * rx) * s * c;
+            g_t = rx * x * c + ry * y * s + (ry * ry - rx * rx) * (c * c - s * s);
+            if (absf_(g_t) < 1.0e-12f) break;
+            float next = t - g / g_t;
+            next = next < 0.f ? 0.f : (next > 1.57079632679489661923f ? 1.57079632679489661923f : next);
+            if (absf_(next - t) < 1.0e-6f) { t = next; break; }
+            t = next;
+        }
+        float s = sinf(t), c = cosf(t);
+        float d_t = 0.f;
+        drx += dcx * sx * c;
+        d_t += dcx * sx * (-rx * s);
+        dry += dcy * sy * s;
+        d_t += dcy * sy * (ry * c);
+        float g_a = x * s - 2.0f * rx * s * c;
+        float g_b = -y * c + 2.0f * ry * s * c;
+        float g_x = rx * s, g_y = -ry * c;
+        if (absf_(g_t) > 1.0e-12f) {
+            float inv = -d_t / g_t;
+            drx += inv * g_a;
+            dry += inv * g_b;
+            float d_x = inv * g_x, d_y = inv * g_y;
+            dpx += d_x * sx;
+            dpy += d_y * sy;
+            dcen_x -= d_x * sx;
+            dcen_y -= d_y * sy;
+        }
+    }
+    unsafeAtomicAdd(dp + 0, dcen_x);
+    unsafeAtomicAdd(dp + 1, dcen_y);
+    unsafeAtomicAdd(dp + 2, drx);
+    unsafeAtomicAdd(dp + 3, dry);
+}
+
+// d_closest_point_rect (backward/distance.rs:407-486)
+DVG_DEV void d_closest_rect(float x0, float y0, float x1, float y1, float px, float py, float dcx, float dcy,
+                            float *dp, float &dpx, float &dpy) {
+    // corners lt (x0,y0), rt (x1,y0), lb (x0,y1), rb (x1,y1); edges
+    // 0 lt-lb, 1 lt-rt, 2 rt-rb, 3 lb-rb
+    float ex[4][4] = {{x0, y0, x0, y1}, {x0, y0, x1, y0}, {x1, y0, x1, y1}, {x0, y1, x1, y1}};
+    auto seg_d = [&](int k) {
+        float ax = ex[k][0], ay = ex[k][1], bx = ex[k][2], by = ex[k][3];
+        float vx = bx - ax, vy = by - ay;
+        float t = ((px - ax) * vx + (py - ay) * vy) / (vx * vx + vy * vy);
+        if (t < 0.f) return dist_pt(ax, ay, px, py);
+        if (t > 1.f) return dist_pt(bx, by, px, py);
+        return dist_pt(ax + vx * t, ay + vy * t, px, py);
+    };
+    int id = 0;
+    float md = seg_d(0);
+    for (int k = 1; k < 4; ++k) {
+        float dk = seg_d(k);
+        if (dk < md) { md = dk; id = k; }
+    }
+    float ax = ex[id][0], ay = ex[id][1], bx = ex[id][2], by = ex[id][3];
+    float d0x = 0.f, d0y = 0.f, d1x = 0.f, d1y = 0.f;
+    float vx = bx - ax, vy = by - ay;
+    float t = ((px - ax) * vx + (py - ay) * vy) / (vx * vx + vy * vy);
+    if (t < 0.f) { d0x += dcx; d0y += dcy; }
+    else if (t > 1.f) { d1x += dcx; d1y += dcy; }
+    else {
+        d0x += dcx * (1.0f - t); d0y += dcy * (1.0f - t);
+        d1x += dcx * t; d1y += dcy * t;
+        float d_t = dcx * vx + dcy * vy;
+        float den = vx * vx + vy * vy;
+        float d_num = d_t / den, d_den = d_t * (-t) / den;
+        dpx += vx * d_num; dpy += vy * d_num;
+        d1x += (px - ax) * d_num; d1y += (py - ay) * d_num;
+        d0x += (((ax - bx) + ax) - px) * d_num; d0y += (((ay - by) + ay) - py) * d_num;
+        d1x += vx * (2.0f * d_den); d1y += vy * (2.0f * d_den);
+        d0x += (ax - bx) * (2.0f * d_den); d0y += (ay - by) * (2.0f * d_den);
+    }
+    // corner gradients -> min / max (distance.rs:479-485)
+    float dmin_x = 0.f, dmin_y = 0.f, dmax_x = 0.f, dmax_y = 0.f;
+    auto corner = [&](int which, float gxv, float gyv) {  // 0 lt, 1 rt, 2 lb, 3 rb
+        if (which == 0) { dmin_x += gxv; dmin_y += gyv; }
+        else if (which == 1) { dmax_x += gxv; dmin_y += gyv; }
+        else if (which == 2) { dmin_x += gxv; dmax_y += gyv; }
+        else { dmax_x += gxv; dmax_y += gyv; }
+    };
+    const int ends[4][2] = {{0, 2}, {0, 1}, {1, 3}, {2, 3}};
+    corner(ends[id][0], d0x, d0y);
+    corner(ends[id][1], d1x, d1y);
+    unsafeAtomicAdd(dp + 0, dmin_x);
+    unsafeAtomicAdd(dp + 1, dmin_y);
+    unsafeAtomicAdd(dp + 2, dmax_x);
+    unsafeAtomicAdd(dp + 3, dmax_y);
+}
+
+// d_compute_distance (backward/distance.rs:50-126) for the closest point of a
+// fragment; adds -d(pt) into (dtx, dty).
+DVG_DEV void d_distance_query(const DevScene &S, const DevGrads &G, uint32_t g, const PHit &h, float px, float py,
+                              float d_dist, float &dtx, float &dty) {
+    float ddx = px - h.cx, ddy = py - h.cy;
+    if (ddx * ddx + ddy * ddy < 1.0e-10f) return;
+    uint32_t prep = h.prep, si = S.prepared_to_scene[prep];
+    M3 C2S = m3_affine(S.group_xform + g * 6u), S2C = m3_affine(S.group_shape_xform + g * 6u);
+    M3 T = m3_affine(S.shape_transform + si * 6u), Ti = m3_affine(S.shape_xform + prep * 6u);
+    float lpgx = C2S.m[0][0] * px + C2S.m[0][1] * py + C2S.m[0][2];
+    float lpgy = C2S.m[1][0] * px + C2S.m[1][1] * py + C2S.m[1][2];
+    float lcgx = C2S.m[0][0] * h.cx + C2S.m[0][1] * h.cy + C2S.m[0][2];
+    float lcgy = C2S.m[1][0] * h.cx + C2S.m[1][1] * h.cy + C2S.m[1][2];
+    float lpsx = Ti.m[0][0] * lpgx + Ti.m[0][1] * lpgy + Ti.m[0][2];
+    float lpsy = Ti.m[1][0] * lpgx + Ti.m[1][1] * lpgy + Ti.m[1][2];
+    float lcsx = Ti.m[0][0] * lcgx + Ti.m[0][1] * lcgy + Ti.m[0][2];
+    float lcsy = Ti.m[1][0] * lcgx + Ti.m[1][1] * lcgy + Ti.m[1][2];
+    float vx, vy;
+    d_length2(ddx, ddy, d_dist, vx, vy);  // d_distance(closest, pt)
+    float dptx = vx, dpty = vy, dclx = -vx, dcly = -vy;
+    M3 d_s2c, d_st, d_si, d_c2s;
+    m3_zero(d_s2c); m3_zero(d_st); m3_zero(d_si); m3_zero(d_c2s);
+    float dlcgx = 0.f, dlcgy = 0.f;
+    d_xform_pt(S2C, lcgx, lcgy, dclx, dcly, d_s2c, dlcgx, dlcgy);
+    float dlcsx = 0.f, dlcsy = 0.f;
+    d_xform_pt(T, lcsx, lcsy, dlcgx, dlcgy, d_st, dlcsx, dlcsy);
+    float dlpsx = 0.f, dlpsy = 0.f;
+    const float *sd = S.shape_data + prep * 15u;
+    uint32_t kind = (uint32_t)sd[0];
+    float *dp = G.d_shape_params + si * 8u;
+    if (kind == DVG_SHAPE_CIRCLE) {
+        float cx = sd[4], cy = sd[5], r = sd[6];
+        float vx2 = lpsx - cx, vy2 = lpsy - cy;
+        float l = sqrtf(vx2 * vx2 + vy2 * vy2);
+        float nx = 0.f, ny = 0.f;
+        if (l != 0.f) { nx = vx2 / l; ny = vy2 / l; }
+        float dcenx = dlcsx, dceny = dlcsy;
+        float drad = dlcsx * nx + dlcsy * ny;
+        float dnx = dlcsx * r, dny = dlcsy * r;
+        float dvx = 0.f, dvy = 0.f;
+        if (l != 0.f) {  // d_normalize (backward/math.rs:54-64)
+            float d_l = -(dnx * nx + dny * ny) / l;
+            float lx2, ly2;
+            d_length2(vx2, vy2, d_l, lx2, ly2);
+            dvx = dnx / l + lx2;
+            dvy = dny / l + ly2;
+        }
+        dcenx -= dvx; dceny -= dvy;
+        dlpsx += dvx; dlpsy += dvy;
+        unsafeAtomicAdd(dp + 0, dcenx);
+        unsafeAtomicAdd(dp + 1, dceny);
+        unsafeAtomicAdd(dp + 2, drad);
+    } else if (kind == DVG_SHAPE_ELLIPSE) {
+        d_closest_ellipse(sd[4], sd[5], sd[6], sd[7], lpsx, lpsy, dlcsx, dlcsy, dp, dlpsx, dlpsy);
+    } else if (kind == DVG_SHAPE_RECT) {
+        d_closest_rect(sd[4], sd[5], sd[6], sd[7], lpsx, lpsy, dlcsx, dlcsy, dp, dlpsx, dlpsy);
+    } else if (h.path) {
+        d_closest_path(S, G, si, h.seg, h.t, lpsx, lpsy, dlcsx, dlcsy, dlpsx, dlpsy);
+    }
+    float dlpgx = 0.f, dlpgy = 0.f;
+    d_xform_pt(Ti, lpgx, lpgy, dlpsx, dlpsy, d_si, dlpgx, dlpgy);
+    d_xform_pt(C2S, px, py, dlpgx, dlpgy, d_c2s, dptx, dpty);
+    // matrix-inverse corrections -M^T dM M^T (distance.rs:116-124)
+    M3 tc = m3_transpose(C2S), ts = m3_transpose(Ti);
+    M3 cg = m3_mul(m3_mul(tc, d_c2s), tc), cs = m3_mul(m3_mul(ts, d_si), ts);
+    float *dgt = G.d_group_transform + g * 9u;
+    float *dst = G.d_shape_transform + si * 9u;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            unsafeAtomicAdd(dgt + i * 3 + j, d_s2c.m[i][j] + (-cg.m[i][j]));
+            unsafeAtomicAdd(dst + i * 3 + j, (-cs.m[i][j]) + d_st.m[i][j]);
+        }
+    dtx -= dptx;
+    dty -= dpty;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, const float *__restrict__ weight,
+                                                                const float *__restrict__ d_render, DevGrads G) {
+    __shared__ float s_paint[LDS_GROUP_CAP * 8];
+    __shared__ float s_red[4][6];
+    __shared__ uint32_t s_ids[PF_WINDOW * BLOCK];
+    __shared__ float s_cov[PF_WINDOW * BLOCK];
+    uint32_t *ids = s_ids + threadIdx.x;
+    float *cov = s_cov + threadIdx.x;
+    uint32_t tile = blockIdx.x;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    const uint32_t *list = S.tile_entries + lb;
+    uint32_t n = le - lb;
+    uint32_t nlds = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
+    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) s_paint[i] = 0.f;
+    __syncthreads();
+    float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
+    uint32_t total = TILE * TILE * S.spp;
+    float Wf = (float)S.W, Hf = (float)S.H;
+    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+        uint32_t x, y, sx, sy;
+        tile_sample(S, tile, s, x, y, sx, sy);
+        if (x >= S.W || y >= S.H) continue;
+        // render.rs:96-111: sample centres (no jitter when prefiltering)
+        float px0 = (float)x + ((float)sx + 0.5f) / (float)S.nsx;
+        float py0 = (float)y + ((float)sy + 0.5f) / (float)S.nsy;
+        float px = (px0 / Wf) * Wf, py = (py0 / Hf) * Hf;
+        size_t pix = (size_t)y * S.W + x;
+        float4 bgs = background_at(S, x, y);
+        float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
+        float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
+        float4 pre = bg, fin = bg;
+        uint32_t nf = pf_walk(S, list, n, px, py, 0, ids, cov, pre, fin);
+        float4 col;
+        float dtx = 0.f, dty = 0.f;
+        if (nf == 0) {
+            col = bg;
+            acc_bg[0] += dcol.x; acc_bg[1] += dcol.y; acc_bg[2] += dcol.z; acc_bg[3] += dcol.w;
+            if (S.bg_image && G.d_background_image) {
+                float *db = G.d_background_image + pix * 4u;
+                unsafeAtomicAdd(db + 0, dcol.x); unsafeAtomicAdd(db + 1, dcol.y);
+                unsafeAtomicAdd(db + 2, dcol.z); unsafeAtomicAdd(db + 3, dcol.w);
+            }
+        } else {
+            float fa = fin.w;
+            float fr = fin.x, fg = fin.y, fb = fin.z;
+            if (fa > 1.0e-6f) {
+                float sc = 1.0f / fa;
+                fr = fr * sc; fg = fg * sc; fb = fb * sc;
+            }
+            col = make_float4(fr, fg, fb, fa);
+            float dr = dcol.x, dg = dcol.y, db = dcol.z, da = dcol.w;
+            if (fa > 1.0e-6f) {
+                float sc = 1.0f / fa;
+                float dot = dr * fr + dg * fg + db * fb;
+                dr = dr * sc; dg = dg * sc; db = db * sc;
+                da -= dot / fa;
+            }
+            uint32_t nwin = (nf + PF_WINDOW - 1) / PF_WINDOW;
+            for (int w = (int)nwin - 1; w >= 0; --w) {
+                uint32_t wstart = (uint32_t)w * PF_WINDOW;
+                float4 wpre = bg;
+                if (nwin > 1) {
+                    float4 f2 = bg;
+                    pf_walk(S, list, n, px, py, wstart, ids, cov, wpre, f2);
+                }
+                uint32_t m = min((uint32_t)PF_WINDOW, nf - wstart);
+                for (int i = (int)m - 1; i >= 0; --i) {
+                    float4 pc = wpre;
+                    for (int j = 0; j < i; ++j) {
+                        uint32_t id = ids[j * BLOCK];
+                        blend_cpu(pc, pf_color(S, list[id >> 2], (id & 1u) != 0, cov[j * BLOCK], px, py));
+                    }
+                    uint32_t id = ids[i * BLOCK];
+                    uint32_t e = id >> 2;
+                    bool is_stroke = (id & 1u) != 0, within = (id & 2u) != 0;
+                    uint32_t g = list[e];
+                    float4 c = pf_color(S, g, is_stroke, cov[i * BLOCK], px, py);
+                    float a = c.w;
+                    float d_prev_a = da * (1.0f - a);
+                    float d_alpha = da * (1.0f - pc.w) + (dr * (c.x - pc.x) + dg * (c.y - pc.y) + db * (c.z - pc.z));
+                    // the fragment's geometry again (one closest-point query)
+                    PHit h;
+                    float wcov, d;
+                    bool ok;
+                    if (is_stroke) {
+                        ok = group_closest(S, g, px, py, BIG, h);
+                        d = h.dist;
+                        float sw = S.shape_data[h.prep * 15u + 3u];
+                        float ap = absf_(d) + sw, am = absf_(d) - sw;
+                        wcov = smoothstep_unit(ap) - smoothstep_unit(am);
+                        if (ok && wcov != 0.f) {
+                            float d_w = wcov > 0.f ? (a / wcov) * d_alpha : 0.f;
+                            float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha * wcov);
+                            const float *gd = S.group_data + g * 16u;
+                            if ((uint32_t)gd[4] == DVG_PAINT_SOLID) {
+                                if (e < LDS_GROUP_CAP) {
+                                    float *sp = s_paint + e * 8u + 4u;
+                                    atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
+                                    atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
+                                } else {
+                                    float *dgp = G.d_group_data + g * 16u + 12u;
+                                    unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
+                                    unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
+                                }
+                            } else {
+                                d_gradient_paint(S, G, (uint32_t)gd[5], px, py, dv, dtx, dty);
+                            }
+                            float d_ap = d_smoothstep_unit(ap, d_w);
+                            float d_am = -d_smoothstep_unit(am, d_w);
+                            float d_d = d_ap + d_am;
+                            if (d < 0.f) d_d = -d_d;
+                            unsafeAtomicAdd(G.d_shape_stroke_width + S.prepared_to_scene[h.prep], d_ap - d_am);
+                            if (absf_(d_d) > 1.0e-10f) d_distance_query(S, G, g, h, px, py, d_d, dtx, dty);
+                        }
+                    } else {
+                        d = cov[i * BLOCK];
+                        wcov = smoothstep_unit(d);
+                        if (wcov != 0.f) {
+                            float d_w = wcov > 0.f ? (a / wcov) * d_alpha : 0.f;
+                            float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha * wcov);
+                            const float *gd = S.group_data + g * 16u;
+                            if ((uint32_t)gd[2] == DVG_PAINT_SOLID) {
+                                if (e < LDS_GROUP_CAP) {
+                                    float *sp = s_paint + e * 8u;
+                                    atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
+                                    atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
+                                } else {
+                                    float *dgp = G.d_group_data + g * 16u + 8u;
+                                    unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
+                                    unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
+                                }
+                            } else {
+                                d_gradient_paint(S, G, (uint32_t)gd[3], px, py, dv, dtx, dty);
+                            }
+                            float d_d = d_smoothstep_unit(d, d_w);
+                            if (d < 0.f) d_d = -d_d;
+                            if (absf_(d_d) > 1.0e-10f && within && group_closest(S, g, px, py, 1.0f, h))
+                                d_distance_query(S, G, g, h, px, py, d_d, dtx, dty);
+                        }
+                    }
+                    dr = dr * (1.0f - a); dg = dg * (1.0f - a); db = db * (1.0f - a); da = d_prev_a;
+                }
+            }
+            acc_bg[0] += dr; acc_bg[1] += dg; acc_bg[2] += db; acc_bg[3] += da;
+            if (S.bg_image && G.d_background_image) {
+                float *dbp = G.d_background_image + pix * 4u;
+                unsafeAtomicAdd(dbp + 0, dr); unsafeAtomicAdd(dbp + 1, dg);
+                unsafeAtomicAdd(dbp + 2, db); unsafeAtomicAdd(dbp + 3, da);
+            }
+        }
+        if (G.d_translation && (dtx != 0.f || dty != 0.f)) {
+            unsafeAtomicAdd(G.d_translation + pix * 2u, dtx);
+            unsafeAtomicAdd(G.d_translation + pix * 2u + 1, dty);
+        }
+        acc_fr += filter_grad(S, weight, d_render, col, px0, py0);
+    }
+    float v[5] = {acc_bg[0], acc_bg[1], acc_bg[2], acc_bg[3], acc_fr};
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 5; ++k) {
+        float r = wave_sum(v[k]);
+        if (lane == 0) s_red[wave][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        float r = s_red[0][threadIdx.x] + s_red[1][threadIdx.x] + s_red[2][threadIdx.x] + s_red[3][threadIdx.x];
+        if (threadIdx.x < 4) {
+            if (!S.bg_image) unsafeAtomicAdd(G.d_background + threadIdx.x, r);
+        } else {
+            unsafeAtomicAdd(G.d_filter_radius, r);
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) {
+        float v2 = s_paint[i];
+        if (v2 != 0.f) {
+            uint32_t e = i >> 3, c = i & 7u;
+            uint32_t g = list[e];
+            unsafeAtomicAdd(G.d_group_data + g * 16u + 8u + c, v2);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Boundary sampling (backward/boundary.rs:113-1148).
 
@@ -1148,24 +1990,6 @@ DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint
         return make_float4(acc.x * sc, acc.y * sc, acc.z * sc, fa);
     }
     return acc;
-}
-
-// 3x3 helpers on affine matrices given by their first two rows.
-struct M3 {
-    float m[3][3];
-};
-DVG_DEV M3 m3_affine(const float *r) {
-    M3 o;
-    o.m[0][0] = r[0]; o.m[0][1] = r[1]; o.m[0][2] = r[2];
-    o.m[1][0] = r[3]; o.m[1][1] = r[4]; o.m[1][2] = r[5];
-    o.m[2][0] = 0.f; o.m[2][1] = 0.f; o.m[2][2] = 1.f;
-    return o;
-}
-DVG_DEV M3 m3_mul(const M3 &a, const M3 &b) {  // math.rs:185-195
-    M3 o;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
-    return o;
 }
 
 // One boundary sample of boundary_sampling (boundary.rs:138-197): shape by
@@ -1587,6 +2411,11 @@ void launch_resolve(const DevScene &S, const float *weight, const float *accum, 
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                               hipStream_t stream) {
     hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
+                       d_render, G);
+}
+void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(k_backward_prefiltered, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
                        d_render, G);
 }
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream) {

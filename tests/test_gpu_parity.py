@@ -21,12 +21,20 @@ FLIP_FRACTION = 1e-3
 GRAD_REL = 1e-3
 
 
+def _prefiltered_opts():
+    from pydvg.scene import RenderOptions
+    return RenderOptions(samples_x=1, samples_y=1, seed=0, jitter=False, use_prefiltering=True)
+
+
 def _cases():
     return {
         "cubic32": scenes.small_cubic_scene(32, 32, 4, seed=0, spp=2),
         "cubic64_spp4": scenes.small_cubic_scene(64, 64, 12, seed=1, spp=4),
         "config1_64": (scenes.cubic_paths_scene(64, 64, 16, seed=3), scenes.config1()[1]),
         "mixed64": (scenes.mixed_scene(64, 64, 24, seed=4), scenes.config2()[1]),
+        "prefiltered48": scenes.prefiltered_scene(48, 48, 8, 4, seed=5, step=8.0, radius=(4.0, 14.0),
+                                                  seg_range=(2, 6)),
+        "prefiltered_mixed48": (scenes.mixed_scene(48, 48, 16, seed=6), _prefiltered_opts()),
     }
 
 
