@@ -1,18 +1,23 @@
 """Benchmark: forward + backward pixel-samples/s and Adam steps/s on one MI355X.
 
-Workload (the headline configuration): 1024x1024 canvas, 1000 closed cubic
-paths of 3-8 segments (centre uniform, control points centre + U(-64,64)^2),
-solid RGBA fills with alpha in [0.3, 1), white background, 2x2 jittered
-samples, box filter r = 0.5, non-prefiltered (boundary sampling on), seed 0.
-d_render = 2 (image - target), the target rendered from the same scene with
-path points jittered by N(0, 2 px).
+Default workload (the headline configuration): 1024x1024 canvas, 1000 closed
+cubic paths of 3-8 segments (centre uniform, control points centre +
+U(-64,64)^2), solid RGBA fills with alpha in [0.3, 1), white background, 2x2
+jittered samples, box filter r = 0.5, non-prefiltered (boundary sampling on),
+seed 0.  d_render = 2 (image - target), the target rendered from the same
+scene with path points jittered by N(0, 2 px).
 
 One step = weights + forward + resolve + weights + interior backward +
 boundary sampling over all 4,194,304 pixel-samples, scene resident in HBM.
 `value` = pixel-samples / s of that step.  Adam steps/s (pack on the host,
 upload, forward, loss gradient, backward, Adam update) is reported beside it.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+--workload picks another configuration for the same measurement (one JSON
+line each): c1 (256x256, 64 cubic paths, 4x4 spp), c2 (512x512, 512 mixed
+shapes, tent filter), c3 (1024x1024 prefiltered: 2000 stroked cubic paths +
+500 polygons, 1 spp), c5 (2048x2048, 10k cubic paths, 2x2 spp).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -24,6 +29,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "diffvg-rs_amd"))
 
 METRIC = "fwd+bwd pixel-samples/s and Adam steps/s @1024², 1k paths, grad err ≤1e-3"
+# name -> (scenes function, description)
+WORKLOADS = {
+    "headline": ("headline", "headline: 1024x1024, 1000 closed cubic paths (3-8 segments), 2x2 jittered spp, "
+                             "box r=0.5, non-prefiltered fwd + interior bwd + boundary sampling"),
+    "c1": ("config1", "256x256, 64 closed cubic paths, 4x4 jittered spp, box r=0.5, non-prefiltered"),
+    "c2": ("config2", "512x512, 512 mixed shapes (circles/ellipses/rects/paths, 30% stroked, gradients), "
+                      "2x2 spp, tent r=1, non-prefiltered"),
+    "c3": ("config3", "1024x1024 prefiltered (analytic coverage), 2000 stroked open cubic paths + 500 filled "
+                      "polygons, 1 spp"),
+    "c5": ("config5", "2048x2048, 10000 closed cubic paths, 2x2 jittered spp, box r=0.5, non-prefiltered"),
+}
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -39,6 +55,7 @@ def main():
     ap.add_argument("--adam-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -58,11 +75,13 @@ def main():
     r = Renderer(0)
     r.set_stream(stream.cuda_stream)
 
-    sc, opts = scenes.headline(seed=0)
+    fn_name, workload_desc = WORKLOADS[args.workload]
+    headline = args.workload == "headline"
+    sc, opts = getattr(scenes, fn_name)(seed=0)
     W, H = sc.width, sc.height
     spp = opts.samples_x * opts.samples_y
     n_samples = W * H * spp
-    log(f"bench: headline {W}x{H}, {len(sc.shapes)} paths, spp {spp}")
+    log(f"bench: {args.workload} {W}x{H}, {len(sc.shapes)} shapes, spp {spp}")
     prep = PreparedScene(sc, opts)
     ds = DeviceScene(r, prep)
     tgt_prep = PreparedScene(scenes.jittered(sc, 2.0, seed=1), opts)
@@ -114,13 +133,15 @@ def main():
     torch.cuda.synchronize()
 
     # Adam steps/s: host packing + upload + forward + loss + backward + update
-    loop = AdamLoop(r, sc, opts, tgt, lr_points=0.5, lr_colors=1e-2)
-    loop.step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    losses = [loop.step() for _ in range(args.adam_steps)]
-    adam_s = (time.perf_counter() - t1) / args.adam_steps
-    log(f"bench: Adam {1.0 / adam_s:.2f} steps/s, losses {losses}")
+    adam_s = None
+    if headline and args.adam_steps > 0:
+        loop = AdamLoop(r, sc, opts, tgt, lr_points=0.5, lr_colors=1e-2)
+        loop.step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        losses = [loop.step() for _ in range(args.adam_steps)]
+        adam_s = (time.perf_counter() - t1) / args.adam_steps
+        log(f"bench: Adam {1.0 / adam_s:.2f} steps/s, losses {losses}")
 
     # per-kernel durations from HIP events recorded on the kernels' stream
     r.set_timing(True)
@@ -135,7 +156,7 @@ def main():
     log("bench: per-kernel ms " + json.dumps({k: round(v, 4) for k, v in per_kernel_ms.items()}))
 
     roofline = None
-    flops_path = os.path.join(ROOT, "profiles", "flops_headline.json")
+    flops_path = os.path.join(ROOT, "profiles", f"flops_{args.workload}.json")
     if os.path.exists(flops_path):
         with open(flops_path) as f:
             fl = json.load(f)
@@ -146,7 +167,7 @@ def main():
         ms = per_kernel_ms[dom]
         achieved = alg[dom] / (ms * 1e-3) / 1e12
         traffic = None
-        tr_path = os.path.join(ROOT, "profiles", "traffic_headline.json")
+        tr_path = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
         if os.path.exists(tr_path):
             with open(tr_path) as f:
                 traffic = json.load(f).get("traffic_bytes", {}).get(dom)
@@ -156,10 +177,10 @@ def main():
                     "per_kernel_tflops": {k: round(alg[k] / (per_kernel_ms[k] * 1e-3) / 1e12, 3)
                                           for k in alg if per_kernel_ms.get(k)},
                     "note": "peak is MI355X fp32 (vector) 157.3 TFLOP/s: the path is scalar fp32 VALU work, "
-                            "no MFMA; algorithmic flops per sample from profiles/flops_headline.json"}
+                            f"no MFMA; algorithmic flops per sample from profiles/flops_{args.workload}.json"}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if headline and not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py
         from pydvg.scene import SceneDesc
@@ -169,13 +190,11 @@ def main():
                "sample": cpu["sample"]}
 
     out = {
-        "metric": METRIC, "value": value, "unit": "pixel-samples/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC if headline else f"fwd+bwd pixel-samples/s ({args.workload})", "value": value, "unit": "pixel-samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "headline: 1024x1024, 1000 closed cubic paths (3-8 segments), 2x2 jittered spp, "
-                               "box r=0.5, non-prefiltered fwd + interior bwd + boundary sampling",
-                   "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu"},
-        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": 1.0 / adam_s,
+        "config": {"workload": workload_desc, "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu"},
+        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None,
         "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
