@@ -722,9 +722,32 @@ DVG_DEV uint32_t walk_window(const DevScene &S, const uint32_t *list, uint32_t n
     return count;
 }
 
-// d_sample_paint for gradient paints (paint.rs:77-280), into global memory.
-DVG_DEV void d_gradient_paint(const DevScene &S, const DevGrads &G, uint32_t gidx, float px, float py, float4 d,
-                              float &dtx, float &dty) {
+// Paint-gradient staging.  A tile's list entries e < LDS_GROUP_CAP get LDS
+// slots for their fill and stroke paint adjoints (solid: 4 floats RGBA;
+// gradient: 4 floats for p0..p3, then one offset and 4 colour floats per
+// stop), allocated by a prefix sum over the entries; an entry whose slot does
+// not fit in STAGE_FLOATS adds to global memory directly.  The slots are
+// flushed with one global atomic per non-zero float at the end of the tile.
+constexpr uint32_t STAGE_FLOATS = 4096;
+constexpr uint32_t STAGE_NONE = 0xffffffffu;
+
+template <bool LDS>
+DVG_DEV void add_f(float *p, float v) {
+    if (LDS) atomicAdd(p, v);
+    else unsafeAtomicAdd(p, v);
+}
+
+DVG_DEV uint32_t paint_slot_size(const DevScene &S, uint32_t kind, uint32_t gidx) {
+    if (kind == DVG_PAINT_SOLID) return 4u;
+    if (kind == DVG_PAINT_LINEAR || kind == DVG_PAINT_RADIAL) return 4u + 5u * (uint32_t)S.gradient_data[gidx * 8u + 6u];
+    return 0u;
+}
+
+// d_sample_paint for gradient paints (paint.rs:77-280).  dgp receives the
+// p0..p3 adjoint (4 floats), doffs the stop offsets, dcols the stop colours.
+template <bool LDS>
+DVG_DEV void d_gradient_paint(const DevScene &S, uint32_t gidx, float px, float py, float4 d, float *dgp,
+                              float *doffs, float *dcols, float &dtx, float &dty) {
     const float *gd = S.gradient_data + gidx * 8u;
     uint32_t gtype = (uint32_t)gd[0];
     float p0 = gd[1], p1 = gd[2], p2 = gd[3], p3 = gd[4];
@@ -744,10 +767,9 @@ DVG_DEV void d_gradient_paint(const DevScene &S, const DevGrads &G, uint32_t gid
         ny = offy / p3;
         t = sqrtf(nx * nx + ny * ny);
     }
-    float *dsc = G.d_stop_colors + so * 4u;
     if (t < offs[0]) {
-        unsafeAtomicAdd(dsc + 0, d.x); unsafeAtomicAdd(dsc + 1, d.y);
-        unsafeAtomicAdd(dsc + 2, d.z); unsafeAtomicAdd(dsc + 3, d.w);
+        add_f<LDS>(dcols + 0, d.x); add_f<LDS>(dcols + 1, d.y);
+        add_f<LDS>(dcols + 2, d.z); add_f<LDS>(dcols + 3, d.w);
         return;
     }
     for (uint32_t i = 0; i + 1 < sc; ++i) {
@@ -755,32 +777,31 @@ DVG_DEV void d_gradient_paint(const DevScene &S, const DevGrads &G, uint32_t gid
         if (t >= cu && t < nxo) {
             float tt = (t - cu) / (nxo - cu);
             const float *c0 = cols + i * 4u, *c1 = c0 + 4;
-            float *d0 = dsc + i * 4u, *d1 = d0 + 4;
-            unsafeAtomicAdd(d0 + 0, d.x * (1.0f - tt)); unsafeAtomicAdd(d0 + 1, d.y * (1.0f - tt));
-            unsafeAtomicAdd(d0 + 2, d.z * (1.0f - tt)); unsafeAtomicAdd(d0 + 3, d.w * (1.0f - tt));
-            unsafeAtomicAdd(d1 + 0, d.x * tt); unsafeAtomicAdd(d1 + 1, d.y * tt);
-            unsafeAtomicAdd(d1 + 2, d.z * tt); unsafeAtomicAdd(d1 + 3, d.w * tt);
+            float *d0 = dcols + i * 4u, *d1 = d0 + 4;
+            add_f<LDS>(d0 + 0, d.x * (1.0f - tt)); add_f<LDS>(d0 + 1, d.y * (1.0f - tt));
+            add_f<LDS>(d0 + 2, d.z * (1.0f - tt)); add_f<LDS>(d0 + 3, d.w * (1.0f - tt));
+            add_f<LDS>(d1 + 0, d.x * tt); add_f<LDS>(d1 + 1, d.y * tt);
+            add_f<LDS>(d1 + 2, d.z * tt); add_f<LDS>(d1 + 3, d.w * tt);
             float d_tt = d.x * (c1[0] - c0[0]) + d.y * (c1[1] - c0[1]) + d.z * (c1[2] - c0[2]) + d.w * (c1[3] - c0[3]);
             float dof = nxo - cu;
             if (fabsf(dof) > 0.f) {
-                unsafeAtomicAdd(G.d_stop_offsets + so + i, d_tt * (tt - 1.0f) / dof);
-                unsafeAtomicAdd(G.d_stop_offsets + so + i + 1, -d_tt * tt / dof);
+                add_f<LDS>(doffs + i, d_tt * (tt - 1.0f) / dof);
+                add_f<LDS>(doffs + i + 1, -d_tt * tt / dof);
             }
             float d_t = d_tt / dof;
-            float *dg = G.d_gradient_data + gidx * 8u;
             if (gtype == 0) {
                 float dbx = d_t * (-(px - p0) - (p2 - p0)) / denom, dby = d_t * (-(py - p1) - (p3 - p1)) / denom;
                 float dex = d_t * (px - p0) / denom, dey = d_t * (py - p1) / denom;
                 float d_l = -d_t * t / denom;
                 float l2 = (p2 - p0) * (p2 - p0) + (p3 - p1) * (p3 - p1);
                 if (l2 > 1.0e-3f) {
-                    unsafeAtomicAdd(dg + 1, dbx + 2.0f * d_l * (p0 - p2));
-                    unsafeAtomicAdd(dg + 2, dby + 2.0f * d_l * (p1 - p3));
-                    unsafeAtomicAdd(dg + 3, dex + 2.0f * d_l * (p2 - p0));
-                    unsafeAtomicAdd(dg + 4, dey + 2.0f * d_l * (p3 - p1));
+                    add_f<LDS>(dgp + 0, dbx + 2.0f * d_l * (p0 - p2));
+                    add_f<LDS>(dgp + 1, dby + 2.0f * d_l * (p1 - p3));
+                    add_f<LDS>(dgp + 2, dex + 2.0f * d_l * (p2 - p0));
+                    add_f<LDS>(dgp + 3, dey + 2.0f * d_l * (p3 - p1));
                 } else {
-                    unsafeAtomicAdd(dg + 1, dbx); unsafeAtomicAdd(dg + 2, dby);
-                    unsafeAtomicAdd(dg + 3, dex); unsafeAtomicAdd(dg + 4, dey);
+                    add_f<LDS>(dgp + 0, dbx); add_f<LDS>(dgp + 1, dby);
+                    add_f<LDS>(dgp + 2, dex); add_f<LDS>(dgp + 3, dey);
                 }
                 dtx += dbx + dex;
                 dty += dby + dey;
@@ -794,19 +815,116 @@ DVG_DEV void d_gradient_paint(const DevScene &S, const DevGrads &G, uint32_t gid
                     dny = ny * (2.0f * dls);
                 }
                 float dox = dnx / p2, doy = dny / p3;
-                unsafeAtomicAdd(dg + 1, -dox);
-                unsafeAtomicAdd(dg + 2, -doy);
-                unsafeAtomicAdd(dg + 3, -dnx * offx / (p2 * p2));
-                unsafeAtomicAdd(dg + 4, -dny * offy / (p3 * p3));
+                add_f<LDS>(dgp + 0, -dox);
+                add_f<LDS>(dgp + 1, -doy);
+                add_f<LDS>(dgp + 2, -dnx * offx / (p2 * p2));
+                add_f<LDS>(dgp + 3, -dny * offy / (p3 * p3));
                 dtx += -dox;
                 dty += -doy;
             }
             return;
         }
     }
-    float *dl = dsc + (sc - 1) * 4u;
-    unsafeAtomicAdd(dl + 0, d.x); unsafeAtomicAdd(dl + 1, d.y);
-    unsafeAtomicAdd(dl + 2, d.z); unsafeAtomicAdd(dl + 3, d.w);
+    float *dl = dcols + (sc - 1) * 4u;
+    add_f<LDS>(dl + 0, d.x); add_f<LDS>(dl + 1, d.y);
+    add_f<LDS>(dl + 2, d.z); add_f<LDS>(dl + 3, d.w);
+}
+
+// Paint adjoint of fragment (list entry e, stroke or fill) at (px, py).
+DVG_DEV void paint_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, const uint32_t *s_off,
+                           const uint32_t *list, uint32_t e, bool is_stroke, float4 dv, float px, float py, float &dtx,
+                           float &dty) {
+    uint32_t g = list[e];
+    const float *gd = S.group_data + g * 16u;
+    uint32_t kind = (uint32_t)gd[is_stroke ? 4 : 2];
+    uint32_t off = e < LDS_GROUP_CAP ? s_off[2u * e + (is_stroke ? 1u : 0u)] : STAGE_NONE;
+    if (kind == DVG_PAINT_SOLID) {
+        if (off != STAGE_NONE) {
+            float *sp = s_stage + off;
+            atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
+            atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
+        } else {
+            float *dgp = G.d_group_data + g * 16u + (is_stroke ? 12u : 8u);
+            unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
+            unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
+        }
+    } else if (kind == DVG_PAINT_LINEAR || kind == DVG_PAINT_RADIAL) {
+        uint32_t gidx = (uint32_t)gd[is_stroke ? 5 : 3];
+        if (off != STAGE_NONE) {
+            uint32_t sc = (uint32_t)S.gradient_data[gidx * 8u + 6u];
+            float *b = s_stage + off;
+            d_gradient_paint<true>(S, gidx, px, py, dv, b, b + 4, b + 4 + sc, dtx, dty);
+        } else {
+            uint32_t so = (uint32_t)S.gradient_data[gidx * 8u + 5u];
+            d_gradient_paint<false>(S, gidx, px, py, dv, G.d_gradient_data + gidx * 8u + 1u, G.d_stop_offsets + so,
+                                    G.d_stop_colors + so * 4u, dtx, dty);
+        }
+    }
+}
+
+// Slot allocation for a tile (all threads call it; wave 0 computes).
+DVG_DEV void stage_build(const DevScene &S, const uint32_t *list, uint32_t n, uint32_t *s_off, float *s_stage,
+                         uint32_t *s_total) {
+    uint32_t m = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
+    if (threadIdx.x < 64) {
+        uint32_t lane = threadIdx.x;
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < m; base += 64) {
+            uint32_t e = base + lane;
+            uint32_t sz0 = 0, sz1 = 0;
+            if (e < m) {
+                const float *gd = S.group_data + list[e] * 16u;
+                sz0 = paint_slot_size(S, (uint32_t)gd[2], (uint32_t)gd[3]);
+                sz1 = paint_slot_size(S, (uint32_t)gd[4], (uint32_t)gd[5]);
+            }
+            uint32_t sz = sz0 + sz1, v = sz;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t t = __shfl_up(v, o, 64);
+                if ((int)lane >= o) v += t;
+            }
+            uint32_t start = carry + v - sz;
+            if (e < m) {
+                bool fits = start + sz <= STAGE_FLOATS;
+                s_off[2u * e] = (fits && sz0) ? start : STAGE_NONE;
+                s_off[2u * e + 1u] = (fits && sz1) ? start + sz0 : STAGE_NONE;
+            }
+            carry += __shfl(v, 63, 64);
+        }
+        if (lane == 0) *s_total = carry < STAGE_FLOATS ? carry : STAGE_FLOATS;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < *s_total; i += BLOCK) s_stage[i] = 0.f;
+    __syncthreads();
+}
+
+// Flush of the staged paint adjoints (after a barrier).
+DVG_DEV void stage_flush(const DevScene &S, const DevGrads &G, const uint32_t *list, uint32_t n,
+                         const uint32_t *s_off, const float *s_stage) {
+    uint32_t m = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
+    for (uint32_t i = threadIdx.x; i < 2u * m; i += BLOCK) {
+        uint32_t off = s_off[i];
+        if (off == STAGE_NONE) continue;
+        uint32_t e = i >> 1;
+        bool is_stroke = (i & 1u) != 0;
+        uint32_t g = list[e];
+        const float *gd = S.group_data + g * 16u;
+        uint32_t kind = (uint32_t)gd[is_stroke ? 4 : 2];
+        const float *b = s_stage + off;
+        if (kind == DVG_PAINT_SOLID) {
+            float *dst = G.d_group_data + g * 16u + (is_stroke ? 12u : 8u);
+            for (int c = 0; c < 4; ++c)
+                if (b[c] != 0.f) unsafeAtomicAdd(dst + c, b[c]);
+        } else {
+            uint32_t gidx = (uint32_t)gd[is_stroke ? 5 : 3];
+            uint32_t so = (uint32_t)S.gradient_data[gidx * 8u + 5u], sc = (uint32_t)S.gradient_data[gidx * 8u + 6u];
+            for (int j = 0; j < 4; ++j)
+                if (b[j] != 0.f) unsafeAtomicAdd(G.d_gradient_data + gidx * 8u + 1u + j, b[j]);
+            for (uint32_t k = 0; k < sc; ++k)
+                if (b[4 + k] != 0.f) unsafeAtomicAdd(G.d_stop_offsets + so + k, b[4 + k]);
+            for (uint32_t k = 0; k < 4u * sc; ++k)
+                if (b[4 + sc + k] != 0.f) unsafeAtomicAdd(G.d_stop_colors + so * 4u + k, b[4 + sc + k]);
+        }
+    }
 }
 
 DVG_DEV float wave_sum(float v) {
@@ -814,19 +932,61 @@ DVG_DEV float wave_sum(float v) {
     return v;
 }
 
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(DevScene S, const float *__restrict__ weight,
-                                                             const float *__restrict__ d_render, DevGrads G) {
-    __shared__ float s_paint[LDS_GROUP_CAP * 8];
+// Reverse-mode step of one fragment (sampling.rs:177-218): colour adjoint of
+// fragment (e, stroke) with colour c over prefix colour pc; the running
+// adjoint (dr, dg, db, da) moves to the prefix.
+DVG_DEV void frag_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, const uint32_t *s_off,
+                          const uint32_t *list, uint32_t e, bool is_stroke, float4 c, float4 pc, float px, float py,
+                          float &dr, float &dg, float &db, float &da, float &dtx, float &dty) {
+    float a = c.w;
+    float d_prev_a = da * (1.0f - a);
+    float d_alpha = da * (1.0f - pc.w) + (dr * (c.x - pc.x) + dg * (c.y - pc.y) + db * (c.z - pc.z));
+    float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha);
+    paint_adjoint(S, G, s_stage, s_off, list, e, is_stroke, dv, px, py, dtx, dty);
+    dr = dr * (1.0f - a); dg = dg * (1.0f - a); db = db * (1.0f - a); da = d_prev_a;
+}
+
+// Fragment flags of one sample as a bit set over the tile list: bit 2e is
+// the stroke of list entry e, bit 2e + 1 its fill, so bit order is
+// compositing order.  `mask` is the lane's column of an LDS table (stride
+// BLOCK); MASK_WORDS words cover lists of up to 16 * MASK_WORDS groups.
+constexpr int MASK_WORDS = FRAG_WINDOW;
+constexpr uint32_t MASK_GROUPS = 16u * MASK_WORDS;
+
+DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t *mask,
+                           float4 &fin) {
+    uint32_t nw = (2u * n + 31u) / 32u;
+    for (uint32_t w = 0; w < nw; ++w) mask[w * BLOCK] = 0u;
+    uint32_t count = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        uint32_t g = list[e];
+        bool st, fi;
+        group_frags(S, g, px, py, st, fi);
+        if (!(st || fi)) continue;
+        uint32_t bits = (st ? 1u : 0u) | (fi ? 2u : 0u);
+        uint32_t bit = 2u * e;
+        mask[(bit >> 5) * BLOCK] |= bits << (bit & 31u);
+        if (st) { blend_cpu(fin, frag_color(S, g, true, px, py)); ++count; }
+        if (fi) { blend_cpu(fin, frag_color(S, g, false, px, py)); ++count; }
+    }
+    return count;
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(
+    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G) {
+    __shared__ float s_stage[STAGE_FLOATS];
+    __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
+    __shared__ uint32_t s_total;
     __shared__ float s_red[4][6];
+    // per lane: fragment bit set (lists <= MASK_GROUPS) or fragment window
     __shared__ uint32_t s_ids[FRAG_WINDOW * BLOCK];
     uint32_t *ids = s_ids + threadIdx.x;
     uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
-    uint32_t nlds = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
-    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) s_paint[i] = 0.f;
-    __syncthreads();
+    bool use_mask = n <= MASK_GROUPS;
+    stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
     uint32_t total = TILE * TILE * S.spp;
     float Wf = (float)S.W, Hf = (float)S.H;
@@ -844,7 +1004,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
         float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
         float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
         float4 pre = bg, fin = bg;
-        uint32_t nf = walk_window(S, list, n, px, py, 0, ids, pre, fin);
+        uint32_t nf = use_mask ? walk_mask(S, list, n, px, py, ids, fin)
+                               : walk_window(S, list, n, px, py, 0, ids, pre, fin);
         float4 col;
         float dtx = 0.f, dty = 0.f;
         if (nf == 0) {
@@ -870,48 +1031,56 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
                 dr = dr * sc; dg = dg * sc; db = db * sc;
                 da -= dot / fa;
             }
-            uint32_t nwin = (nf + FRAG_WINDOW - 1) / FRAG_WINDOW;
-            for (int w = (int)nwin - 1; w >= 0; --w) {
-                uint32_t wstart = (uint32_t)w * FRAG_WINDOW;
-                float4 wpre = bg;
-                if (nwin > 1) {
-                    float4 f2 = bg;
-                    wpre = bg;
-                    walk_window(S, list, n, px, py, wstart, ids, wpre, f2);
-                }
-                uint32_t m = min((uint32_t)FRAG_WINDOW, nf - wstart);
-                for (int i = (int)m - 1; i >= 0; --i) {
-                    // prefix colour before fragment i of this window
-                    float4 pc = wpre;
-                    for (int j = 0; j < i; ++j) {
-                        uint32_t id = ids[j * BLOCK];
-                        blend_cpu(pc, frag_color(S, list[id >> 1], (id & 1u) != 0, px, py));
-                    }
-                    uint32_t id = ids[i * BLOCK];
-                    uint32_t e = id >> 1;
-                    bool is_stroke = (id & 1u) != 0;
-                    uint32_t g = list[e];
-                    float4 c = frag_color(S, g, is_stroke, px, py);
-                    float a = c.w;
-                    float d_prev_a = da * (1.0f - a);
-                    float d_alpha = da * (1.0f - pc.w) + (dr * (c.x - pc.x) + dg * (c.y - pc.y) + db * (c.z - pc.z));
-                    float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha);
-                    const float *gd = S.group_data + g * 16u;
-                    uint32_t kind = (uint32_t)gd[is_stroke ? 4 : 2];
-                    if (kind == DVG_PAINT_SOLID) {
-                        if (e < LDS_GROUP_CAP) {
-                            float *sp = s_paint + e * 8u + (is_stroke ? 4u : 0u);
-                            atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
-                            atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
-                        } else {
-                            float *dgp = G.d_group_data + g * 16u + (is_stroke ? 12u : 8u);
-                            unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
-                            unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
+            if (use_mask) {
+                // fragments in reverse order; each prefix colour is recomposited
+                // from the bit set (no second walk over the geometry)
+                uint32_t nw = (2u * n + 31u) / 32u;
+                for (int w = (int)nw - 1; w >= 0; --w) {
+                    uint32_t word = ids[w * BLOCK];
+                    while (word) {
+                        uint32_t b = 31u - (uint32_t)__clz(word);
+                        word &= ~(1u << b);
+                        uint32_t bit = (uint32_t)w * 32u + b;
+                        float4 pc = bg;
+                        for (int w2 = 0; w2 <= w; ++w2) {
+                            uint32_t m2 = ids[w2 * BLOCK];
+                            if (w2 == w) m2 &= (1u << b) - 1u;
+                            while (m2) {
+                                uint32_t b2 = (uint32_t)__ffs(m2) - 1u;
+                                m2 &= m2 - 1u;
+                                uint32_t bit2 = (uint32_t)w2 * 32u + b2;
+                                blend_cpu(pc, frag_color(S, list[bit2 >> 1], (bit2 & 1u) == 0u, px, py));
+                            }
                         }
-                    } else {
-                        d_gradient_paint(S, G, (uint32_t)gd[is_stroke ? 5 : 3], px, py, dv, dtx, dty);
+                        uint32_t e = bit >> 1;
+                        bool is_stroke = (bit & 1u) == 0u;
+                        float4 c = frag_color(S, list[e], is_stroke, px, py);
+                        frag_adjoint(S, G, s_stage, s_off, list, e, is_stroke, c, pc, px, py, dr, dg, db, da, dtx, dty);
                     }
-                    dr = dr * (1.0f - a); dg = dg * (1.0f - a); db = db * (1.0f - a); da = d_prev_a;
+                }
+            } else {
+                uint32_t nwin = (nf + FRAG_WINDOW - 1) / FRAG_WINDOW;
+                for (int w = (int)nwin - 1; w >= 0; --w) {
+                    uint32_t wstart = (uint32_t)w * FRAG_WINDOW;
+                    float4 wpre = bg;
+                    if (nwin > 1) {
+                        float4 f2 = bg;
+                        walk_window(S, list, n, px, py, wstart, ids, wpre, f2);
+                    }
+                    uint32_t m = min((uint32_t)FRAG_WINDOW, nf - wstart);
+                    for (int i = (int)m - 1; i >= 0; --i) {
+                        // prefix colour before fragment i of this window
+                        float4 pc = wpre;
+                        for (int j = 0; j < i; ++j) {
+                            uint32_t id = ids[j * BLOCK];
+                            blend_cpu(pc, frag_color(S, list[id >> 1], (id & 1u) != 0, px, py));
+                        }
+                        uint32_t id = ids[i * BLOCK];
+                        uint32_t e = id >> 1;
+                        bool is_stroke = (id & 1u) != 0;
+                        float4 c = frag_color(S, list[e], is_stroke, px, py);
+                        frag_adjoint(S, G, s_stage, s_off, list, e, is_stroke, c, pc, px, py, dr, dg, db, da, dtx, dty);
+                    }
                 }
             }
             acc_bg[0] += dr; acc_bg[1] += dg; acc_bg[2] += db; acc_bg[3] += da;
@@ -943,14 +1112,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
             unsafeAtomicAdd(G.d_filter_radius, r);
         }
     }
-    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) {
-        float v2 = s_paint[i];
-        if (v2 != 0.f) {
-            uint32_t e = i >> 3, c = i & 7u;
-            uint32_t g = list[e];
-            unsafeAtomicAdd(G.d_group_data + g * 16u + 8u + c, v2);
-        }
-    }
+    stage_flush(S, G, list, n, s_off, s_stage);
 }
 
 // 3x3 helpers on affine matrices given by their first two rows.
@@ -1598,7 +1760,9 @@ DVG_DEV void d_distance_query(const DevScene &S, const DevGrads &G, uint32_t g, 
 
 __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, const float *__restrict__ weight,
                                                                 const float *__restrict__ d_render, DevGrads G) {
-    __shared__ float s_paint[LDS_GROUP_CAP * 8];
+    __shared__ float s_stage[STAGE_FLOATS];
+    __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
+    __shared__ uint32_t s_total;
     __shared__ float s_red[4][6];
     __shared__ uint32_t s_ids[PF_WINDOW * BLOCK];
     __shared__ float s_cov[PF_WINDOW * BLOCK];
@@ -1608,9 +1772,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
-    uint32_t nlds = n < LDS_GROUP_CAP ? n : LDS_GROUP_CAP;
-    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) s_paint[i] = 0.f;
-    __syncthreads();
+    stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
     uint32_t total = TILE * TILE * S.spp;
     float Wf = (float)S.W, Hf = (float)S.H;
@@ -1689,20 +1851,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
                         if (ok && wcov != 0.f) {
                             float d_w = wcov > 0.f ? (a / wcov) * d_alpha : 0.f;
                             float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha * wcov);
-                            const float *gd = S.group_data + g * 16u;
-                            if ((uint32_t)gd[4] == DVG_PAINT_SOLID) {
-                                if (e < LDS_GROUP_CAP) {
-                                    float *sp = s_paint + e * 8u + 4u;
-                                    atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
-                                    atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
-                                } else {
-                                    float *dgp = G.d_group_data + g * 16u + 12u;
-                                    unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
-                                    unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
-                                }
-                            } else {
-                                d_gradient_paint(S, G, (uint32_t)gd[5], px, py, dv, dtx, dty);
-                            }
+                            paint_adjoint(S, G, s_stage, s_off, list, e, true, dv, px, py, dtx, dty);
                             float d_ap = d_smoothstep_unit(ap, d_w);
                             float d_am = -d_smoothstep_unit(am, d_w);
                             float d_d = d_ap + d_am;
@@ -1716,20 +1865,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
                         if (wcov != 0.f) {
                             float d_w = wcov > 0.f ? (a / wcov) * d_alpha : 0.f;
                             float4 dv = make_float4(dr * a, dg * a, db * a, d_alpha * wcov);
-                            const float *gd = S.group_data + g * 16u;
-                            if ((uint32_t)gd[2] == DVG_PAINT_SOLID) {
-                                if (e < LDS_GROUP_CAP) {
-                                    float *sp = s_paint + e * 8u;
-                                    atomicAdd(sp + 0, dv.x); atomicAdd(sp + 1, dv.y);
-                                    atomicAdd(sp + 2, dv.z); atomicAdd(sp + 3, dv.w);
-                                } else {
-                                    float *dgp = G.d_group_data + g * 16u + 8u;
-                                    unsafeAtomicAdd(dgp + 0, dv.x); unsafeAtomicAdd(dgp + 1, dv.y);
-                                    unsafeAtomicAdd(dgp + 2, dv.z); unsafeAtomicAdd(dgp + 3, dv.w);
-                                }
-                            } else {
-                                d_gradient_paint(S, G, (uint32_t)gd[3], px, py, dv, dtx, dty);
-                            }
+                            paint_adjoint(S, G, s_stage, s_off, list, e, false, dv, px, py, dtx, dty);
                             float d_d = d_smoothstep_unit(d, d_w);
                             if (d < 0.f) d_d = -d_d;
                             if (absf_(d_d) > 1.0e-10f && within && group_closest(S, g, px, py, 1.0f, h))
@@ -1767,14 +1903,7 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
             unsafeAtomicAdd(G.d_filter_radius, r);
         }
     }
-    for (uint32_t i = threadIdx.x; i < nlds * 8; i += BLOCK) {
-        float v2 = s_paint[i];
-        if (v2 != 0.f) {
-            uint32_t e = i >> 3, c = i & 7u;
-            uint32_t g = list[e];
-            unsafeAtomicAdd(G.d_group_data + g * 16u + 8u + c, v2);
-        }
-    }
+    stage_flush(S, G, list, n, s_off, s_stage);
 }
 
 // ---------------------------------------------------------------------------
