@@ -15,6 +15,10 @@ constexpr int LDS_GROUP_CAP = 512;  // tile groups whose paint grads are staged 
 // PreparedBackwardScene / BoundarySamplingData plus the tile lists.
 struct DevScene {
     uint32_t W, H, nsx, nsy, spp, seed, jitter, prefilter, filter_type;
+    // launch shapes (log2 edges in pixels): samples of a tile are visited in
+    // square blocks of 1 << sblock_log2 pixels; boundary samples are bucketed
+    // by blocks of 1 << bnd_log2 pixels
+    uint32_t sblock_log2, bnd_log2;
     float filter_radius;
     int ri;
     float bg[4];            // straight RGBA
@@ -66,12 +70,13 @@ void launch_backward_interior(const DevScene &S, const float *weight, const floa
 // Prefiltered (analytic coverage) colour adjoint with its distance adjoint.
 void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                                  hipStream_t stream);
-// Boundary samples are sorted by the 4x4 pixel block of their pixel, blocks
-// numbered tile-major (16 per tile), so a wave shares one tile group list and
-// a compact footprint.
-constexpr int BND_BLOCK = 4;
-constexpr int BND_BLOCKS_PER_TILE = (TILE / BND_BLOCK) * (TILE / BND_BLOCK);
-inline uint32_t boundary_buckets(const DevScene &S) { return S.tiles_x * S.tiles_y * BND_BLOCKS_PER_TILE; }
+// Boundary samples are sorted by the pixel block (1 << bnd_log2 pixels a
+// side) of their pixel, blocks numbered tile-major, so a wave shares one tile
+// group list and a compact footprint.
+inline uint32_t boundary_buckets(const DevScene &S) {
+    uint32_t per_row = (uint32_t)TILE >> S.bnd_log2;
+    return S.tiles_x * S.tiles_y * per_row * per_row;
+}
 // Work buffers of the boundary pass: keys/order hold one u32 per sample,
 // counts buckets+1, offsets buckets+2, cursor buckets.
 struct BoundaryWork {
@@ -83,5 +88,45 @@ void launch_boundary_eval(const DevScene &S, const float *weight, const float *d
                           const BoundaryWork &bw, hipStream_t stream);
 // Zero `count` (<= 14) device float arrays of the given lengths.
 void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
+
+// ---- batches: many scenes per launch ---------------------------------------
+// One scene of a batch: its device scene and its work buffers.
+constexpr int BATCH_ZERO_ARRAYS = 17;  // 14 gradient arrays, accum, counts, cursor
+struct BatchScene {
+    DevScene S;
+    float *weight, *accum, *img, *target, *d_render;
+    DevGrads G;
+    uint32_t *keys, *order, *counts, *offsets, *cursor;
+    uint32_t nb;  // boundary buckets
+    float *zp[BATCH_ZERO_ARRAYS];
+    uint32_t zn[BATCH_ZERO_ARRAYS];
+};
+// A unit of work: unit `unit` of scene `scene` in some phase.  Zero-phase
+// units encode (array << 20) | chunk of ZERO_CHUNK floats.
+struct WorkItem {
+    uint32_t scene, unit;
+};
+constexpr uint32_t ZERO_CHUNK = 4096;
+enum BatchPhase : int {
+    PH_ZERO = 0,
+    PH_WEIGHTS_FWD,
+    PH_FORWARD,
+    PH_RESOLVE,
+    PH_WEIGHTS_BWD,
+    PH_INTERIOR,
+    PH_PREFILTERED,
+    PH_BND_KEY,
+    PH_SCAN,
+    PH_BND_SCATTER,
+    PH_BND_EVAL,
+    NUM_PHASES
+};
+// One persistent launch of `grid` workgroups draining the phase's work queue
+// (items[0..n_items), next item = atomicAdd(counter, 1)).  `loss` makes the
+// resolve phase also write d_render = 2 (img - target).
+void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,
+                        uint32_t *counter, uint32_t grid, bool loss, hipStream_t stream);
+// n words of p set to v (a kernel, so that it can be captured in a graph).
+void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream);
 
 }  // namespace dvgk

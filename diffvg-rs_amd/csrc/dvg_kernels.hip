@@ -64,18 +64,19 @@ __global__ void k_bin_write(const int4 *__restrict__ rng, uint32_t G, uint32_t t
     }
 }
 
-// Exclusive scan of n counts into offsets[0..n] by one workgroup of 1024.
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ counts, uint32_t n,
-                                               uint32_t *__restrict__ offsets) {
-    __shared__ uint32_t part[1024];
+// Exclusive scan of n counts into offsets[0..n] by one workgroup of NT
+// threads (each sums a contiguous run, then a scan of the run totals).
+template <int NT>
+DVG_DEV void scan_unit(const uint32_t *__restrict__ counts, uint32_t n, uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t part[NT];
     uint32_t tid = threadIdx.x;
-    uint32_t per = (n + 1023) / 1024;
+    uint32_t per = (n + NT - 1) / NT;
     uint32_t b = tid * per, e = min(n, b + per);
     uint32_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += counts[i];
     part[tid] = s;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
+    for (uint32_t off = 1; off < NT; off <<= 1) {
         uint32_t v = tid >= off ? part[tid - off] : 0u;
         __syncthreads();
         part[tid] += v;
@@ -86,17 +87,26 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ coun
         offsets[i] = run;
         run += counts[i];
     }
-    if (tid == 1023) offsets[n] = part[1023];
+    if (tid == NT - 1) offsets[n] = part[NT - 1];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ counts, uint32_t n,
+                                               uint32_t *__restrict__ offsets) {
+    scan_unit<1024>(counts, n, offsets);
 }
 
 // ---------------------------------------------------------------------------
-// Sample addressing inside a tile: pixel p of the tile is visited in 4x4
-// blocks (16 consecutive p share a block), sample s of pixel p at p*spp + s.
+// Sample addressing inside a tile: pixel p of the tile is visited in square
+// blocks of b = 1 << sblock_log2 pixels a side (b*b consecutive p share a
+// block), sample s of pixel p at p*spp + s.
 DVG_DEV void tile_sample(const DevScene &S, uint32_t tile, uint32_t s, uint32_t &x, uint32_t &y, uint32_t &sx,
                          uint32_t &sy) {
     uint32_t pl = s / S.spp, si = s - pl * S.spp;
-    uint32_t lx = (pl & 3u) | (((pl >> 4) & 3u) << 2);
-    uint32_t ly = ((pl >> 2) & 3u) | ((pl >> 6) << 2);
+    uint32_t L = S.sblock_log2, R = 4u - L;  // TILE = 16 = 1 << 4
+    uint32_t blk = pl >> (2u * L), ip = pl & ((1u << (2u * L)) - 1u);
+    uint32_t lx = ((blk & ((1u << R) - 1u)) << L) | (ip & ((1u << L) - 1u));
+    uint32_t ly = ((blk >> R) << L) | (ip >> L);
     x = (tile % S.tiles_x) * TILE + lx;
     y = (tile / S.tiles_x) * TILE + ly;
     sx = si % S.nsx;
@@ -460,9 +470,11 @@ DVG_DEV float4 background_at(const DevScene &S, uint32_t x, uint32_t y) {
 // ---------------------------------------------------------------------------
 // Filter weight sum of every pixel, gathered from the samples of its
 // neighbourhood (the same sum rasterize_weights_f32 scatters).
+// Unit u covers the 16x16 pixel block u of the canvas (row-major blocks).
 template <bool CPU>
-__global__ void k_weights(DevScene S, float *__restrict__ weight) {
-    uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+DVG_DEV void weights_unit(const DevScene &S, uint32_t u, float *__restrict__ weight) {
+    uint32_t gx = (S.W + 15u) / 16u;
+    uint32_t x = (u % gx) * 16 + (threadIdx.x & 15), y = (u / gx) * 16 + (threadIdx.x >> 4);
     if (x >= S.W || y >= S.H) return;
     float xc = (float)x + 0.5f, yc = (float)y + 0.5f;
     float sum = 0.f;
@@ -482,16 +494,20 @@ __global__ void k_weights(DevScene S, float *__restrict__ weight) {
     weight[(size_t)y * S.W + x] = sum;
 }
 
+template <bool CPU>
+__global__ void k_weights(DevScene S, float *__restrict__ weight) {
+    weights_unit<CPU>(S, blockIdx.x, weight);
+}
+
 // The splat of a tile's samples lands in the tile plus a halo of ri pixels.
 // For ri <= SPLAT_HALO it is summed in LDS and flushed with one global atomic
 // per touched pixel channel; wider filters splat straight to global memory.
 constexpr int SPLAT_HALO = 4;
 constexpr int SPLAT_EDGE = TILE + 2 * SPLAT_HALO;
 
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_forward(DevScene S, const float *__restrict__ weight,
-                                                   float *__restrict__ accum) {
+DVG_DEV void forward_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+                          float *__restrict__ accum) {
     __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
-    uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
@@ -554,10 +570,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWA
     }
 }
 
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_forward(
+    DevScene S, const float *__restrict__ weight, float *__restrict__ accum) {
+    forward_unit(S, blockIdx.x, weight, accum);
+}
+
 // resolve_splat_f32 with the premultiplied background of renderer.rs:499-515.
-__global__ void k_resolve(DevScene S, const float *__restrict__ weight, const float *__restrict__ accum,
-                          float *__restrict__ out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Unit u covers pixels [256 u, 256 u + 256).  With target and d_render set,
+// also writes the L2 loss gradient d_render = 2 (out - target).
+DVG_DEV void resolve_unit(const DevScene &S, uint32_t u, const float *__restrict__ weight,
+                          const float *__restrict__ accum, float *__restrict__ out, const float *__restrict__ target,
+                          float *__restrict__ d_render) {
+    uint32_t i = u * BLOCK + threadIdx.x;
     if (i >= S.W * S.H) return;
     float a = S.bg[3];
     float r = S.bg[0] * a, g = S.bg[1] * a, b = S.bg[2] * a;
@@ -567,6 +591,16 @@ __global__ void k_resolve(DevScene S, const float *__restrict__ weight, const fl
     }
     if (a > 1.0e-8f) { r = r / a; g = g / a; b = b / a; }
     reinterpret_cast<float4 *>(out)[i] = make_float4(r, g, b, a);
+    if (target && d_render) {
+        float4 t = reinterpret_cast<const float4 *>(target)[i];
+        reinterpret_cast<float4 *>(d_render)[i] =
+            make_float4(2.0f * (r - t.x), 2.0f * (g - t.y), 2.0f * (b - t.z), 2.0f * (a - t.w));
+    }
+}
+
+__global__ void k_resolve(DevScene S, const float *__restrict__ weight, const float *__restrict__ accum,
+                          float *__restrict__ out) {
+    resolve_unit(S, blockIdx.x, weight, accum, out, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -972,8 +1006,8 @@ DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, 
     return count;
 }
 
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(
-    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G) {
+DVG_DEV void interior_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+                           const float *__restrict__ d_render, const DevGrads &G) {
     __shared__ float s_stage[STAGE_FLOATS];
     __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
     __shared__ uint32_t s_total;
@@ -981,7 +1015,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
     // per lane: fragment bit set (lists <= MASK_GROUPS) or fragment window
     __shared__ uint32_t s_ids[FRAG_WINDOW * BLOCK];
     uint32_t *ids = s_ids + threadIdx.x;
-    uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
@@ -1113,6 +1146,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
         }
     }
     stage_flush(S, G, list, n, s_off, s_stage);
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(
+    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G) {
+    interior_unit(S, blockIdx.x, weight, d_render, G);
 }
 
 // 3x3 helpers on affine matrices given by their first two rows.
@@ -1758,8 +1796,8 @@ DVG_DEV void d_distance_query(const DevScene &S, const DevGrads &G, uint32_t g, 
     dty -= dpty;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, const float *__restrict__ weight,
-                                                                const float *__restrict__ d_render, DevGrads G) {
+DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+                              const float *__restrict__ d_render, const DevGrads &G) {
     __shared__ float s_stage[STAGE_FLOATS];
     __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
     __shared__ uint32_t s_total;
@@ -1768,7 +1806,6 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
     __shared__ float s_cov[PF_WINDOW * BLOCK];
     uint32_t *ids = s_ids + threadIdx.x;
     float *cov = s_cov + threadIdx.x;
-    uint32_t tile = blockIdx.x;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
@@ -1904,6 +1941,11 @@ __global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, cons
         }
     }
     stage_flush(S, G, list, n, s_off, s_stage);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, const float *__restrict__ weight,
+                                                                const float *__restrict__ d_render, DevGrads G) {
+    prefiltered_unit(S, blockIdx.x, weight, d_render, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -2238,15 +2280,16 @@ DVG_DEV bool boundary_point(const DevScene &S, size_t idx, BPoint &P) {
 
 DVG_DEV uint32_t bucket_of(const DevScene &S, uint32_t x, uint32_t y) {
     uint32_t tile = (y / TILE) * S.tiles_x + x / TILE;
-    uint32_t blk = ((y % TILE) / BND_BLOCK) * (TILE / BND_BLOCK) + (x % TILE) / BND_BLOCK;
-    return tile * BND_BLOCKS_PER_TILE + blk;
+    uint32_t L = S.bnd_log2, per_row = (uint32_t)TILE >> L;
+    uint32_t blk = ((y % TILE) >> L) * per_row + ((x % TILE) >> L);
+    return tile * per_row * per_row + blk;
 }
 
 // Pass 1: bucket key of every boundary sample (nb when it contributes
 // nothing) and the histogram of the contributing ones.
-__global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ counts, uint32_t nb) {
-    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+DVG_DEV void bnd_key_unit(const DevScene &S, uint32_t u, uint32_t *__restrict__ keys, uint32_t *__restrict__ counts,
+                          uint32_t nb) {
+    size_t idx = (size_t)u * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
     BPoint P;
@@ -2258,18 +2301,29 @@ __global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restr
     if (key < nb) atomicAdd(counts + key, 1u);
 }
 
+__global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ counts, uint32_t nb) {
+    bnd_key_unit(S, blockIdx.x, keys, counts, nb);
+}
+
 // Pass 2: counting-sort scatter of sample indices by bucket.
-__global__ __launch_bounds__(BLOCK) void k_bnd_scatter(DevScene S, const uint32_t *__restrict__ keys,
-                                                       const uint32_t *__restrict__ offsets,
-                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order,
-                                                       uint32_t nb) {
-    size_t idx = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+DVG_DEV void bnd_scatter_unit(const DevScene &S, uint32_t u, const uint32_t *__restrict__ keys,
+                              const uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
+                              uint32_t *__restrict__ order, uint32_t nb) {
+    size_t idx = (size_t)u * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
     uint32_t key = keys[idx];
     if (key >= nb) return;
     uint32_t pos = offsets[key] + atomicAdd(cursor + key, 1u);
     order[pos] = (uint32_t)idx;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bnd_scatter(DevScene S, const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order,
+                                                       uint32_t nb) {
+    bnd_scatter_unit(S, blockIdx.x, keys, offsets, cursor, order, nb);
 }
 
 DVG_DEV float sel3(uint32_t i, float a, float b, float c) { return i == 0u ? a : (i == 1u ? b : c); }
@@ -2286,16 +2340,15 @@ DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0
 // Pass 3: boundary samples in tile order.  Consecutive lanes sit on the same
 // few shapes of one tile, so the tile's group list is shared by the wave and
 // the shape gradients are reduced per shape instance before the atomics.
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval(DevScene S, const float *__restrict__ weight,
-                                                    const float *__restrict__ d_render, DevGrads G,
-                                                    const uint32_t *__restrict__ offsets,
-                                                    const uint32_t *__restrict__ order, uint32_t nb) {
+DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restrict__ weight,
+                           const float *__restrict__ d_render, const DevGrads &G, const uint32_t *__restrict__ offsets,
+                           const uint32_t *__restrict__ order, uint32_t nb) {
     const float PI = 3.14159274101257324219f;
     __shared__ BPoint s_pt[BLOCK];
     uint32_t n_valid = offsets[nb];
-    uint32_t pos = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t pos = u * BLOCK + threadIdx.x;
     uint32_t lane = threadIdx.x & 63u;
-    if (blockIdx.x * BLOCK >= n_valid) return;  // whole workgroup past the end
+    if (u * BLOCK >= n_valid) return;  // whole workgroup past the end
     BPoint P;
     bool valid = false;
     float contrib = 0.f, nnx = 0.f, nny = 0.f;
@@ -2481,6 +2534,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUND
     }
 }
 
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval(
+    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
+    const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ order, uint32_t nb) {
+    bnd_eval_unit(S, blockIdx.x, weight, d_render, G, offsets, order, nb);
+}
+
 // Zero up to 14 float arrays in one launch (used instead of memset nodes so
 // that the calls stay capturable in a HIP graph).
 struct ZeroList {
@@ -2492,6 +2551,84 @@ __global__ void k_zero(ZeroList z) {
     float *p = z.p[k];
     if (!p) return;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < z.n[k]; i += gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: persistent kernels over a device work queue.  Every workgroup
+// takes the next item with one atomic, so uneven scenes and tiles balance
+// themselves; the loop ends when the queue is drained (every workgroup
+// reaches the exit test on each iteration).
+
+#define DVG_QUEUE_LOOP(...)                                            \
+    __shared__ uint32_t s_item;                                        \
+    for (;;) {                                                         \
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);         \
+        __syncthreads();                                               \
+        uint32_t it = __builtin_amdgcn_readfirstlane(s_item);          \
+        __syncthreads();                                               \
+        if (it >= n_items) break;                                      \
+        const WorkItem w = items[it];                                  \
+        const BatchScene &B = scenes[w.scene];                         \
+        __VA_ARGS__;                                                   \
+    }
+
+#define DVG_QUEUE_ARGS \
+    const BatchScene *__restrict__ scenes, const WorkItem *__restrict__ items, uint32_t n_items, uint32_t *counter
+
+__global__ __launch_bounds__(BLOCK) void k_batch_zero(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP({
+        uint32_t k = w.unit >> 20, chunk = w.unit & 0xfffffu;
+        float *p = B.zp[k];
+        uint32_t n = B.zn[k];
+        uint32_t b = chunk * ZERO_CHUNK, e = min(n, b + ZERO_CHUNK);
+        for (uint32_t i = b + threadIdx.x; i < e; i += BLOCK) p[i] = 0.f;
+    })
+}
+
+template <bool CPU>
+__global__ __launch_bounds__(BLOCK) void k_batch_weights(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(weights_unit<CPU>(B.S, w.unit, B.weight))
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_batch_forward(
+    DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(forward_unit(B.S, w.unit, B.weight, B.accum))
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_resolve(DVG_QUEUE_ARGS, int loss) {
+    DVG_QUEUE_LOOP(resolve_unit(B.S, w.unit, B.weight, B.accum, B.img, loss ? B.target : nullptr,
+                                loss ? B.d_render : nullptr))
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_batch_interior(
+    DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(interior_unit(B.S, w.unit, B.weight, B.d_render, B.G))
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_prefiltered(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(prefiltered_unit(B.S, w.unit, B.weight, B.d_render, B.G))
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_bnd_key(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(bnd_key_unit(B.S, w.unit, B.keys, B.counts, B.nb))
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_scan(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(scan_unit<BLOCK>(B.counts, B.nb + 1u, B.offsets))
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch_bnd_scatter(DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(bnd_scatter_unit(B.S, w.unit, B.keys, B.offsets, B.cursor, B.order, B.nb))
+}
+
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_batch_bnd_eval(
+    DVG_QUEUE_ARGS) {
+    DVG_QUEUE_LOOP(bnd_eval_unit(B.S, w.unit, B.weight, B.d_render, B.G, B.offsets, B.order, B.nb))
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -2528,8 +2665,9 @@ void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uin
                        reinterpret_cast<const int4 *>(group_aabb), G, tiles_x, tiles_y, offsets, entries);
 }
 void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream) {
-    if (cpu) hipLaunchKernelGGL(k_weights<true>, dim3(cdiv(S.W, 16), cdiv(S.H, 16)), dim3(256), 0, stream, S, weight);
-    else hipLaunchKernelGGL(k_weights<false>, dim3(cdiv(S.W, 16), cdiv(S.H, 16)), dim3(256), 0, stream, S, weight);
+    uint32_t nb = cdiv(S.W, 16) * cdiv(S.H, 16);
+    if (cpu) hipLaunchKernelGGL(k_weights<true>, dim3(nb), dim3(256), 0, stream, S, weight);
+    else hipLaunchKernelGGL(k_weights<false>, dim3(nb), dim3(256), 0, stream, S, weight);
 }
 void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream) {
     hipLaunchKernelGGL(k_forward, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight, accum);
@@ -2566,6 +2704,41 @@ void launch_boundary_eval(const DevScene &S, const float *weight, const float *d
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     hipLaunchKernelGGL(k_bnd_eval, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
                        bw.offsets, bw.order, boundary_buckets(S));
+}
+
+void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,
+                        uint32_t *counter, uint32_t grid, bool loss, hipStream_t stream) {
+    if (n_items == 0 || grid == 0) return;
+    dim3 g(grid), b(BLOCK);
+    switch (phase) {
+        case PH_ZERO: hipLaunchKernelGGL(k_batch_zero, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_WEIGHTS_FWD:
+            hipLaunchKernelGGL(k_batch_weights<false>, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
+        case PH_FORWARD: hipLaunchKernelGGL(k_batch_forward, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_RESOLVE:
+            hipLaunchKernelGGL(k_batch_resolve, g, b, 0, stream, scenes, items, n_items, counter, loss ? 1 : 0);
+            break;
+        case PH_WEIGHTS_BWD:
+            hipLaunchKernelGGL(k_batch_weights<true>, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
+        case PH_INTERIOR: hipLaunchKernelGGL(k_batch_interior, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_PREFILTERED:
+            hipLaunchKernelGGL(k_batch_prefiltered, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
+        case PH_BND_KEY: hipLaunchKernelGGL(k_batch_bnd_key, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_SCAN: hipLaunchKernelGGL(k_batch_scan, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_BND_SCATTER:
+            hipLaunchKernelGGL(k_batch_bnd_scatter, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
+        case PH_BND_EVAL: hipLaunchKernelGGL(k_batch_bnd_eval, g, b, 0, stream, scenes, items, n_items, counter); break;
+        default: break;
+    }
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, stream, p, n, v);
 }
 
 }  // namespace dvgk

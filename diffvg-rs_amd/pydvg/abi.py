@@ -114,6 +114,13 @@ class RenderParamsC(C.Structure):
 
 KERNEL_NAMES = ["zero", "weights", "forward", "resolve", "backward_interior", "boundary_sort", "boundary_eval"]
 
+BATCH_FORWARD, BATCH_LOSS, BATCH_BACKWARD = 1, 2, 4
+
+
+class BatchConfigC(C.Structure):
+    _fields_ = [("sblock_log2", C.c_uint32), ("bnd_log2", C.c_uint32), ("wg_per_cu", C.c_uint32), ("ms", C.c_float)]
+
+
 # Every symbol include/dvg.h declares, with its ctypes signature.
 _vp = C.c_void_p
 SIGNATURES = {
@@ -140,6 +147,17 @@ SIGNATURES = {
     "dvg_sync": (C.c_int32, [_vp]),
     "dvg_ctx_set_timing": (C.c_int32, [_vp, C.c_int]),
     "dvg_kernel_times": (C.c_int32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int, C.c_int]),
+    "dvg_batch_create": (C.c_int32, [_vp, C.c_uint32, C.POINTER(ScenePackC), C.POINTER(BackwardPackC),
+                                     C.POINTER(RenderParamsC), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "dvg_batch_destroy": (None, [_vp]),
+    "dvg_batch_buffers": (C.c_int32, [_vp, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                      C.POINTER(GradsC)]),
+    "dvg_batch_read": (C.c_int32, [_vp, _vp, C.c_uint32, _vp, C.POINTER(GradsC)]),
+    "dvg_batch_step": (C.c_int32, [_vp, _vp, C.c_uint32]),
+    "dvg_batch_set_graph": (C.c_int32, [_vp, C.c_int]),
+    "dvg_batch_get_config": (C.c_int32, [_vp, C.POINTER(BatchConfigC)]),
+    "dvg_batch_set_config": (C.c_int32, [_vp, _vp, C.POINTER(BatchConfigC)]),
+    "dvg_batch_autotune": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(BatchConfigC)]),
     "dvg_version": (C.c_char_p, []),
 }
 

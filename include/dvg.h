@@ -403,6 +403,53 @@ int32_t dvg_ctx_set_timing(dvg_ctx *ctx, int enable);
  * milliseconds and launch counts (arrays of n); reset != 0 clears them. */
 int32_t dvg_kernel_times(dvg_ctx *ctx, double *total_ms, uint32_t *calls, int n, int reset);
 
+/* ---- batches of scenes --------------------------------------------------- */
+/* Many scenes (uneven sizes and modes) stepped together: forward, L2 loss
+ * gradient and backward with one persistent launch per phase over a device
+ * work queue of (scene, tile | sample block) items, heaviest tiles first.
+ * This is the batched step of the scope (dvg_batch_step); the reference runs
+ * one scene per call (renderer.rs:39, backward_gpu.rs:21) and has no
+ * counterpart.  Everything lives in HBM; the caller owns the target images. */
+typedef struct dvg_batch dvg_batch;
+
+#define DVG_BATCH_FORWARD 1u  /* weights, forward, resolve into the batch images */
+#define DVG_BATCH_LOSS 2u     /* d_render = 2 (image - target) (needs FORWARD) */
+#define DVG_BATCH_BACKWARD 4u /* gradients of every scene from its d_render */
+
+/* Launch shapes of a batch (all scenes): sample blocks of 1 << sblock_log2
+ * pixels a side inside a tile (0..4), boundary buckets of 1 << bnd_log2
+ * pixels (1..4), persistent workgroups per compute unit (1..64); ms is the
+ * step time the autotuner measured for them (0 when not measured). */
+typedef struct dvg_batch_config {
+    uint32_t sblock_log2;
+    uint32_t bnd_log2;
+    uint32_t wg_per_cu;
+    float ms;
+} dvg_batch_config;
+
+/* Uploads n scenes (packs as for dvg_device_scene_create) and allocates their
+ * work buffers.  targets: n device pointers to W*H*4 target images (or NULL
+ * when the loss is never asked for); they must outlive the batch. */
+int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, const dvg_backward_pack *bpacks,
+                         const dvg_render_params *params, const float *const *targets, dvg_batch **out);
+void dvg_batch_destroy(dvg_batch *batch);
+/* Device pointers of scene i's image, loss gradient (d_render, writable: the
+ * caller may supply its own before a BACKWARD-only step) and gradients. */
+int32_t dvg_batch_buffers(dvg_batch *batch, uint32_t i, float **img, float **d_render, dvg_grads *grads);
+/* Copies scene i's image (W*H*4, may be NULL) and gradient arrays (NULL
+ * members skipped) to host or device memory, then synchronizes. */
+int32_t dvg_batch_read(dvg_ctx *ctx, dvg_batch *batch, uint32_t i, float *img, const dvg_grads *grads);
+/* One asynchronous step on the context's stream; flags = DVG_BATCH_*. */
+int32_t dvg_batch_step(dvg_ctx *ctx, dvg_batch *batch, uint32_t flags);
+/* enable != 0: the step is captured once per flag set in a HIP graph and
+ * replayed (the context's stream must not be the null stream). */
+int32_t dvg_batch_set_graph(dvg_batch *batch, int enable);
+int32_t dvg_batch_get_config(const dvg_batch *batch, dvg_batch_config *cfg);
+int32_t dvg_batch_set_config(dvg_ctx *ctx, dvg_batch *batch, const dvg_batch_config *cfg);
+/* Measures candidate launch shapes on this batch (reps timed steps each,
+ * coordinate search) and keeps the fastest; best may be NULL. */
+int32_t dvg_batch_autotune(dvg_ctx *ctx, dvg_batch *batch, uint32_t flags, int reps, dvg_batch_config *best);
+
 /* Version string of the library build. */
 const char *dvg_version(void);
 
