@@ -39,6 +39,8 @@ WORKLOADS = {
     "c3": ("config3", "1024x1024 prefiltered (analytic coverage), 2000 stroked open cubic paths + 500 filled "
                       "polygons, 1 spp"),
     "c5": ("config5", "2048x2048, 10000 closed cubic paths, 2x2 jittered spp, box r=0.5, non-prefiltered"),
+    "c4": ("config4", "64 scenes of uneven size (16-4096 mixed shapes log-uniform, 128^2-1024^2), 2x2 spp, "
+                      "batched fwd + L2 loss + bwd in one persistent launch per phase, HIP graph, autotuned shapes"),
 }
 FP32_PEAK_TFLOPS = 157.3
 
@@ -75,6 +77,8 @@ def main():
     r = Renderer(0)
     r.set_stream(stream.cuda_stream)
 
+    if args.workload == "c4":
+        return bench_batch(args, r, stream, dev)
     fn_name, workload_desc = WORKLOADS[args.workload]
     headline = args.workload == "headline"
     sc, opts = getattr(scenes, fn_name)(seed=0)
@@ -197,6 +201,70 @@ def main():
         "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None,
         "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_batch(args, r, stream, dev):
+    """Config 4: the batch stepped together against a plain loop that steps
+    the same scenes one at a time (one set of launches per scene)."""
+    import torch
+    from pydvg import scenes
+    from pydvg.batch import Batch
+    from pydvg.renderer import DeviceScene, PreparedScene
+
+    items = scenes.config4(seed=0)
+    n_samples = sum(sc.width * sc.height * o.samples_x * o.samples_y for sc, o in items)
+    log(f"bench: c4 {len(items)} scenes, {n_samples} samples per step")
+    targets, singles = [], []
+    for sc, opts in items:
+        tds = DeviceScene(r, PreparedScene(scenes.jittered(sc, 2.0, seed=1), opts))
+        tgt = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device=dev)
+        tds.render(tgt)
+        targets.append(tgt)
+        ds = DeviceScene(r, PreparedScene(sc, opts))
+        bufs, sizes = ds.grad_buffers(torch)
+        singles.append((ds, tgt, torch.empty_like(tgt), torch.empty_like(tgt), bufs, sizes))
+    torch.cuda.synchronize()
+
+    def plain_step():
+        for ds, tgt, img, dr, bufs, sizes in singles:
+            ds.render(img)
+            torch.sub(img, tgt, out=dr)
+            dr.mul_(2.0)
+            ds.render_backward(dr, bufs, sizes)
+
+    def timed(fn, k):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    plain_s = timed(plain_step, args.steps)
+    log(f"bench: plain loop {plain_s * 1e3:.3f} ms/step")
+    batch = Batch(r, items, targets)
+    t_tune = time.perf_counter()
+    cfg = batch.autotune(reps=1)
+    tune_s = time.perf_counter() - t_tune
+    log(f"bench: autotuned {cfg} in {tune_s:.2f} s")
+    batch.set_graph(False)
+    eager_s = timed(batch.step, args.steps)
+    batch.set_graph(True)
+    graph_s = timed(batch.step, args.steps)
+    log(f"bench: batched {eager_s * 1e3:.3f} ms/step eager, {graph_s * 1e3:.3f} ms/step graph")
+    out = {
+        "metric": "fwd+bwd pixel-samples/s (c4, batched)", "value": n_samples / graph_s, "unit": "pixel-samples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": graph_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": WORKLOADS["c4"][1], "global_batch": len(items), "samples_per_step": n_samples,
+                   "parallelism": "single-gpu"},
+        "plain_loop_ms_per_step": plain_s * 1e3, "batched_eager_ms_per_step": eager_s * 1e3,
+        "speedup_vs_plain_loop": plain_s / graph_s, "autotuned": cfg, "autotune_seconds": tune_s,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 

@@ -359,6 +359,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.filter_type = prm->filter_type;
     D.sblock_log2 = 2;  // 4x4 pixel sample blocks
     D.bnd_log2 = 2;     // 4x4 pixel boundary buckets
+    D.unit_samples = (uint32_t)dvgk::TILE * dvgk::TILE * D.spp;  // whole tiles
     D.filter_radius = prm->filter_radius;
     D.ri = (int)std::fmax(std::ceil(prm->filter_radius), 0.0f);
     for (int c = 0; c < 4; ++c) D.bg[c] = prm->background[c];

@@ -29,8 +29,11 @@ struct dvg_batch {
     std::vector<WorkItem> items[dvgk::NUM_PHASES];
     WorkItem *dev_items = nullptr;
     size_t item_off[dvgk::NUM_PHASES] = {};
+    size_t items_cap = 0;
+    std::vector<std::vector<uint32_t>> tile_off;  // tile list offsets per scene
+    std::vector<uint32_t> nbmax;                   // bucket capacity per scene
     int n_cu = 256;
-    dvg_batch_config cfg = {2, 2, 4, 0.f};
+    dvg_batch_config cfg = {2, 2, 4, 8, 0.f};
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
     uint32_t exec_flags = 0;
@@ -45,16 +48,100 @@ void drop_graph(dvg_batch *b) {
     b->exec_flags = 0;
 }
 
-// Pushes the current launch shapes into every scene and uploads the array.
+uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Work items of every phase for the current launch shapes.  Tile units are
+// ordered heaviest first over the whole batch (cost: group-list length times
+// samples in the unit), so the longest units start first.
+int build_items(dvg_ctx *ctx, dvg_batch *b) {
+    for (auto &v : b->items) v.clear();
+    struct Costed {
+        uint64_t cost;
+        WorkItem w;
+    };
+    std::vector<Costed> fwd_units, bwd_units, pre_units;
+    for (uint32_t i = 0; i < b->n; ++i) {
+        const BatchScene &B = b->bs[i];
+        const dvgk::DevScene &S = B.S;
+        size_t npix = (size_t)S.W * S.H, ns = npix * S.spp;
+        if (npix == 0 || S.num_groups == 0) continue;
+        for (int k = 0; k < dvgk::BATCH_ZERO_ARRAYS; ++k) {
+            uint64_t len = k < 15 ? B.zn[k] : (uint64_t)b->nbmax[i] + (k == 15 ? 1 : 0);
+            if (!B.zp[k] || len == 0) continue;
+            for (uint32_t c = 0; c < cdiv(len, dvgk::ZERO_CHUNK); ++c)
+                b->items[dvgk::PH_ZERO].push_back({i, ((uint32_t)k << 20) | c});
+        }
+        uint32_t nblk16 = cdiv(S.W, 16) * cdiv(S.H, 16);
+        for (uint32_t u = 0; u < nblk16; ++u) {
+            b->items[dvgk::PH_WEIGHTS_FWD].push_back({i, u});
+            b->items[dvgk::PH_WEIGHTS_BWD].push_back({i, u});
+        }
+        for (uint32_t u = 0; u < cdiv(npix, dvgk::BLOCK); ++u) b->items[dvgk::PH_RESOLVE].push_back({i, u});
+        uint32_t nt = S.tiles_x * S.tiles_y, upt = dvgk::units_per_tile(S);
+        uint32_t tile_samples = (uint32_t)dvgk::TILE * dvgk::TILE * S.spp;
+        for (uint32_t t = 0; t < nt; ++t) {
+            uint64_t len = b->tile_off[i][t + 1] - b->tile_off[i][t] + 1;
+            for (uint32_t k = 0; k < upt; ++k) {
+                uint32_t ns_unit = std::min(S.unit_samples, tile_samples - k * S.unit_samples);
+                Costed c = {len * ns_unit, {i, t * upt + k}};
+                fwd_units.push_back(c);
+                (S.prefilter ? pre_units : bwd_units).push_back(c);
+            }
+        }
+        if (B.keys) {
+            uint32_t nu = cdiv(ns, dvgk::BLOCK);
+            for (uint32_t u = 0; u < nu; ++u) {
+                b->items[dvgk::PH_BND_KEY].push_back({i, u});
+                b->items[dvgk::PH_BND_SCATTER].push_back({i, u});
+                b->items[dvgk::PH_BND_EVAL].push_back({i, u});
+            }
+            b->items[dvgk::PH_SCAN].push_back({i, 0});
+        }
+    }
+    auto heavy_first = [](std::vector<Costed> &v, std::vector<WorkItem> &outv) {
+        std::stable_sort(v.begin(), v.end(), [](const Costed &a, const Costed &c) { return a.cost > c.cost; });
+        for (const Costed &c : v) outv.push_back(c.w);
+    };
+    heavy_first(fwd_units, b->items[dvgk::PH_FORWARD]);
+    heavy_first(bwd_units, b->items[dvgk::PH_INTERIOR]);
+    heavy_first(pre_units, b->items[dvgk::PH_PREFILTERED]);
+    size_t n_items = 0;
+    for (int ph = 0; ph < dvgk::NUM_PHASES; ++ph) {
+        b->item_off[ph] = n_items;
+        n_items += b->items[ph].size();
+    }
+    if (n_items > b->items_cap) {
+        if (b->dev_items) (void)hipFree(b->dev_items);
+        b->dev_items = nullptr;
+        b->items_cap = 0;
+        HIP_TRY(ctx, hipMalloc(&b->dev_items, std::max<size_t>(n_items, 1) * sizeof(WorkItem)));
+        b->items_cap = n_items;
+    }
+    for (int ph = 0; ph < dvgk::NUM_PHASES; ++ph) {
+        if (b->items[ph].empty()) continue;
+        HIP_TRY(ctx, hipMemcpy(b->dev_items + b->item_off[ph], b->items[ph].data(),
+                               b->items[ph].size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+    return DVG_OK;
+}
+
+// Pushes the current launch shapes into every scene, rebuilds the work
+// items and uploads both.
 int upload_config(dvg_ctx *ctx, dvg_batch *b) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // no step may still read the old arrays
     for (uint32_t i = 0; i < b->n; ++i) {
         BatchScene &B = b->bs[i];
         B.S.sblock_log2 = b->cfg.sblock_log2;
         B.S.bnd_log2 = b->cfg.bnd_log2;
+        uint32_t tile_samples = (uint32_t)dvgk::TILE * dvgk::TILE * B.S.spp;
+        uint64_t us = (uint64_t)dvgk::BLOCK << b->cfg.unit_log2;
+        B.S.unit_samples = (uint32_t)std::min<uint64_t>(us, tile_samples);
         B.nb = dvgk::boundary_buckets(B.S);
         B.zn[15] = B.counts ? B.nb + 1 : 0;
         B.zn[16] = B.cursor ? B.nb : 0;
     }
+    int st = build_items(ctx, b);
+    if (st) return st;
     HIP_TRY(ctx, hipMemcpyAsync(b->dev_bs, b->bs.data(), b->bs.size() * sizeof(BatchScene), hipMemcpyHostToDevice,
                                 ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -89,8 +176,6 @@ void launch_step(dvg_ctx *ctx, dvg_batch *b, uint32_t flags) {
         run(dvgk::PH_BND_EVAL);
     }
 }
-
-uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
 
@@ -202,64 +287,8 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
     if (e != hipSuccess) return fail(hip_err(ctx, e, "hipMalloc(batch scenes)"));
     e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(hip_err(ctx, e, "batch setup"));
-    // work items per phase
-    struct Costed {
-        uint64_t cost;
-        WorkItem w;
-    };
-    std::vector<Costed> fwd_tiles, bwd_tiles, pre_tiles;
-    for (uint32_t i = 0; i < n; ++i) {
-        const dvgk::DevScene &S = b->scenes[i]->S;
-        const BatchScene &B = b->bs[i];
-        size_t npix = (size_t)S.W * S.H, ns = npix * S.spp;
-        if (npix == 0 || S.num_groups == 0) continue;
-        for (int k = 0; k < dvgk::BATCH_ZERO_ARRAYS; ++k) {
-            uint64_t len = k < 15 ? B.zn[k] : (uint64_t)lay[i].nbmax + (k == 15 ? 1 : 0);
-            if (!B.zp[k] || len == 0) continue;
-            for (uint32_t c = 0; c < cdiv(len, dvgk::ZERO_CHUNK); ++c) b->items[dvgk::PH_ZERO].push_back({i, ((uint32_t)k << 20) | c});
-        }
-        uint32_t nblk16 = cdiv(S.W, 16) * cdiv(S.H, 16);
-        for (uint32_t u = 0; u < nblk16; ++u) {
-            b->items[dvgk::PH_WEIGHTS_FWD].push_back({i, u});
-            b->items[dvgk::PH_WEIGHTS_BWD].push_back({i, u});
-        }
-        for (uint32_t u = 0; u < cdiv(npix, dvgk::BLOCK); ++u) b->items[dvgk::PH_RESOLVE].push_back({i, u});
-        uint32_t nt = S.tiles_x * S.tiles_y;
-        for (uint32_t t = 0; t < nt; ++t) {
-            uint64_t cost = (uint64_t)(tile_off[i][t + 1] - tile_off[i][t] + 1) * S.spp;
-            fwd_tiles.push_back({cost, {i, t}});
-            (S.prefilter ? pre_tiles : bwd_tiles).push_back({cost, {i, t}});
-        }
-        if (B.keys) {
-            uint32_t nu = cdiv(ns, dvgk::BLOCK);
-            for (uint32_t u = 0; u < nu; ++u) {
-                b->items[dvgk::PH_BND_KEY].push_back({i, u});
-                b->items[dvgk::PH_BND_SCATTER].push_back({i, u});
-                b->items[dvgk::PH_BND_EVAL].push_back({i, u});
-            }
-            b->items[dvgk::PH_SCAN].push_back({i, 0});
-        }
-    }
-    auto heavy_first = [](std::vector<Costed> &v, std::vector<WorkItem> &outv) {
-        std::stable_sort(v.begin(), v.end(), [](const Costed &a, const Costed &c) { return a.cost > c.cost; });
-        for (const Costed &c : v) outv.push_back(c.w);
-    };
-    heavy_first(fwd_tiles, b->items[dvgk::PH_FORWARD]);
-    heavy_first(bwd_tiles, b->items[dvgk::PH_INTERIOR]);
-    heavy_first(pre_tiles, b->items[dvgk::PH_PREFILTERED]);
-    size_t n_items = 0;
-    for (int ph = 0; ph < dvgk::NUM_PHASES; ++ph) {
-        b->item_off[ph] = n_items;
-        n_items += b->items[ph].size();
-    }
-    e = hipMalloc(&b->dev_items, std::max<size_t>(n_items, 1) * sizeof(WorkItem));
-    if (e != hipSuccess) return fail(hip_err(ctx, e, "hipMalloc(batch items)"));
-    for (int ph = 0; ph < dvgk::NUM_PHASES; ++ph) {
-        if (b->items[ph].empty()) continue;
-        e = hipMemcpy(b->dev_items + b->item_off[ph], b->items[ph].data(), b->items[ph].size() * sizeof(WorkItem),
-                      hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail(hip_err(ctx, e, "batch items upload"));
-    }
+    b->tile_off = std::move(tile_off);
+    for (uint32_t i = 0; i < n; ++i) b->nbmax.push_back(lay[i].nbmax);
     int st = upload_config(ctx, b);
     if (st) return fail(st);
     *out = b;
@@ -356,17 +385,19 @@ int32_t dvg_batch_get_config(const dvg_batch *b, dvg_batch_config *cfg) {
 
 int32_t dvg_batch_set_config(dvg_ctx *ctx, dvg_batch *b, const dvg_batch_config *cfg) {
     if (!ctx || !b || !cfg) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
-    if (cfg->sblock_log2 > 4 || cfg->bnd_log2 < 1 || cfg->bnd_log2 > 4 || cfg->wg_per_cu < 1 || cfg->wg_per_cu > 64)
+    if (cfg->sblock_log2 > 4 || cfg->bnd_log2 < 1 || cfg->bnd_log2 > 4 || cfg->wg_per_cu < 1 || cfg->wg_per_cu > 64 ||
+        cfg->unit_log2 > 8)
         return set_err(ctx, DVG_INVALID_SCENE, "launch shape out of range");
     b->cfg.sblock_log2 = cfg->sblock_log2;
     b->cfg.bnd_log2 = cfg->bnd_log2;
     b->cfg.wg_per_cu = cfg->wg_per_cu;
+    b->cfg.unit_log2 = cfg->unit_log2;
     return upload_config(ctx, b);
 }
 
-// Coordinate search over the launch shapes: persistent workgroups per CU,
-// sample-block edge, boundary-bucket edge; each candidate is timed with HIP
-// events over `reps` steps after one warm-up step.
+// Coordinate search over the launch shapes: tile unit size, persistent
+// workgroups per CU, sample-block edge, boundary-bucket edge; each candidate
+// is timed with HIP events over `reps` steps after one warm-up step.
 int32_t dvg_batch_autotune(dvg_ctx *ctx, dvg_batch *b, uint32_t flags, int reps, dvg_batch_config *best) {
     if (!ctx || !b) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
     reps = std::max(reps, 1);
@@ -390,16 +421,19 @@ int32_t dvg_batch_autotune(dvg_ctx *ctx, dvg_batch *b, uint32_t flags, int reps,
     dvg_batch_config cur = b->cfg;
     float cur_ms = 0.f;
     int st = measure(cur, cur_ms);
-    const uint32_t wg[] = {2, 4, 8, 16}, sb[] = {1, 2, 3}, bb[] = {1, 2, 3};
-    for (int knob = 0; knob < 3 && st == DVG_OK; ++knob) {
-        const uint32_t *vals = knob == 0 ? wg : (knob == 1 ? sb : bb);
-        int nv = knob == 0 ? 4 : 3;
+    const uint32_t ul[] = {0, 1, 2, 8}, wg[] = {2, 4, 8, 16}, sb[] = {1, 2, 3}, bb[] = {1, 2, 3};
+    for (int knob = 0; knob < 4 && st == DVG_OK; ++knob) {
+        const uint32_t *vals = knob == 0 ? ul : (knob == 1 ? wg : (knob == 2 ? sb : bb));
+        int nv = knob < 2 ? 4 : 3;
         for (int k = 0; k < nv && st == DVG_OK; ++k) {
             dvg_batch_config c = cur;
-            if (knob == 0) c.wg_per_cu = vals[k];
-            else if (knob == 1) c.sblock_log2 = vals[k];
+            if (knob == 0) c.unit_log2 = vals[k];
+            else if (knob == 1) c.wg_per_cu = vals[k];
+            else if (knob == 2) c.sblock_log2 = vals[k];
             else c.bnd_log2 = vals[k];
-            if (c.wg_per_cu == cur.wg_per_cu && c.sblock_log2 == cur.sblock_log2 && c.bnd_log2 == cur.bnd_log2) continue;
+            if (c.wg_per_cu == cur.wg_per_cu && c.sblock_log2 == cur.sblock_log2 && c.bnd_log2 == cur.bnd_log2 &&
+                c.unit_log2 == cur.unit_log2)
+                continue;
             float ms = 0.f;
             st = measure(c, ms);
             if (st == DVG_OK && ms < cur_ms) {

@@ -19,6 +19,9 @@ struct DevScene {
     // square blocks of 1 << sblock_log2 pixels; boundary samples are bucketed
     // by blocks of 1 << bnd_log2 pixels
     uint32_t sblock_log2, bnd_log2;
+    // tile work is split into units of unit_samples consecutive samples of a
+    // tile (a multiple of the workgroup size; >= TILE*TILE*spp = whole tiles)
+    uint32_t unit_samples;
     float filter_radius;
     int ri;
     float bg[4];            // straight RGBA
@@ -73,7 +76,11 @@ void launch_backward_prefiltered(const DevScene &S, const float *weight, const f
 // Boundary samples are sorted by the pixel block (1 << bnd_log2 pixels a
 // side) of their pixel, blocks numbered tile-major, so a wave shares one tile
 // group list and a compact footprint.
-inline uint32_t boundary_buckets(const DevScene &S) {
+__host__ __device__ inline uint32_t units_per_tile(const DevScene &S) {
+    uint32_t total = TILE * TILE * S.spp;
+    return (total + S.unit_samples - 1) / S.unit_samples;
+}
+__host__ __device__ inline uint32_t boundary_buckets(const DevScene &S) {
     uint32_t per_row = (uint32_t)TILE >> S.bnd_log2;
     return S.tiles_x * S.tiles_y * per_row * per_row;
 }

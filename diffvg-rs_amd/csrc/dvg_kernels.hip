@@ -505,13 +505,15 @@ __global__ void k_weights(DevScene S, float *__restrict__ weight) {
 constexpr int SPLAT_HALO = 4;
 constexpr int SPLAT_EDGE = TILE + 2 * SPLAT_HALO;
 
-DVG_DEV void forward_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restrict__ weight,
                           float *__restrict__ accum) {
     __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
+    uint32_t upt = units_per_tile(S);
+    uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
-    uint32_t total = TILE * TILE * S.spp;
+    uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
     int ri = S.ri;
     bool lds = ri <= SPLAT_HALO;
     int edge = TILE + 2 * ri;
@@ -521,7 +523,7 @@ DVG_DEV void forward_unit(const DevScene &S, uint32_t tile, const float *__restr
             for (int c = 0; c < 4; ++c) lacc[c][i] = 0.f;
         __syncthreads();
     }
-    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+    for (uint32_t s = s0 + threadIdx.x; s < total; s += BLOCK) {
         uint32_t x, y, sx, sy;
         tile_sample(S, tile, s, x, y, sx, sy);
         if (x >= S.W || y >= S.H) continue;
@@ -1006,7 +1008,7 @@ DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, 
     return count;
 }
 
-DVG_DEV void interior_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__restrict__ weight,
                            const float *__restrict__ d_render, const DevGrads &G) {
     __shared__ float s_stage[STAGE_FLOATS];
     __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
@@ -1015,15 +1017,17 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t tile, const float *__rest
     // per lane: fragment bit set (lists <= MASK_GROUPS) or fragment window
     __shared__ uint32_t s_ids[FRAG_WINDOW * BLOCK];
     uint32_t *ids = s_ids + threadIdx.x;
+    uint32_t upt = units_per_tile(S);
+    uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
     bool use_mask = n <= MASK_GROUPS;
     stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
-    uint32_t total = TILE * TILE * S.spp;
+    uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
     float Wf = (float)S.W, Hf = (float)S.H;
-    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+    for (uint32_t s = s0 + threadIdx.x; s < total; s += BLOCK) {
         uint32_t x, y, sx, sy;
         tile_sample(S, tile, s, x, y, sx, sy);
         if (x >= S.W || y >= S.H) continue;
@@ -1796,7 +1800,7 @@ DVG_DEV void d_distance_query(const DevScene &S, const DevGrads &G, uint32_t g, 
     dty -= dpty;
 }
 
-DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t tile, const float *__restrict__ weight,
+DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t unit, const float *__restrict__ weight,
                               const float *__restrict__ d_render, const DevGrads &G) {
     __shared__ float s_stage[STAGE_FLOATS];
     __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
@@ -1806,14 +1810,16 @@ DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t tile, const float *__r
     __shared__ float s_cov[PF_WINDOW * BLOCK];
     uint32_t *ids = s_ids + threadIdx.x;
     float *cov = s_cov + threadIdx.x;
+    uint32_t upt = units_per_tile(S);
+    uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
     stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
-    uint32_t total = TILE * TILE * S.spp;
+    uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
     float Wf = (float)S.W, Hf = (float)S.H;
-    for (uint32_t s = threadIdx.x; s < total; s += BLOCK) {
+    for (uint32_t s = s0 + threadIdx.x; s < total; s += BLOCK) {
         uint32_t x, y, sx, sy;
         tile_sample(S, tile, s, x, y, sx, sy);
         if (x >= S.W || y >= S.H) continue;
@@ -2670,20 +2676,21 @@ void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stre
     else hipLaunchKernelGGL(k_weights<false>, dim3(nb), dim3(256), 0, stream, S, weight);
 }
 void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream) {
-    hipLaunchKernelGGL(k_forward, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight, accum);
+    hipLaunchKernelGGL(k_forward, dim3(S.tiles_x * S.tiles_y * units_per_tile(S)), dim3(BLOCK), 0, stream, S, weight,
+                       accum);
 }
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream) {
     hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)S.W * S.H, 256)), dim3(256), 0, stream, S, weight, accum, out);
 }
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                               hipStream_t stream) {
-    hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
-                       d_render, G);
+    hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y * units_per_tile(S)), dim3(BLOCK), 0, stream,
+                       S, weight, d_render, G);
 }
 void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                                  hipStream_t stream) {
-    hipLaunchKernelGGL(k_backward_prefiltered, dim3(S.tiles_x * S.tiles_y), dim3(BLOCK), 0, stream, S, weight,
-                       d_render, G);
+    hipLaunchKernelGGL(k_backward_prefiltered, dim3(S.tiles_x * S.tiles_y * units_per_tile(S)), dim3(BLOCK), 0,
+                       stream, S, weight, d_render, G);
 }
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;

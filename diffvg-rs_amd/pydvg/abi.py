@@ -118,7 +118,8 @@ BATCH_FORWARD, BATCH_LOSS, BATCH_BACKWARD = 1, 2, 4
 
 
 class BatchConfigC(C.Structure):
-    _fields_ = [("sblock_log2", C.c_uint32), ("bnd_log2", C.c_uint32), ("wg_per_cu", C.c_uint32), ("ms", C.c_float)]
+    _fields_ = [("sblock_log2", C.c_uint32), ("bnd_log2", C.c_uint32), ("wg_per_cu", C.c_uint32),
+                ("unit_log2", C.c_uint32), ("ms", C.c_float)]
 
 
 # Every symbol include/dvg.h declares, with its ctypes signature.

@@ -16,6 +16,11 @@ FORWARD, LOSS, BACKWARD = abi.BATCH_FORWARD, abi.BATCH_LOSS, abi.BATCH_BACKWARD
 STEP = FORWARD | LOSS | BACKWARD
 
 
+def _cfg(c):
+    return {"sblock_log2": c.sblock_log2, "bnd_log2": c.bnd_log2, "wg_per_cu": c.wg_per_cu,
+            "unit_log2": c.unit_log2, "ms": c.ms}
+
+
 class Batch:
     """`items` is a list of (Scene, RenderOptions); `targets` (optional) a list
     of torch CUDA float32 tensors of W*H*4 target pixels, kept alive here."""
@@ -50,17 +55,17 @@ class Batch:
     def config(self):
         c = abi.BatchConfigC()
         self.r.lib.dvg_batch_get_config(self.handle, C.byref(c))
-        return {"sblock_log2": c.sblock_log2, "bnd_log2": c.bnd_log2, "wg_per_cu": c.wg_per_cu, "ms": c.ms}
+        return _cfg(c)
 
-    def set_config(self, sblock_log2=2, bnd_log2=2, wg_per_cu=4):
-        c = abi.BatchConfigC(sblock_log2, bnd_log2, wg_per_cu, 0.0)
+    def set_config(self, sblock_log2=2, bnd_log2=2, wg_per_cu=4, unit_log2=8):
+        c = abi.BatchConfigC(sblock_log2, bnd_log2, wg_per_cu, unit_log2, 0.0)
         _check(self.r.lib, self.r.ctx, self.r.lib.dvg_batch_set_config(self.r.ctx, self.handle, C.byref(c)))
 
     def autotune(self, flags=STEP, reps=2):
         c = abi.BatchConfigC()
         st = self.r.lib.dvg_batch_autotune(self.r.ctx, self.handle, flags, reps, C.byref(c))
         _check(self.r.lib, self.r.ctx, st)
-        return {"sblock_log2": c.sblock_log2, "bnd_log2": c.bnd_log2, "wg_per_cu": c.wg_per_cu, "ms": c.ms}
+        return _cfg(c)
 
     def read(self, i):
         """Scene i's image (numpy W*H*4) and gradients (FlatGrads), synchronously."""

@@ -418,12 +418,15 @@ typedef struct dvg_batch dvg_batch;
 
 /* Launch shapes of a batch (all scenes): sample blocks of 1 << sblock_log2
  * pixels a side inside a tile (0..4), boundary buckets of 1 << bnd_log2
- * pixels (1..4), persistent workgroups per compute unit (1..64); ms is the
- * step time the autotuner measured for them (0 when not measured). */
+ * pixels (1..4), persistent workgroups per compute unit (1..64), tile work
+ * split into units of 256 << unit_log2 samples (0..8; a unit never spans two
+ * tiles); ms is the step time the autotuner measured for them (0 when not
+ * measured). */
 typedef struct dvg_batch_config {
     uint32_t sblock_log2;
     uint32_t bnd_log2;
     uint32_t wg_per_cu;
+    uint32_t unit_log2;
     float ms;
 } dvg_batch_config;
 

@@ -76,7 +76,7 @@ def _check(batch, refs, label):
 def test_batch_step_matches_single_scene_path(renderer, setup):
     batch, refs = setup
     batch.set_graph(False)
-    batch.set_config(2, 2, 4)
+    batch.set_config(2, 2, 4, 8)
     print("case batch eager", flush=True)
     batch.step()
     _check(batch, refs, "eager")
@@ -85,7 +85,7 @@ def test_batch_step_matches_single_scene_path(renderer, setup):
 def test_batch_graph_replays_match(renderer, setup):
     import torch
     batch, refs = setup
-    batch.set_config(2, 2, 4)
+    batch.set_config(2, 2, 4, 0)
     batch.set_graph(True)
     print("case batch graph", flush=True)
     for rep in range(3):
@@ -107,7 +107,7 @@ def test_batch_autotuned_shapes_match(renderer, setup):
     batch.step()
     _check(batch, refs, "autotuned")
     # every candidate shape gives the same results
-    for sb, bb in ((1, 1), (3, 3)):
-        batch.set_config(sb, bb, 2)
+    for sb, bb, ul in ((1, 1, 0), (3, 3, 1), (2, 2, 8)):
+        batch.set_config(sb, bb, 2, ul)
         batch.step()
-        _check(batch, refs, f"shape {sb}/{bb}")
+        _check(batch, refs, f"shape {sb}/{bb}/{ul}")
