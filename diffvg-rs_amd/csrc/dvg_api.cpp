@@ -359,7 +359,9 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.filter_type = prm->filter_type;
     D.sblock_log2 = 2;  // 4x4 pixel sample blocks
     D.bnd_log2 = 2;     // 4x4 pixel boundary buckets
-    D.unit_samples = (uint32_t)dvgk::TILE * dvgk::TILE * D.spp;  // whole tiles
+    // tile work in units of one sample per lane: many more workgroups than
+    // compute units even for small canvases, and no tile on the critical path
+    D.unit_samples = std::min((uint32_t)dvgk::BLOCK, (uint32_t)dvgk::TILE * dvgk::TILE * D.spp);
     D.filter_radius = prm->filter_radius;
     D.ri = (int)std::fmax(std::ceil(prm->filter_radius), 0.0f);
     for (int c = 0; c < 4; ++c) D.bg[c] = prm->background[c];
@@ -500,6 +502,17 @@ static void grads_to_dev(const dvg_grads *g, dvgk::DevGrads &G) {
     G.d_background = g->d_background;
     G.d_background_image = g->d_background_image;
     G.d_translation = g->d_translation;
+}
+
+int32_t dvg_device_scene_set_shapes(dvg_device_scene *ds, uint32_t sblock_log2, uint32_t bnd_log2,
+                                    uint32_t unit_log2) {
+    if (!ds || sblock_log2 > 4 || bnd_log2 < 1 || bnd_log2 > 4 || unit_log2 > 8) return DVG_INVALID_SCENE;
+    dvgk::DevScene &S = ds->S;
+    S.sblock_log2 = sblock_log2;
+    S.bnd_log2 = bnd_log2;
+    uint64_t us = (uint64_t)dvgk::BLOCK << unit_log2, tile = (uint64_t)dvgk::TILE * dvgk::TILE * S.spp;
+    S.unit_samples = (uint32_t)std::min(us, tile);
+    return DVG_OK;
 }
 
 int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const float *d_render,

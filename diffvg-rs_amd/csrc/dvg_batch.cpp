@@ -33,7 +33,7 @@ struct dvg_batch {
     std::vector<std::vector<uint32_t>> tile_off;  // tile list offsets per scene
     std::vector<uint32_t> nbmax;                   // bucket capacity per scene
     int n_cu = 256;
-    dvg_batch_config cfg = {2, 2, 4, 8, 0.f};
+    dvg_batch_config cfg = {2, 2, 4, 0, 0.f};
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
     uint32_t exec_flags = 0;

@@ -137,6 +137,11 @@ class DeviceScene:
             self.r.lib.dvg_device_scene_destroy(h)
             self.handle = None
 
+    def set_shapes(self, sblock_log2=2, bnd_log2=2, unit_log2=0):
+        """Launch shapes (see dvg_device_scene_set_shapes)."""
+        st = self.r.lib.dvg_device_scene_set_shapes(self.handle, sblock_log2, bnd_log2, unit_log2)
+        _check(self.r.lib, self.r.ctx, st)
+
     def render(self, out) -> None:
         st = self.r.lib.dvg_device_render(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))
         _check(self.r.lib, self.r.ctx, st)
