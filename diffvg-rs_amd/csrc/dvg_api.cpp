@@ -214,6 +214,69 @@ static std::vector<float> group_cpu_bounds(const dvg_scene_pack *pk, const dvg_b
     return out;
 }
 
+// Plan of device-side geometry updates: for every packed curve the global
+// path-point indices of its 4 control points (the walk of path.rs:100-173),
+// and for every prepared instance the padding of its bounds -- the packer's
+// stroke padding with the radius taken as max(|stroke width|, max |thickness|)
+// (an upper bound of the flattened radius, so culling stays conservative)
+// plus the prefiltering band.
+static bool make_update_plan(const dvg_scene_pack *pk, const dvg_backward_pack *bp, const dvg_render_params *prm,
+                             std::vector<uint32_t> &curve_pt, std::vector<float> &inst_pad) {
+    uint32_t SP = pk->num_prepared_shapes, G = pk->num_groups;
+    if (!bp->prepared_to_scene || !bp->shape_transform) return false;
+    size_t ncurves = pk->curve_data_len / 13;
+    curve_pt.assign(ncurves * 4, 0u);
+    inst_pad.assign(std::max(SP, 1u), 0.f);
+    for (uint32_t g = 0; g < G; ++g) {
+        const float *gd = pk->group_data + 16 * (size_t)g;
+        uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+        float aa = prm->use_prefiltering ? pk->group_inv_scale[g] : 0.f;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint32_t prep = (uint32_t)pk->group_shapes[off + i];
+            const float *sd = pk->shape_data + 15 * (size_t)prep;
+            uint32_t si = bp->prepared_to_scene[prep];
+            const float *T = bp->shape_transform + 6 * (size_t)si;
+            float r = std::fabs(sd[3]);
+            if ((uint32_t)sd[0] == DVG_SHAPE_PATH) {
+                const float *th = bp->path_thickness + bp->shape_path_thickness_offsets[si];
+                for (uint32_t k = 0; k < bp->shape_path_thickness_counts[si]; ++k) r = std::fmax(r, std::fabs(th[k]));
+            }
+            float scale = std::fmax(std::fmax(std::sqrt(T[0] * T[0] + T[3] * T[3]), std::sqrt(T[1] * T[1] + T[4] * T[4])), 0.f);
+            float sp = r * scale;
+            if ((uint32_t)sd[9] == DVG_JOIN_MITER) sp *= std::fmax(sd[11], 1.0f);
+            inst_pad[prep] = std::fmax(aa + sp, 0.f);
+            if ((uint32_t)sd[0] != DVG_SHAPE_PATH) continue;
+            uint32_t coff = (uint32_t)sd[12], ccnt = (uint32_t)sd[13];
+            uint32_t np = bp->shape_path_point_counts[si], nc = bp->shape_path_ctrl_counts[si];
+            bool closed = bp->shape_path_is_closed[si] != 0;
+            const uint32_t *ctrl = bp->path_num_control_points + bp->shape_path_ctrl_offsets[si];
+            uint32_t base = bp->shape_path_offsets[si];
+            uint32_t pid = 0, made = 0;
+            for (uint32_t k = 0; k < nc && made < ccnt && np > 0; ++k) {
+                uint32_t c = ctrl[k];
+                if (c > 2) break;
+                uint32_t idx[4];
+                bool ok = true;
+                for (uint32_t j = 0; j <= c + 1; ++j) {
+                    uint32_t q = pid + j;
+                    if (closed) q %= np;
+                    else if (q >= np) { ok = false; break; }
+                    idx[j] = q;
+                }
+                if (!ok) break;
+                for (uint32_t j = c + 2; j < 4; ++j) idx[j] = idx[c + 1];
+                size_t cv = (size_t)coff + made;
+                if (cv >= ncurves) return false;
+                for (int j = 0; j < 4; ++j) curve_pt[cv * 4 + j] = base + idx[j];
+                ++made;
+                pid += c + 1;
+            }
+            if (made != ccnt) return false;
+        }
+    }
+    return true;
+}
+
 int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dvg_backward_pack *bp,
                                 const dvg_render_params *prm, dvg_device_scene **out) {
     if (!ctx || !pk || !prm || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
@@ -281,7 +344,9 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
            i_toff = addv(nullptr, 0), i_tent = addv(nullptr, 0), i_tcnt = addv(nullptr, 0);
     // tile work buffers (no host source)
     A.srcs[i_toff].second = ((size_t)ntiles + 1) * 4;
-    A.srcs[i_tent].second = std::max<uint64_t>(n_entries, 1) * 4;
+    // room for the lists to grow when a device-side update moves shapes
+    uint64_t tile_cap = std::max<uint64_t>(2 * n_entries + 4ull * ntiles, 1);
+    A.srcs[i_tent].second = tile_cap * 4;
     A.srcs[i_tcnt].second = (size_t)std::max(ntiles, 1u) * 4;
     // recompute offsets after resizing the sourceless entries
     {
@@ -329,6 +394,25 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         i_s2p = addv(s2p.data(), (size_t)std::max(S, 1u) * 4);
     }
     size_t i_gcb = addv(cpu_bounds.data(), cpu_bounds.size() * 4);
+    // plan of device-side updates: control-point indices of every curve and
+    // the padding of every instance's bounds
+    std::vector<uint32_t> curve_pt;
+    std::vector<float> inst_pad;
+    bool plan_ok = bp && make_update_plan(pk, bp, prm, curve_pt, inst_pad);
+    size_t i_cpt = addv(plan_ok ? curve_pt.data() : nullptr, plan_ok ? curve_pt.size() * 4 : 0);
+    size_t i_pad = addv(plan_ok ? inst_pad.data() : nullptr, plan_ok ? inst_pad.size() * 4 : 0);
+    size_t i_cib = addv(nullptr, 0), i_flag = addv(nullptr, 0);
+    A.srcs[i_cib].second = (size_t)std::max(SP, 1u) * 16;
+    A.srcs[i_flag].second = 4;
+    {  // offsets again: sizes of sourceless entries changed
+        size_t t = 0;
+        for (size_t k = 0; k < A.offs.size(); ++k) {
+            size_t o = (t + 255) & ~(size_t)255;
+            A.offs[k] = o;
+            t = o + std::max(A.srcs[k].second, (size_t)4);
+        }
+        A.total = t;
+    }
 
     dvg_device_scene *ds = new dvg_device_scene();
     hipError_t e = hipMalloc(&ds->arena, A.total);
@@ -413,6 +497,18 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         D.scene_to_prepared = (const uint32_t *)at(i_s2p);
     }
     D.group_cpu_bounds = (const float *)at(i_gcb);
+    if (plan_ok) {
+        dvgk::UpdatePlan &P = ds->plan;
+        P.ncurves = (uint32_t)(curve_pt.size() / 4);
+        P.n_points = (uint32_t)(bp->path_points_len / 2);
+        P.tile_cap = (uint32_t)tile_cap;
+        P.curve_pt = (const uint32_t *)at(i_cpt);
+        P.inst_pad = (const float *)at(i_pad);
+        P.cpu_inst_bounds = (float *)at(i_cib);
+        P.ranges = (int32_t *)at(i_rng);
+        P.tile_counts = (uint32_t *)at(i_tcnt);
+        P.overflow = (uint32_t *)at(i_flag);
+    }
     ds->num_groups = G;
     ds->num_shapes = S;
     {
@@ -502,6 +598,27 @@ static void grads_to_dev(const dvg_grads *g, dvgk::DevGrads &G) {
     G.d_background = g->d_background;
     G.d_background_image = g->d_background_image;
     G.d_translation = g->d_translation;
+}
+
+int32_t dvg_device_scene_update(dvg_ctx *ctx, dvg_device_scene *ds, const float *path_points,
+                                const float *group_colors) {
+    if (!ctx || !ds) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (!ds->plan.curve_pt && !ds->plan.ranges)
+        return set_err(ctx, DVG_INVALID_SCENE, "scene was created without the backward tables an update needs");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvgk::launch_scene_update(ds->S, ds->plan, path_points, group_colors, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
+int32_t dvg_device_scene_check(dvg_ctx *ctx, dvg_device_scene *ds) {
+    if (!ctx || !ds) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (!ds->plan.overflow) return DVG_OK;
+    uint32_t flag = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&flag, ds->plan.overflow, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (flag) return set_err(ctx, DVG_DEVICE_ERROR, "tile lists outgrew their buffer after an update; recreate the scene");
+    return DVG_OK;
 }
 
 int32_t dvg_device_scene_set_shapes(dvg_device_scene *ds, uint32_t sblock_log2, uint32_t bnd_log2,

@@ -40,6 +40,7 @@ struct dvg_ctx {
 
 struct dvg_device_scene {
     dvgk::DevScene S;
+    dvgk::UpdatePlan plan = {};  // device-side updates (plan.curve_pt null: unsupported)
     void *arena = nullptr;
     size_t arena_bytes = 0;
     uint32_t num_groups = 0, num_shapes = 0;

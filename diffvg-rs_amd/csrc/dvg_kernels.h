@@ -96,6 +96,24 @@ void launch_boundary_eval(const DevScene &S, const float *weight, const float *d
 // Zero `count` (<= 14) device float arrays of the given lengths.
 void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
 
+// ---- device-side scene update (optimisation loops) --------------------------
+// Buffers of a device scene that a geometry update rewrites, plus the plan
+// computed once at creation.
+struct UpdatePlan {
+    uint32_t ncurves, n_points, tile_cap;
+    const uint32_t *curve_pt;  // 4 per curve: global index of its control points
+    const float *inst_pad;     // per prepared instance: padding of its bounds
+    float *cpu_inst_bounds;    // per prepared instance: 4 floats scratch
+    int32_t *ranges;           // per group: tile range (int4)
+    uint32_t *tile_counts;     // per tile
+    uint32_t *overflow;        // set when the tile lists outgrow tile_cap
+};
+// Rewrites the packed geometry of S from path_points (scene layout; NULL =
+// unchanged) and group colours (8 per group: fill then stroke RGBA; NULL =
+// unchanged), rebuilds tile lists and boundary tables.  Asynchronous.
+void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
+                         hipStream_t stream);
+
 // ---- batches: many scenes per launch ---------------------------------------
 // One scene of a batch: its device scene and its work buffers.
 constexpr int BATCH_ZERO_ARRAYS = 17;  // 14 gradient arrays, accum, counts, cursor

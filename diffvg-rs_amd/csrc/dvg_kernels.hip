@@ -57,8 +57,10 @@ __global__ void k_bin_write(const int4 *__restrict__ rng, uint32_t G, uint32_t t
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= tiles_x * tiles_y) return;
     int tx = (int)(t % tiles_x), ty = (int)(t / tiles_x);
-    uint32_t o = offsets[t];
-    for (uint32_t g = 0; g < G; ++g) {
+    // the tile's slot [offsets[t], offsets[t+1]) bounds the writes (offsets
+    // clamped to a buffer capacity cut lists short instead of overrunning)
+    uint32_t o = offsets[t], e = offsets[t + 1];
+    for (uint32_t g = 0; g < G && o < e; ++g) {
         int4 r = rng[g];
         if (tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w) entries[o++] = g;
     }
@@ -2560,6 +2562,230 @@ __global__ void k_zero(ZeroList z) {
 }
 
 // ---------------------------------------------------------------------------
+// Device-side scene update: the packing steps whose inputs change in an
+// optimisation loop (control points, colours) restated as kernels, so an
+// Adam step never leaves the device.  The arena buffers written here are the
+// scene's own (DevScene holds them as const for the render kernels).
+// prepare_shape / bounds: prepare.rs:459-561, 207-237; binning.rs:10-313;
+// compute_shape_lengths / build_shape_cdf / build_path_cdfs:
+// backward/boundary.rs:200-451.
+
+__global__ void k_copy_f32(float *__restrict__ dst, const float *__restrict__ src, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Control points of every curve from the path points (path.rs:100-173).
+__global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
+                             const float *__restrict__ pts, uint32_t ncurves) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncurves) return;
+    float *cv = curve_data + (size_t)c * 13u;
+    for (int j = 0; j < 4; ++j) {
+        uint32_t p = curve_pt[4u * c + j];
+        cv[1 + 2 * j] = pts[2u * p];
+        cv[2 + 2 * j] = pts[2u * p + 1u];
+    }
+}
+
+// Per prepared instance: path box (shape_data params), padded group-local
+// bounds (the packer's), and the reference CPU shape_bounds box
+// (distance/shape.rs:10-91) into cpu_inst_bounds.
+__global__ void k_upd_instances(DevScene S, float *__restrict__ shape_data, float *__restrict__ shape_bounds,
+                                const float *__restrict__ inst_pad, float *__restrict__ cpu_b) {
+    uint32_t prep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (prep >= S.num_prepared) return;
+    float *sd = shape_data + prep * 15u;
+    uint32_t si = S.prepared_to_scene[prep], kind = (uint32_t)sd[0];
+    const float *T = S.shape_transform + si * 6u;
+    float scale = fmaxf(fmaxf(sqrtf(T[0] * T[0] + T[3] * T[3]), sqrtf(T[1] * T[1] + T[4] * T[4])), 0.f);
+    float r = fabsf(sd[3]) * scale;
+    if (kind == DVG_SHAPE_PATH && S.st_cnt[si] > 0) {
+        const float *th = S.path_thickness + S.st_off[si];
+        float mt = 0.f;
+        for (uint32_t k = 0; k < S.st_cnt[si]; ++k) mt = fmaxf(mt, fabsf(th[k]));
+        r = fmaxf(r, mt * scale);
+    }
+    if (r > 0.f && (uint32_t)sd[9] == DVG_JOIN_MITER) r *= fmaxf(sd[11], 1.0f);
+    float cpad = fmaxf(r, 0.f);
+    float *cb = cpu_b + prep * 4u;
+    if (kind == DVG_SHAPE_PATH) {
+        uint32_t np = S.sp_cnt[si];
+        if (np == 0) {
+            cb[0] = 1.f; cb[1] = 1.f; cb[2] = 0.f; cb[3] = 0.f;
+            return;
+        }
+        const float *pts = S.path_points + (size_t)S.sp_off[si] * 2u;
+        float bx0 = INFINITY, by0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY;
+        float tx0 = INFINITY, ty0 = INFINITY, tx1 = -INFINITY, ty1 = -INFINITY;
+        for (uint32_t k = 0; k < np; ++k) {
+            float x = pts[2u * k], y = pts[2u * k + 1u];
+            bx0 = fminf(bx0, x); by0 = fminf(by0, y); bx1 = fmaxf(bx1, x); by1 = fmaxf(by1, y);
+            float gx = T[0] * x + T[1] * y + T[2], gy = T[3] * x + T[4] * y + T[5];
+            tx0 = fminf(tx0, gx); ty0 = fminf(ty0, gy); tx1 = fmaxf(tx1, gx); ty1 = fmaxf(ty1, gy);
+        }
+        sd[4] = bx0; sd[5] = by0; sd[6] = bx1; sd[7] = by1;
+        float cx[4] = {bx0, bx1, bx1, bx0}, cy[4] = {by0, by0, by1, by1};
+        float mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+        for (int k = 0; k < 4; ++k) {
+            float gx = T[0] * cx[k] + T[1] * cy[k] + T[2], gy = T[3] * cx[k] + T[4] * cy[k] + T[5];
+            mnx = fminf(mnx, gx); mny = fminf(mny, gy); mxx = fmaxf(mxx, gx); mxy = fmaxf(mxy, gy);
+        }
+        float pad = inst_pad[prep];
+        float *b = shape_bounds + prep * 4u;
+        b[0] = mnx - pad; b[1] = mny - pad; b[2] = mxx + pad; b[3] = mxy + pad;
+        cb[0] = tx0 - cpad; cb[1] = ty0 - cpad; cb[2] = tx1 + cpad; cb[3] = ty1 + cpad;
+        return;
+    }
+    float mnx, mny, mxx, mxy;
+    if (kind == DVG_SHAPE_CIRCLE || kind == DVG_SHAPE_ELLIPSE) {
+        float rx = fabsf(sd[6]), ry = kind == DVG_SHAPE_CIRCLE ? fabsf(sd[6]) : fabsf(sd[7]);
+        float cx = T[0] * sd[4] + T[1] * sd[5] + T[2], cy = T[3] * sd[4] + T[4] * sd[5] + T[5];
+        float ex = sqrtf((T[0] * rx) * (T[0] * rx) + (T[1] * ry) * (T[1] * ry));
+        float ey = sqrtf((T[3] * rx) * (T[3] * rx) + (T[4] * ry) * (T[4] * ry));
+        mnx = cx - ex; mny = cy - ey; mxx = cx + ex; mxy = cy + ey;
+    } else {
+        float xs[4] = {sd[4], sd[6], sd[6], sd[4]}, ys[4] = {sd[5], sd[5], sd[7], sd[7]};
+        mnx = mny = INFINITY;
+        mxx = mxy = -INFINITY;
+        for (int k = 0; k < 4; ++k) {
+            float gx = T[0] * xs[k] + T[1] * ys[k] + T[2], gy = T[3] * xs[k] + T[4] * ys[k] + T[5];
+            mnx = fminf(mnx, gx); mny = fminf(mny, gy); mxx = fmaxf(mxx, gx); mxy = fmaxf(mxy, gy);
+        }
+    }
+    cb[0] = mnx - cpad; cb[1] = mny - cpad; cb[2] = mxx + cpad; cb[3] = mxy + cpad;
+}
+
+// Per group: tile range from the union of its bounds, the CPU root box, and
+// the colours.
+__global__ void k_upd_groups(DevScene S, const float *__restrict__ shape_bounds, const float *__restrict__ cpu_b,
+                             float *__restrict__ group_data, const float *__restrict__ colors, int4 *__restrict__ rng,
+                             float *__restrict__ cpu_root) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.num_groups) return;
+    float *gd = group_data + g * 16u;
+    if (colors)
+        for (int k = 0; k < 8; ++k) gd[8 + k] = colors[g * 8u + k];
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    float u[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY}, c[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
+    bool any = false, cany = false;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        uint32_t prep = (uint32_t)S.group_shapes[off + i];
+        const float *b = shape_bounds + prep * 4u;
+        if (b[0] <= b[2] && b[1] <= b[3]) {
+            u[0] = fminf(u[0], b[0]); u[1] = fminf(u[1], b[1]); u[2] = fmaxf(u[2], b[2]); u[3] = fmaxf(u[3], b[3]);
+            any = true;
+        }
+        const float *q = cpu_b + prep * 4u;
+        if (q[0] <= q[2] && q[1] <= q[3]) {
+            c[0] = fminf(c[0], q[0]); c[1] = fminf(c[1], q[1]); c[2] = fmaxf(c[2], q[2]); c[3] = fmaxf(c[3], q[3]);
+            cany = true;
+        }
+    }
+    float *cr = cpu_root + g * 4u;
+    if (cany) { cr[0] = c[0]; cr[1] = c[1]; cr[2] = c[2]; cr[3] = c[3]; }
+    else { cr[0] = 1.f; cr[1] = 1.f; cr[2] = 0.f; cr[3] = 0.f; }
+    int4 r = make_int4(1, 1, 0, 0);
+    if (any && S.tiles_x > 0 && S.tiles_y > 0) {
+        const float *m = S.group_shape_xform + g * 6u;
+        float xs[4] = {m[0] * u[0] + m[1] * u[1] + m[2], m[0] * u[2] + m[1] * u[1] + m[2],
+                       m[0] * u[2] + m[1] * u[3] + m[2], m[0] * u[0] + m[1] * u[3] + m[2]};
+        float ys[4] = {m[3] * u[0] + m[4] * u[1] + m[5], m[3] * u[2] + m[4] * u[1] + m[5],
+                       m[3] * u[2] + m[4] * u[3] + m[5], m[3] * u[0] + m[4] * u[3] + m[5]};
+        float mnx = fminf(fminf(xs[0], xs[1]), fminf(xs[2], xs[3])), mxx = fmaxf(fmaxf(xs[0], xs[1]), fmaxf(xs[2], xs[3]));
+        float mny = fminf(fminf(ys[0], ys[1]), fminf(ys[2], ys[3])), mxy = fmaxf(fmaxf(ys[0], ys[1]), fmaxf(ys[2], ys[3]));
+        if (S.prefilter) { mnx -= 1.0f; mny -= 1.0f; mxx += 1.0f; mxy += 1.0f; }
+        if (!(mxx < 0.f || mxy < 0.f || mnx >= (float)S.W || mny >= (float)S.H)) {
+            int tx0 = (int)floorf(mnx / (float)TILE), tx1 = (int)floorf(mxx / (float)TILE);
+            int ty0 = (int)floorf(mny / (float)TILE), ty1 = (int)floorf(mxy / (float)TILE);
+            int hx = (int)S.tiles_x - 1, hy = (int)S.tiles_y - 1;
+            tx0 = tx0 < 0 ? 0 : (tx0 > hx ? hx : tx0); tx1 = tx1 < 0 ? 0 : (tx1 > hx ? hx : tx1);
+            ty0 = ty0 < 0 ? 0 : (ty0 > hy ? hy : ty0); ty1 = ty1 < 0 ? 0 : (ty1 > hy ? hy : ty1);
+            if (tx0 <= tx1 && ty0 <= ty1) r = make_int4(tx0, ty0, tx1, ty1);
+        }
+    }
+    rng[g] = r;
+}
+
+// Offsets past the tile-list capacity are clamped (every list stays inside
+// the buffer) and reported.
+__global__ void k_clamp_offsets(uint32_t *__restrict__ offsets, uint32_t n, uint32_t cap, uint32_t *flag) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (offsets[i] > cap) {
+        offsets[i] = cap;
+        *flag = 1u;
+    }
+}
+
+// polyline_length (backward/boundary.rs:230-309) of packed curve c.
+DVG_DEV float curve_polyline_length(const float *cv) {
+    uint32_t k = (uint32_t)cv[0];
+    float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+    if (k == 0) return sqrtf((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    if (k == 1) {
+        float t = 0.5f, tt = 1.0f - t;
+        float vx = ((x0 * (tt * tt)) + (x1 * (2.0f * tt * t))) + (x2 * (t * t));
+        float vy = ((y0 * (tt * tt)) + (y1 * (2.0f * tt * t))) + (y2 * (t * t));
+        return sqrtf((vx - x0) * (vx - x0) + (vy - y0) * (vy - y0)) +
+               sqrtf((x2 - vx) * (x2 - vx) + (y2 - vy) * (y2 - vy));
+    }
+    float t1 = 1.0f / 3.0f, t2 = 2.0f / 3.0f;
+    float u1 = 1.0f - t1, u2 = 1.0f - t2;
+    float ax = (((x0 * (u1 * u1 * u1)) + (x1 * (3.0f * u1 * u1 * t1))) + (x2 * (3.0f * u1 * t1 * t1))) + (x3 * (t1 * t1 * t1));
+    float ay = (((y0 * (u1 * u1 * u1)) + (y1 * (3.0f * u1 * u1 * t1))) + (y2 * (3.0f * u1 * t1 * t1))) + (y3 * (t1 * t1 * t1));
+    float bx = (((x0 * (u2 * u2 * u2)) + (x1 * (3.0f * u2 * u2 * t2))) + (x2 * (3.0f * u2 * t2 * t2))) + (x3 * (t2 * t2 * t2));
+    float by = (((y0 * (u2 * u2 * u2)) + (y1 * (3.0f * u2 * u2 * t2))) + (y2 * (3.0f * u2 * t2 * t2))) + (y3 * (t2 * t2 * t2));
+    return (sqrtf((ax - x0) * (ax - x0) + (ay - y0) * (ay - y0)) + sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay))) +
+           sqrtf((x3 - bx) * (x3 - bx) + (y3 - by) * (y3 - by));
+}
+
+// Per scene path: total length and its segment CDF / PMF (written in place;
+// the table layout is fixed at packing).
+__global__ void k_upd_lengths(DevScene S, float *__restrict__ shape_lengths, float *__restrict__ path_cdf,
+                              float *__restrict__ path_pmf) {
+    uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= S.num_shapes) return;
+    uint32_t prep = S.scene_to_prepared[si];
+    const float *sd = S.shape_data + prep * 15u;
+    if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return;
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    float len = 0.f;
+    for (uint32_t k = 0; k < cnt; ++k) len += curve_polyline_length(S.curve_data + (size_t)(off + k) * 13u);
+    shape_lengths[si] = len;
+    uint32_t n = S.path_cdf_cnt[si];
+    if (n == 0) return;
+    float *cdf = path_cdf + S.path_cdf_off[si], *pmf = path_pmf + S.path_cdf_off[si];
+    bool ok = len > 0.f && isfinite(len);
+    float inv = ok ? 1.0f / len : 0.f, prev = 0.f;
+    for (uint32_t k = 0; k < n; ++k) {
+        float sl = k < cnt ? curve_polyline_length(S.curve_data + (size_t)(off + k) * 13u) : 0.f;
+        float sn = sl * inv;
+        pmf[k] = sn;
+        prev = sn + prev;
+        cdf[k] = prev;
+    }
+}
+
+// Shape CDF over prepared instances, summed in order by one thread (the
+// host's order, so the tables match a host packing bit for bit).
+__global__ void k_upd_shape_cdf(DevScene S, float *__restrict__ cdf, float *__restrict__ pmf) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float accum = 0.f;
+    for (uint32_t i = 0; i < S.n_cdf; ++i) {
+        float len = S.shape_lengths[S.shape_ids[i]];
+        accum += len;
+        cdf[i] = accum;
+        pmf[i] = len;
+    }
+    bool ok = accum > 0.f && isfinite(accum);
+    for (uint32_t i = 0; i < S.n_cdf; ++i) {
+        cdf[i] = ok ? cdf[i] / accum : 0.f;
+        pmf[i] = ok ? pmf[i] / accum : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batches: persistent kernels over a device work queue.  Every workgroup
 // takes the next item with one atomic, so uneven scenes and tiles balance
 // themselves; the loop ends when the queue is drained (every workgroup
@@ -2746,6 +2972,48 @@ void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *ite
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, stream, p, n, v);
+}
+
+void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
+                         hipStream_t stream) {
+    // the scene's own arena buffers, written in place
+    float *pts = const_cast<float *>(S.path_points);
+    float *curve_data = const_cast<float *>(S.curve_data);
+    float *shape_data = const_cast<float *>(S.shape_data);
+    float *shape_bounds = const_cast<float *>(S.shape_bounds);
+    float *group_data = const_cast<float *>(S.group_data);
+    float *cpu_root = const_cast<float *>(S.group_cpu_bounds);
+    uint32_t *offsets = const_cast<uint32_t *>(S.tile_offsets);
+    uint32_t *entries = const_cast<uint32_t *>(S.tile_entries);
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    if (path_points && P.n_points)
+        hipLaunchKernelGGL(k_copy_f32, dim3(cdiv(2ull * P.n_points, 256)), dim3(256), 0, stream, pts, path_points,
+                           2u * P.n_points);
+    if (P.ncurves)
+        hipLaunchKernelGGL(k_upd_curves, dim3(cdiv(P.ncurves, 256)), dim3(256), 0, stream, curve_data, P.curve_pt, pts,
+                           P.ncurves);
+    if (S.num_prepared)
+        hipLaunchKernelGGL(k_upd_instances, dim3(cdiv(S.num_prepared, 256)), dim3(256), 0, stream, S, shape_data,
+                           shape_bounds, P.inst_pad, P.cpu_inst_bounds);
+    if (S.num_groups)
+        hipLaunchKernelGGL(k_upd_groups, dim3(cdiv(S.num_groups, 256)), dim3(256), 0, stream, S, shape_bounds,
+                           P.cpu_inst_bounds, group_data, group_colors, reinterpret_cast<int4 *>(P.ranges), cpu_root);
+    if (ntiles > 0) {
+        launch_bin_count(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, P.tile_counts,
+                         stream);
+        launch_scan(P.tile_counts, ntiles, offsets, stream);
+        hipLaunchKernelGGL(k_clamp_offsets, dim3(cdiv(ntiles + 1, 256)), dim3(256), 0, stream, offsets, ntiles,
+                           P.tile_cap, P.overflow);
+        launch_bin_write(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, offsets,
+                         entries, stream);
+    }
+    if (S.n_cdf > 0 && !S.prefilter) {
+        hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
+                           const_cast<float *>(S.shape_lengths), const_cast<float *>(S.path_cdf),
+                           const_cast<float *>(S.path_pmf));
+        hipLaunchKernelGGL(k_upd_shape_cdf, dim3(1), dim3(64), 0, stream, S, const_cast<float *>(S.shape_cdf),
+                           const_cast<float *>(S.shape_pmf));
+    }
 }
 
 }  // namespace dvgk

@@ -144,6 +144,8 @@ SIGNATURES = {
                                             C.POINTER(RenderParamsC), C.POINTER(_vp)]),
     "dvg_device_scene_destroy": (None, [_vp]),
     "dvg_device_scene_set_shapes": (C.c_int32, [_vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "dvg_device_scene_update": (C.c_int32, [_vp, _vp, _vp, _vp]),
+    "dvg_device_scene_check": (C.c_int32, [_vp, _vp]),
     "dvg_device_render": (C.c_int32, [_vp, _vp, _vp]),
     "dvg_device_render_backward": (C.c_int32, [_vp, _vp, _vp, C.POINTER(GradsC)]),
     "dvg_sync": (C.c_int32, [_vp]),

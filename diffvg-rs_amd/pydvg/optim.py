@@ -84,3 +84,70 @@ class AdamLoop:
         g_cols = np.concatenate([d_grp[gi, 8:12] for gi in self.solid]) if self.solid else np.zeros(0, np.float32)
         self._set(self.opt_p.update(pts, d_pts), self.opt_c.update(cols, g_cols))
         return loss
+
+
+class DeviceAdamLoop:
+    """The optimisation of AdamLoop with the parameters, the Adam state and
+    the scene packing resident on the device: a step is device re-packing
+    (dvg_device_scene_update) -> forward -> loss gradient -> backward -> Adam,
+    with no host round trip except the optional loss readback.
+
+    The renderer's stream is set to torch's current stream, so that the
+    library's kernels and the torch arithmetic are ordered on one stream."""
+
+    def __init__(self, renderer, scene, options, target, lr_points=0.5, lr_colors=1e-2):
+        import torch
+        self.torch = torch
+        self.r = renderer
+        renderer.set_stream(torch.cuda.current_stream().cuda_stream)
+        self.prepared = PreparedScene(scene, options)
+        self.ds = DeviceScene(renderer, self.prepared)
+        desc = self.prepared.desc
+        G = desc.num_groups
+        self.G = G
+        self.points = torch.tensor(np.asarray(desc.path_points, np.float32), device="cuda")
+        cols = np.concatenate([np.asarray(desc.fill_color, np.float32).reshape(G, 4),
+                               np.asarray(desc.stroke_color, np.float32).reshape(G, 4)], axis=1)
+        self.colors = torch.tensor(cols.reshape(-1), device="cuda")
+        mask = np.zeros((G, 8), np.float32)
+        for gi, g in enumerate(scene.groups):
+            if isinstance(g.fill, Solid):
+                mask[gi, :4] = 1.0
+        self.mask = torch.tensor(mask.reshape(-1), device="cuda")
+        self.lr_p, self.lr_c = lr_points, lr_colors
+        self.b1, self.b2, self.eps = 0.9, 0.999, 1e-8
+        self.t = 0
+        self.m_p = torch.zeros_like(self.points)
+        self.v_p = torch.zeros_like(self.points)
+        self.m_c = torch.zeros_like(self.colors)
+        self.v_c = torch.zeros_like(self.colors)
+        self.target = target
+        self.img = torch.empty(scene.width * scene.height * 4, dtype=torch.float32, device="cuda")
+        self.d_render = torch.empty_like(self.img)
+        self.bufs, self.sizes = self.ds.grad_buffers(torch)
+
+    def _adam(self, x, g, m, v, lr):
+        m.mul_(self.b1).add_(g, alpha=1 - self.b1)
+        v.mul_(self.b2).addcmul_(g, g, value=1 - self.b2)
+        mh = m / (1 - self.b1 ** self.t)
+        vh = v / (1 - self.b2 ** self.t)
+        x.sub_(lr * mh / (vh.sqrt() + self.eps))
+
+    def step(self, want_loss=True):
+        torch = self.torch
+        self.ds.update(self.points, self.colors)
+        self.ds.render(self.img)
+        torch.sub(self.img, self.target, out=self.d_render)
+        loss = (self.d_render * self.d_render).sum()
+        self.d_render.mul_(2.0)
+        self.ds.render_backward(self.d_render, self.bufs, self.sizes)
+        self.t += 1
+        n = self.points.numel()
+        if n:
+            self._adam(self.points, self.bufs["d_shape_points"][:n], self.m_p, self.v_p, self.lr_p)
+        g_c = self.bufs["d_group_data"][: 16 * self.G].view(self.G, 16)[:, 8:16].reshape(-1) * self.mask
+        old = self.colors.clone()
+        self._adam(self.colors, g_c, self.m_c, self.v_c, self.lr_c)
+        # only solid fill colours move, kept in [0, 1]
+        self.colors.copy_(torch.where(self.mask > 0, self.colors.clamp(0.0, 1.0), old))
+        return float(loss) if want_loss else loss

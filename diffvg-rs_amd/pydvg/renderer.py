@@ -137,6 +137,17 @@ class DeviceScene:
             self.r.lib.dvg_device_scene_destroy(h)
             self.handle = None
 
+    def update(self, path_points=None, group_colors=None):
+        """Device-side re-packing from torch CUDA tensors: path points (P x 2,
+        scene layout) and group colours (G x 8: fill then stroke RGBA)."""
+        pp = C.c_void_p(path_points.data_ptr()) if path_points is not None else None
+        gc = C.c_void_p(group_colors.data_ptr()) if group_colors is not None else None
+        st = self.r.lib.dvg_device_scene_update(self.r.ctx, self.handle, pp, gc)
+        _check(self.r.lib, self.r.ctx, st)
+
+    def check(self):
+        _check(self.r.lib, self.r.ctx, self.r.lib.dvg_device_scene_check(self.r.ctx, self.handle))
+
     def set_shapes(self, sblock_log2=2, bnd_log2=2, unit_log2=0):
         """Launch shapes (see dvg_device_scene_set_shapes)."""
         st = self.r.lib.dvg_device_scene_set_shapes(self.handle, sblock_log2, bnd_log2, unit_log2)

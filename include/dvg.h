@@ -384,6 +384,20 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,
 /* Block until the context's stream is idle. */
 int32_t dvg_sync(dvg_ctx *ctx);
 
+/* Device-side re-packing for optimisation loops (the per-call prepare_scene
+ * of renderer.rs:39 and build_boundary_sampling_data of
+ * backward/boundary.rs:46, restated on the GPU for the inputs that change):
+ * path_points (device, scene layout of dvg_backward_pack.path_points, or
+ * NULL) and group_colors (device, 8 floats per group: fill RGBA then stroke
+ * RGBA, or NULL) replace the scene's; curves, bounds, tile lists and boundary
+ * tables follow.  Asynchronous on the context's stream.  The scene must have
+ * been created with its backward tables. */
+int32_t dvg_device_scene_update(dvg_ctx *ctx, dvg_device_scene *scene, const float *path_points,
+                                const float *group_colors);
+/* Synchronizes and reports DVG_DEVICE_ERROR if an update made the tile
+ * lists outgrow their buffer (lists were cut short: recreate the scene). */
+int32_t dvg_device_scene_check(dvg_ctx *ctx, dvg_device_scene *scene);
+
 /* Launch shapes of one device scene, as in dvg_batch_config (sample blocks
  * 1 << sblock_log2 pixels, boundary buckets 1 << bnd_log2 pixels, tile work
  * in units of 256 << unit_log2 samples); results do not depend on them
