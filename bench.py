@@ -9,8 +9,10 @@ scene with path points jittered by N(0, 2 px).
 
 One step = weights + forward + resolve + weights + interior backward +
 boundary sampling over all 4,194,304 pixel-samples, scene resident in HBM.
-`value` = pixel-samples / s of that step.  Adam steps/s (pack on the host,
-upload, forward, loss gradient, backward, Adam update) is reported beside it.
+`value` = pixel-samples / s of that step.  Adam steps/s (re-pack on the
+device, forward, loss gradient, backward, Adam update, all device resident)
+is reported beside it; for c5 the Adam loop runs against the procedural
+Gaussian target and the line adds its loss curve and device memory in use.
 
 --workload picks another configuration for the same measurement (one JSON
 line each): c1 (256x256, 64 cubic paths, 4x4 spp), c2 (512x512, 512 mixed
@@ -54,7 +56,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--adam-steps", type=int, default=5)
+    ap.add_argument("--adam-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
@@ -66,7 +68,7 @@ def main():
     import numpy as np
     import torch
     from pydvg import scenes
-    from pydvg.optim import AdamLoop
+    from pydvg.optim import DeviceAdamLoop
     from pydvg.renderer import DeviceScene, PreparedScene, Renderer
 
     torch.cuda.init()
@@ -136,16 +138,33 @@ def main():
     cal_dst.copy_(cal_src)
     torch.cuda.synchronize()
 
-    # Adam steps/s: host packing + upload + forward + loss + backward + update
+    # Adam steps/s: device re-packing + forward + loss + backward + Adam update,
+    # parameters and optimiser state resident on the device
     adam_s = None
-    if headline and args.adam_steps > 0:
-        loop = AdamLoop(r, sc, opts, tgt, lr_points=0.5, lr_colors=1e-2)
-        loop.step()
+    adam = None
+    if (headline or args.workload == "c5") and args.adam_steps > 0:
+        if args.workload == "c5":
+            tgt_adam = torch.tensor(scenes.gaussian_target(W, H, 256, seed=0), device=dev)
+        else:
+            tgt_adam = tgt
+        loop = DeviceAdamLoop(r, sc, opts, tgt_adam, lr_points=0.5, lr_colors=1e-2)
+        losses = [loop.step(want_loss=False)]
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        losses = [loop.step() for _ in range(args.adam_steps)]
+        for i in range(args.adam_steps):
+            losses.append(loop.step(want_loss=False))
+            if (i + 1) % 50 == 0:
+                log(f"bench: Adam step {i + 1}, loss {float(losses[-1]):.1f}")
+        torch.cuda.synchronize()
         adam_s = (time.perf_counter() - t1) / args.adam_steps
-        log(f"bench: Adam {1.0 / adam_s:.2f} steps/s, losses {losses}")
+        free, total = torch.cuda.mem_get_info()
+        curve = [float(x) for x in losses]
+        adam = {"steps": args.adam_steps, "steps_per_s": 1.0 / adam_s,
+                "loss_first": curve[:6], "loss_every_10": curve[::10], "loss_last": curve[-1],
+                "target": "256 anisotropic Gaussians" if args.workload == "c5" else "scene with points jittered N(0, 2 px)",
+                "device_used_bytes": total - free, "torch_peak_bytes": torch.cuda.max_memory_allocated()}
+        log(f"bench: Adam {1.0 / adam_s:.2f} steps/s, losses {curve[:6]} ... {curve[-1]:.1f}, "
+            f"device memory in use {(total - free) / 2**30:.2f} GiB")
 
     # per-kernel durations from HIP events recorded on the kernels' stream
     r.set_timing(True)
@@ -198,7 +217,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload_desc, "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu"},
-        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None,
+        "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None, "adam": adam,
         "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }

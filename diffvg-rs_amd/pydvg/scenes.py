@@ -21,6 +21,10 @@ coincide.
   config4(): 64 scenes of config2 shapes, path counts log-uniform in 16-4096,
       resolutions from {128, 256, 512, 1024}^2.
   config5(): 2048x2048, 10000 closed cubic paths, 2x2 samples.
+  config5_reduced(): the config5 generator at 256x256 with the path density
+      kept (156 paths): the size at which the CPU oracle runs its Adam loop.
+  gaussian_target(): the procedural optimisation target of config5, a sum of
+      256 random anisotropic Gaussians in RGB.
   jittered(scene, sigma, seed): the same scene with every path point moved by
       N(0, sigma) -- the optimisation target of configs 1 and 5.
 """
@@ -84,6 +88,41 @@ def config5(seed=0):
     sc = cubic_paths_scene(2048, 2048, 10000, seed, spread=64.0)
     opts = RenderOptions(samples_x=2, samples_y=2, seed=0, jitter=True, use_prefiltering=False)
     return sc, opts
+
+
+def config5_reduced(seed=0):
+    sc = cubic_paths_scene(256, 256, 156, seed, spread=64.0)
+    opts = RenderOptions(samples_x=2, samples_y=2, seed=0, jitter=True, use_prefiltering=False)
+    return sc, opts
+
+
+def gaussian_target(width, height, n=256, seed=0):
+    """RGBA float32 image (H*W*4, alpha 1): RGB = clip(sum_k a_k g_k(x), 0, 1)
+    with g_k(x) = exp(-0.5 (x - c_k)^T R_k diag(s_k)^-2 R_k^T (x - c_k)),
+    centre c_k uniform in the image, standard deviations s_k uniform in
+    [0.01, 0.08] x width, rotation angle uniform in [0, pi), amplitude a_k
+    uniform in [-0.3, 0.7]^3, evaluated in float64 at pixel centres within
+    4.5 standard deviations of the centre (the rest is below 4e-5)."""
+    rng = np.random.default_rng(seed)
+    img = np.zeros((height, width, 3), np.float64)
+    for _ in range(n):
+        cx, cy = rng.uniform(0, width), rng.uniform(0, height)
+        sx, sy = rng.uniform(0.01, 0.08, size=2) * width
+        th = rng.uniform(0.0, np.pi)
+        amp = rng.uniform(-0.3, 0.7, size=3)
+        r = 4.5 * max(sx, sy)
+        x0, x1 = max(int(cx - r), 0), min(int(cx + r) + 1, width)
+        y0, y1 = max(int(cy - r), 0), min(int(cy + r) + 1, height)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        ys, xs = np.mgrid[y0:y1, x0:x1]
+        dx, dy = xs + 0.5 - cx, ys + 0.5 - cy
+        c, s = np.cos(th), np.sin(th)
+        u, v = (c * dx + s * dy) / sx, (-s * dx + c * dy) / sy
+        img[y0:y1, x0:x1] += np.exp(-0.5 * (u * u + v * v))[..., None] * amp
+    out = np.ones((height, width, 4), np.float32)
+    out[..., :3] = np.clip(img, 0.0, 1.0)
+    return out.reshape(-1)
 
 
 def _stops(rng):

@@ -94,3 +94,24 @@ def test_device_adam_follows_host_adam(renderer):
     for a, b in zip(dev_losses, host_losses):
         assert abs(a - b) <= 1e-4 * abs(b)
     assert dev_losses[-1] < dev_losses[0]
+
+
+def test_device_adam_follows_oracle_loss_curve(renderer):
+    """The optimisation workload at 256x256 (156 paths, Gaussian target):
+    the device loop's first five losses against the CPU oracle's Adam loop
+    (tests/golden/c5_reduced_loss.json, written by make_c5_loss.py)."""
+    import json
+    import os
+    import torch
+    from pydvg.optim import DeviceAdamLoop
+    with open(os.path.join(os.path.dirname(__file__), "golden", "c5_reduced_loss.json")) as f:
+        ref = json.load(f)["losses"]
+    sc, opts = scenes.config5_reduced(seed=0)
+    tgt = torch.tensor(scenes.gaussian_target(sc.width, sc.height, 256, seed=0), device="cuda")
+    print("case device adam vs oracle loss curve", flush=True)
+    loop = DeviceAdamLoop(renderer, sc, opts, tgt, lr_points=0.5, lr_colors=1e-2)
+    losses = [loop.step() for _ in range(len(ref))]
+    rel = [abs(a - b) / b for a, b in zip(losses, ref)]
+    print("device", losses, "oracle", ref, "rel", rel)
+    assert rel[0] <= 1e-5
+    assert max(rel) <= 1e-3
