@@ -58,6 +58,11 @@ dvg_ctx *dvg_ctx_create(int device_ordinal) {
         return nullptr;
     }
     ctx->stream = ctx->own_stream;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && ncu > 0)
+        ctx->n_cu = (uint32_t)ncu;
+    else
+        ctx->n_cu = 256;
     return ctx;
 }
 
@@ -68,6 +73,7 @@ void dvg_ctx_destroy(dvg_ctx *ctx) {
     if (ctx->weight) (void)hipFree(ctx->weight);
     if (ctx->accum) (void)hipFree(ctx->accum);
     if (ctx->bwork) (void)hipFree(ctx->bwork);
+    if (ctx->imask) (void)hipFree(ctx->imask);
     for (const TimedLaunch &tl : ctx->pending) ctx->event_pool.push_back({tl.start, tl.stop});
     for (auto &e : ctx->event_pool) {
         (void)hipEventDestroy(e.first);
@@ -324,6 +330,27 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         r[0] = tx0; r[1] = ty0; r[2] = tx1; r[3] = ty1;
         n_entries += (uint64_t)(tx1 - tx0 + 1) * (uint64_t)(ty1 - ty0 + 1);
     }
+    // longest tile list: the group rectangles summed on a 2-D difference grid
+    uint32_t list_max = 0;
+    if (ntiles > 0) {
+        std::vector<int32_t> diff((size_t)(tiles_x + 1) * (tiles_y + 1), 0);
+        for (uint32_t g = 0; g < G; ++g) {
+            const int32_t *r = &ranges[4 * (size_t)g];
+            if (r[0] > r[2] || r[1] > r[3]) continue;
+            diff[(size_t)r[1] * (tiles_x + 1) + r[0]] += 1;
+            diff[(size_t)r[1] * (tiles_x + 1) + r[2] + 1] -= 1;
+            diff[(size_t)(r[3] + 1) * (tiles_x + 1) + r[0]] -= 1;
+            diff[(size_t)(r[3] + 1) * (tiles_x + 1) + r[2] + 1] += 1;
+        }
+        for (uint32_t y = 0; y <= tiles_y; ++y)
+            for (uint32_t x = 0; x <= tiles_x; ++x) {
+                size_t i = (size_t)y * (tiles_x + 1) + x;
+                if (x > 0) diff[i] += diff[i - 1];
+                if (y > 0) diff[i] += diff[i - (tiles_x + 1)];
+                if (x > 0 && y > 0) diff[i] -= diff[i - (tiles_x + 1) - 1];
+                if (x < tiles_x && y < tiles_y) list_max = std::max(list_max, (uint32_t)diff[i]);
+            }
+    }
 
     // scene_to_prepared: first instance of each scene shape
     uint32_t S = bp ? bp->num_shapes : 0;
@@ -511,6 +538,8 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     }
     ds->num_groups = G;
     ds->num_shapes = S;
+    ds->list_max = list_max;
+    ds->gcap = prm->use_prefiltering ? 0u : dvgk::interior_gcap(list_max);
     {
         size_t npix = (size_t)W * H;
         uint64_t lens[14] = {8ull * S,
@@ -657,9 +686,23 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         dvgk::launch_backward_prefiltered(S, ctx->weight, d_render, G, ctx->stream);
         return DVG_OK;
     }
+    dvgk::InteriorWork iw = {nullptr, nullptr, 0u, 0u};
+    if (ds->gcap > 0) {
+        // bit-set slots for the lists the LDS bit set cannot hold
+        uint32_t slots = std::max(ctx->n_cu, 1u) * dvgk::INTERIOR_WG_PER_CU;
+        size_t need = 1 + (size_t)slots * dvgk::interior_slot_words(ds->gcap);
+        if (ctx->imask_cap < need) {
+            if (ctx->imask) (void)hipFree(ctx->imask);
+            ctx->imask = nullptr;
+            ctx->imask_cap = 0;
+            HIP_TRY(ctx, hipMalloc(&ctx->imask, need * sizeof(uint32_t)));
+            ctx->imask_cap = need;
+        }
+        iw = {ctx->imask + 1, ctx->imask, ds->gcap, slots};
+    }
     {
         KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
-        dvgk::launch_backward_interior(S, ctx->weight, d_render, G, ctx->stream);
+        dvgk::launch_backward_interior(S, ctx->weight, d_render, G, iw, ctx->stream);
     }
     if (S.n_cdf > 0) {
         size_t ns = npix * S.spp, nt = dvgk::boundary_buckets(S);

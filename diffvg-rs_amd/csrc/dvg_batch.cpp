@@ -33,6 +33,9 @@ struct dvg_batch {
     std::vector<std::vector<uint32_t>> tile_off;  // tile list offsets per scene
     std::vector<uint32_t> nbmax;                   // bucket capacity per scene
     int n_cu = 256;
+    // colour-adjoint bit-set slots for tile lists longer than the LDS bit set
+    uint32_t *gmask = nullptr;
+    uint32_t gcap = 0;
     dvg_batch_config cfg = {2, 2, 4, 0, 0.f};
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
@@ -153,11 +156,16 @@ void launch_step(dvg_ctx *ctx, dvg_batch *b, uint32_t flags) {
     hipStream_t st = ctx->stream;
     dvgk::launch_fill_u32(b->counters, dvgk::NUM_PHASES, 0u, st);
     uint32_t cap = (uint32_t)b->n_cu * b->cfg.wg_per_cu;
+    // the colour adjoint's workgroups beyond those resident would find the
+    // queue drained; each launched one owns a bit-set slot
+    uint32_t cap_int = std::min(cap, (uint32_t)b->n_cu * dvgk::INTERIOR_WG_PER_CU);
     auto run = [&](int ph, bool loss = false) {
         uint32_t ni = (uint32_t)b->items[ph].size();
         if (ni == 0) return;
+        bool interior = ph == dvgk::PH_INTERIOR;
         dvgk::launch_batch_phase(ph, b->dev_bs, b->dev_items + b->item_off[ph], ni, b->counters + ph,
-                                 std::min(ni, cap), loss, st);
+                                 std::min(ni, interior ? cap_int : cap), loss, interior ? b->gmask : nullptr,
+                                 interior ? b->gcap : 0u, st);
     };
     bool fwd = flags & DVG_BATCH_FORWARD, loss = flags & DVG_BATCH_LOSS, bwd = flags & DVG_BATCH_BACKWARD;
     run(dvgk::PH_ZERO);
@@ -203,6 +211,15 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
         if (st) return fail(st);
         b->scenes.push_back(ds);
         b->targets.push_back(targets ? targets[i] : nullptr);
+        b->gcap = std::max(b->gcap, ds->gcap);
+    }
+    if (b->gcap > 0) {
+        size_t words = (size_t)b->n_cu * dvgk::INTERIOR_WG_PER_CU * dvgk::interior_slot_words(b->gcap);
+        hipError_t e = hipMalloc(&b->gmask, words * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            b->gmask = nullptr;
+            return fail(hip_err(ctx, e, "batch bit-set slots"));
+        }
     }
     // one allocation for every work buffer; buckets sized for the finest
     // boundary blocks the autotuner may pick (2x2 pixels: 64 per tile)
@@ -305,6 +322,7 @@ void dvg_batch_destroy(dvg_batch *b) {
     if (b->work) (void)hipFree(b->work);
     if (b->counters) (void)hipFree(b->counters);
     if (b->dev_items) (void)hipFree(b->dev_items);
+    if (b->gmask) (void)hipFree(b->gmask);
     delete b;
 }
 

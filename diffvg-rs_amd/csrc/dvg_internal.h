@@ -36,6 +36,11 @@ struct dvg_ctx {
     // boundary-pass work buffers (u32), grown on demand
     uint32_t *bwork = nullptr;
     size_t bwork_cap = 0;
+    // colour-adjoint bit-set slots for long tile lists (u32: counter, then
+    // the slots), grown on demand
+    uint32_t *imask = nullptr;
+    size_t imask_cap = 0;
+    uint32_t n_cu = 0;
 };
 
 struct dvg_device_scene {
@@ -45,6 +50,8 @@ struct dvg_device_scene {
     size_t arena_bytes = 0;
     uint32_t num_groups = 0, num_shapes = 0;
     uint32_t grad_len[14] = {};  // lengths of the 14 gradient arrays
+    uint32_t list_max = 0;       // longest tile list at creation
+    uint32_t gcap = 0;           // list capacity of the adjoint's global bit-set slots (0: none needed)
 };
 
 // Brackets one kernel launch with timing events when timing is on.

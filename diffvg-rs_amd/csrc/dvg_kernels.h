@@ -1,6 +1,7 @@
 // Device scene layout and kernel launch interface shared by the host code and
 // dvg_kernels.hip.
 #pragma once
+#include <algorithm>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -68,8 +69,31 @@ void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uin
 void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream);
 void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream);
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
+// Bit-set slots of the colour adjoint for tile lists longer than the LDS bit
+// set (LDS_MASK_GROUPS entries): `slots` workgroup slots of
+// nw = (2 gcap + 31) / 32 words x BLOCK lanes at gmask, and one queue counter.
+// gmask null: no slots (long lists then use fragment windows).
+constexpr uint32_t LDS_MASK_GROUPS = 256;
+// Workgroups of the colour adjoint resident on one CU (4 waves / SIMD).
+constexpr uint32_t INTERIOR_WG_PER_CU = 4;
+// Longest list given a global bit-set slot (a slot is then 256 KiB x 4).
+constexpr uint32_t GMASK_MAX_GROUPS = 16384;
+struct InteriorWork {
+    uint32_t *gmask, *counter;
+    uint32_t gcap, slots;
+};
+inline size_t interior_slot_words(uint32_t gcap) { return (size_t)((2u * gcap + 31u) / 32u) * BLOCK; }
+// List capacity of the global bit-set slots for a scene whose longest tile
+// list has `list_max` entries (0: the LDS bit set holds every list).  The
+// margin leaves room for lists that grow under device-side updates.
+inline uint32_t interior_gcap(uint32_t list_max) {
+    if (list_max <= LDS_MASK_GROUPS) return 0;
+    uint64_t c = (uint64_t)list_max + list_max / 4 + 64;
+    c = (c + 63) / 64 * 64;
+    return (uint32_t)std::min<uint64_t>(c, GMASK_MAX_GROUPS);
+}
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                              hipStream_t stream);
+                              const InteriorWork &iw, hipStream_t stream);
 // Prefiltered (analytic coverage) colour adjoint with its distance adjoint.
 void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                                  hipStream_t stream);
@@ -150,7 +174,8 @@ enum BatchPhase : int {
 // (items[0..n_items), next item = atomicAdd(counter, 1)).  `loss` makes the
 // resolve phase also write d_render = 2 (img - target).
 void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,
-                        uint32_t *counter, uint32_t grid, bool loss, hipStream_t stream);
+                        uint32_t *counter, uint32_t grid, bool loss, uint32_t *gmask, uint32_t gcap,
+                        hipStream_t stream);
 // n words of p set to v (a kernel, so that it can be captured in a graph).
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream);
 

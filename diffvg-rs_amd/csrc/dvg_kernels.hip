@@ -990,6 +990,7 @@ DVG_DEV void frag_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, 
 // BLOCK); MASK_WORDS words cover lists of up to 16 * MASK_WORDS groups.
 constexpr int MASK_WORDS = FRAG_WINDOW;
 constexpr uint32_t MASK_GROUPS = 16u * MASK_WORDS;
+static_assert(MASK_GROUPS == LDS_MASK_GROUPS, "LDS bit set size");
 
 DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t *mask,
                            float4 &fin) {
@@ -1010,8 +1011,80 @@ DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, 
     return count;
 }
 
+// Colour adjoint of the nf fragments recorded in a lane's bit set (bit 2e:
+// stroke of list entry e, bit 2e+1: its fill; fragment order = bit order),
+// taken in reverse as sampling.rs:76-224 does.  The prefix composite before
+// each fragment is recomposited from the bit set, not by a second walk over
+// the geometry: blocks of ADJ_BLOCK fragments from the top, each block's
+// prefix composited once from the background, then within the block from its
+// start -- O(nf^2 / ADJ_BLOCK + nf ADJ_BLOCK) paint evaluations, the same
+// blends in the same order as a full recomposition.
+//
+// The running adjoint shrinks by (1 - alpha) per fragment; once it is exactly
+// zero every fragment below adds exactly zero (finite paints), so the walk
+// down stops there -- under a deep stack of translucent fragments the f32
+// adjoint reaches zero after a few hundred fragments.
+constexpr uint32_t ADJ_BLOCK = 32;
+
+DVG_DEV bool adjoint_zero(float dr, float dg, float db, float da) {
+    return dr == 0.f && dg == 0.f && db == 0.f && da == 0.f;
+}
+
+DVG_DEV uint32_t next_bit(const uint32_t *mask, uint32_t nw, uint32_t &w, uint32_t &word) {
+    while (word == 0u) {
+        if (++w >= nw) return 0xffffffffu;
+        word = mask[w * BLOCK];
+    }
+    uint32_t b = (uint32_t)__ffs(word) - 1u;
+    word &= word - 1u;
+    return w * 32u + b;
+}
+
+DVG_DEV void mask_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, const uint32_t *s_off,
+                          const uint32_t *list, uint32_t n, const uint32_t *mask, uint32_t nf, float4 bg, float px,
+                          float py, float &dr, float &dg, float &db, float &da, float &dtx, float &dty) {
+    uint32_t nw = (2u * n + 31u) / 32u;
+    if (nf == 0u || nw == 0u) return;
+    uint32_t nblk = (nf + ADJ_BLOCK - 1u) / ADJ_BLOCK;
+    for (int jb = (int)nblk - 1; jb >= 0; --jb) {
+        if (adjoint_zero(dr, dg, db, da)) return;
+        uint32_t first = (uint32_t)jb * ADJ_BLOCK, cnt = min(ADJ_BLOCK, nf - first);
+        // prefix composite before fragment `first`; (w0, word0) = the scan
+        // state at the block's first fragment
+        float4 pre = bg;
+        uint32_t w0 = 0u, word0 = mask[0];
+        // (the bit set holds exactly nf bits: next_bit never runs out below)
+        for (uint32_t k = 0; k < first; ++k) {
+            uint32_t bit = next_bit(mask, nw, w0, word0);
+            if (bit >= 2u * n) return;
+            blend_cpu(pre, frag_color(S, list[bit >> 1], (bit & 1u) == 0u, px, py));
+        }
+        for (int i = (int)cnt - 1; i >= 0; --i) {
+            float4 pc = pre;
+            uint32_t w = w0, word = word0;
+            for (int k = 0; k < i; ++k) {
+                uint32_t bit = next_bit(mask, nw, w, word);
+                if (bit >= 2u * n) return;
+                blend_cpu(pc, frag_color(S, list[bit >> 1], (bit & 1u) == 0u, px, py));
+            }
+            uint32_t bit = next_bit(mask, nw, w, word);
+            if (bit >= 2u * n) return;
+            uint32_t e = bit >> 1;
+            bool is_stroke = (bit & 1u) == 0u;
+            float4 c = frag_color(S, list[e], is_stroke, px, py);
+            frag_adjoint(S, G, s_stage, s_off, list, e, is_stroke, c, pc, px, py, dr, dg, db, da, dtx, dty);
+            if (adjoint_zero(dr, dg, db, da)) return;
+        }
+    }
+}
+
+// One unit of the colour adjoint.  `gmask` (or null) is this workgroup's
+// global bit-set slot for lists longer than the LDS bit set holds: word w of
+// lane l at gmask[w * BLOCK + l], for lists of up to `gcap` entries; longer
+// lists fall back to fragment windows (one geometry walk per window).
 DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__restrict__ weight,
-                           const float *__restrict__ d_render, const DevGrads &G) {
+                           const float *__restrict__ d_render, const DevGrads &G, uint32_t *__restrict__ gmask,
+                           uint32_t gcap) {
     __shared__ float s_stage[STAGE_FLOATS];
     __shared__ uint32_t s_off[2 * LDS_GROUP_CAP];
     __shared__ uint32_t s_total;
@@ -1025,6 +1098,8 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
     bool use_mask = n <= MASK_GROUPS;
+    bool use_gmask = !use_mask && gmask != nullptr && n <= gcap;
+    uint32_t *gids = use_gmask ? gmask + threadIdx.x : nullptr;
     stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
     uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
@@ -1043,8 +1118,9 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
         float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
         float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
         float4 pre = bg, fin = bg;
-        uint32_t nf = use_mask ? walk_mask(S, list, n, px, py, ids, fin)
-                               : walk_window(S, list, n, px, py, 0, ids, pre, fin);
+        uint32_t nf = use_mask    ? walk_mask(S, list, n, px, py, ids, fin)
+                      : use_gmask ? walk_mask(S, list, n, px, py, gids, fin)
+                                  : walk_window(S, list, n, px, py, 0, ids, pre, fin);
         float4 col;
         float dtx = 0.f, dty = 0.f;
         if (nf == 0) {
@@ -1071,35 +1147,13 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
                 da -= dot / fa;
             }
             if (use_mask) {
-                // fragments in reverse order; each prefix colour is recomposited
-                // from the bit set (no second walk over the geometry)
-                uint32_t nw = (2u * n + 31u) / 32u;
-                for (int w = (int)nw - 1; w >= 0; --w) {
-                    uint32_t word = ids[w * BLOCK];
-                    while (word) {
-                        uint32_t b = 31u - (uint32_t)__clz(word);
-                        word &= ~(1u << b);
-                        uint32_t bit = (uint32_t)w * 32u + b;
-                        float4 pc = bg;
-                        for (int w2 = 0; w2 <= w; ++w2) {
-                            uint32_t m2 = ids[w2 * BLOCK];
-                            if (w2 == w) m2 &= (1u << b) - 1u;
-                            while (m2) {
-                                uint32_t b2 = (uint32_t)__ffs(m2) - 1u;
-                                m2 &= m2 - 1u;
-                                uint32_t bit2 = (uint32_t)w2 * 32u + b2;
-                                blend_cpu(pc, frag_color(S, list[bit2 >> 1], (bit2 & 1u) == 0u, px, py));
-                            }
-                        }
-                        uint32_t e = bit >> 1;
-                        bool is_stroke = (bit & 1u) == 0u;
-                        float4 c = frag_color(S, list[e], is_stroke, px, py);
-                        frag_adjoint(S, G, s_stage, s_off, list, e, is_stroke, c, pc, px, py, dr, dg, db, da, dtx, dty);
-                    }
-                }
+                mask_adjoint(S, G, s_stage, s_off, list, n, ids, nf, bg, px, py, dr, dg, db, da, dtx, dty);
+            } else if (use_gmask) {
+                mask_adjoint(S, G, s_stage, s_off, list, n, gids, nf, bg, px, py, dr, dg, db, da, dtx, dty);
             } else {
                 uint32_t nwin = (nf + FRAG_WINDOW - 1) / FRAG_WINDOW;
                 for (int w = (int)nwin - 1; w >= 0; --w) {
+                    if (adjoint_zero(dr, dg, db, da)) break;
                     uint32_t wstart = (uint32_t)w * FRAG_WINDOW;
                     float4 wpre = bg;
                     if (nwin > 1) {
@@ -1156,7 +1210,26 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
 
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior(
     DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G) {
-    interior_unit(S, blockIdx.x, weight, d_render, G);
+    interior_unit(S, blockIdx.x, weight, d_render, G, nullptr, 0u);
+}
+
+// Scenes with lists longer than the LDS bit set: a persistent grid takes the
+// units from a queue; workgroup b owns bit-set slot b of `gmask`
+// (nw words x BLOCK lanes a slot).
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_backward_interior_q(
+    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
+    uint32_t *__restrict__ counter, uint32_t n_units, uint32_t *__restrict__ gmask, uint32_t gcap) {
+    __shared__ uint32_t s_unit;
+    uint32_t nw = (2u * gcap + 31u) / 32u;
+    uint32_t *slot = gmask + (size_t)blockIdx.x * nw * BLOCK;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1u);
+        __syncthreads();
+        uint32_t u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= n_units) break;
+        interior_unit(S, u, weight, d_render, G, slot, gcap);
+    }
 }
 
 // 3x3 helpers on affine matrices given by their first two rows.
@@ -2833,8 +2906,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_resolve(DVG_QUEUE_ARGS, int los
 }
 
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_batch_interior(
-    DVG_QUEUE_ARGS) {
-    DVG_QUEUE_LOOP(interior_unit(B.S, w.unit, B.weight, B.d_render, B.G))
+    DVG_QUEUE_ARGS, uint32_t *__restrict__ gmask, uint32_t gcap) {
+    uint32_t nw = (2u * gcap + 31u) / 32u;
+    uint32_t *slot = gmask ? gmask + (size_t)blockIdx.x * nw * BLOCK : nullptr;
+    DVG_QUEUE_LOOP(interior_unit(B.S, w.unit, B.weight, B.d_render, B.G, slot, gcap))
 }
 
 __global__ __launch_bounds__(BLOCK) void k_batch_prefiltered(DVG_QUEUE_ARGS) {
@@ -2909,9 +2984,15 @@ void launch_resolve(const DevScene &S, const float *weight, const float *accum, 
     hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)S.W * S.H, 256)), dim3(256), 0, stream, S, weight, accum, out);
 }
 void launch_backward_interior(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
-                              hipStream_t stream) {
-    hipLaunchKernelGGL(k_backward_interior, dim3(S.tiles_x * S.tiles_y * units_per_tile(S)), dim3(BLOCK), 0, stream,
-                       S, weight, d_render, G);
+                              const InteriorWork &iw, hipStream_t stream) {
+    uint32_t n_units = S.tiles_x * S.tiles_y * units_per_tile(S);
+    if (!iw.gmask || iw.slots == 0) {
+        hipLaunchKernelGGL(k_backward_interior, dim3(n_units), dim3(BLOCK), 0, stream, S, weight, d_render, G);
+        return;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, iw.counter, 1u, 0u);
+    hipLaunchKernelGGL(k_backward_interior_q, dim3(std::min(n_units, iw.slots)), dim3(BLOCK), 0, stream, S, weight,
+                       d_render, G, iw.counter, n_units, iw.gmask, iw.gcap);
 }
 void launch_backward_prefiltered(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                                  hipStream_t stream) {
@@ -2940,7 +3021,8 @@ void launch_boundary_eval(const DevScene &S, const float *weight, const float *d
 }
 
 void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,
-                        uint32_t *counter, uint32_t grid, bool loss, hipStream_t stream) {
+                        uint32_t *counter, uint32_t grid, bool loss, uint32_t *gmask, uint32_t gcap,
+                        hipStream_t stream) {
     if (n_items == 0 || grid == 0) return;
     dim3 g(grid), b(BLOCK);
     switch (phase) {
@@ -2955,7 +3037,9 @@ void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *ite
         case PH_WEIGHTS_BWD:
             hipLaunchKernelGGL(k_batch_weights<true>, g, b, 0, stream, scenes, items, n_items, counter);
             break;
-        case PH_INTERIOR: hipLaunchKernelGGL(k_batch_interior, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_INTERIOR:
+            hipLaunchKernelGGL(k_batch_interior, g, b, 0, stream, scenes, items, n_items, counter, gmask, gcap);
+            break;
         case PH_PREFILTERED:
             hipLaunchKernelGGL(k_batch_prefiltered, g, b, 0, stream, scenes, items, n_items, counter);
             break;

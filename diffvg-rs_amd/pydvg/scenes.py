@@ -100,16 +100,17 @@ def gaussian_target(width, height, n=256, seed=0):
     """RGBA float32 image (H*W*4, alpha 1): RGB = clip(sum_k a_k g_k(x), 0, 1)
     with g_k(x) = exp(-0.5 (x - c_k)^T R_k diag(s_k)^-2 R_k^T (x - c_k)),
     centre c_k uniform in the image, standard deviations s_k uniform in
-    [0.01, 0.08] x width, rotation angle uniform in [0, pi), amplitude a_k
-    uniform in [-0.3, 0.7]^3, evaluated in float64 at pixel centres within
+    [20, 300] px x width / 2048, rotation angle uniform in [0, pi), amplitude
+    a_k uniform in [-0.15, 0.25]^3 (mean image value about 0.5), evaluated in
+    float64 at pixel centres within
     4.5 standard deviations of the centre (the rest is below 4e-5)."""
     rng = np.random.default_rng(seed)
     img = np.zeros((height, width, 3), np.float64)
     for _ in range(n):
         cx, cy = rng.uniform(0, width), rng.uniform(0, height)
-        sx, sy = rng.uniform(0.01, 0.08, size=2) * width
+        sx, sy = rng.uniform(20.0, 300.0, size=2) * (width / 2048.0)
         th = rng.uniform(0.0, np.pi)
-        amp = rng.uniform(-0.3, 0.7, size=3)
+        amp = rng.uniform(-0.15, 0.25, size=3)
         r = 4.5 * max(sx, sy)
         x0, x1 = max(int(cx - r), 0), min(int(cx + r) + 1, width)
         y0, y1 = max(int(cy - r), 0), min(int(cy + r) + 1, height)

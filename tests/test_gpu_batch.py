@@ -30,6 +30,8 @@ def _items():
         scenes.prefiltered_scene(40, 48, 6, 3, seed=13, step=8.0, radius=(4.0, 12.0), seg_range=(2, 5)),
         (scenes.mixed_scene(32, 32, 10, seed=14), pf),
         scenes.small_cubic_scene(20, 20, 3, seed=15, spp=3),
+        # lists longer than the LDS bit set: the batch's global bit-set slots
+        (scenes.mixed_scene(32, 32, 300, seed=16), scenes.config2()[1]),
     ]
 
 

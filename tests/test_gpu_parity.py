@@ -35,6 +35,10 @@ def _cases():
         "prefiltered48": scenes.prefiltered_scene(48, 48, 8, 4, seed=5, step=8.0, radius=(4.0, 14.0),
                                                   seg_range=(2, 6)),
         "prefiltered_mixed48": (scenes.mixed_scene(48, 48, 16, seed=6), _prefiltered_opts()),
+        # tile lists of 300-650 groups: the colour adjoint's global bit-set
+        # slots and its blocked prefix recomposition (nf well above 32)
+        "dense_mixed48": (scenes.mixed_scene(48, 48, 320, seed=7), scenes.config2()[1]),
+        "dense_mixed64": (scenes.mixed_scene(64, 64, 640, seed=8), scenes.config2()[1]),
     }
 
 
