@@ -65,3 +65,32 @@ def test_backward_matches_oracle(renderer, oracle, name):
     total, per = grad_report(g, gref)
     print(f"{name}: gradient rel L2 {total:.3g}; per field {per}")
     assert total <= GRAD_REL
+
+
+@pytest.mark.parametrize("bg_image", [False, True])
+def test_translation_and_background_image_gradients(renderer, oracle, bg_image):
+    """BackwardOptions.compute_translation (the per-pixel d_translation of
+    backward_gpu.rs:174-215) and a background image with its gradient
+    (background.rs:29-61) against the oracle."""
+    from pydvg.scene import BackwardOptions
+    sc, opts = scenes.mixed_scene(48, 48, 16, seed=10), scenes.config2()[1]
+    if bg_image:
+        rng = np.random.default_rng(3)
+        bg = rng.uniform(0.0, 1.0, size=(48, 48, 4)).astype(np.float32)
+        bg[..., 3] = rng.uniform(0.5, 1.0, size=(48, 48))
+        sc.background_image = bg
+    print(f"case translation bg_image={bg_image}", flush=True)
+    img = renderer.render(sc, opts)
+    ref = oracle.render(SceneDesc(sc), opts)
+    frac, mx = image_report(img, ref, IMG_TOL)
+    assert frac <= FLIP_FRACTION
+    tgt = oracle.render(SceneDesc(scenes.jittered(sc, 2.0, seed=9)), opts)
+    d_render = 2.0 * (ref - tgt)
+    g = renderer.render_backward(sc, opts, d_render, BackwardOptions(compute_translation=True))
+    gref = oracle.render_backward(SceneDesc(sc), opts, d_render, translation=True)
+    total, per = grad_report(g, gref)
+    print(f"translation bg_image={bg_image}: gradient rel L2 {total:.3g}; per field {per}")
+    assert total <= GRAD_REL
+    assert per["d_translation"] <= GRAD_REL
+    if bg_image:
+        assert per["d_background_image"] <= GRAD_REL

@@ -229,3 +229,48 @@ def test_backward_colour_gradient_matches_finite_differences(oracle):
             sc.groups[gi].fill = Solid(tuple(col))
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[gi, 8 + ch]) <= 1e-2 * max(1.0, abs(fd)), (gi, ch, fd, g[gi, 8 + ch])
+
+
+def _one_group_scene(geom, colour, size=64):
+    sc = Scene(size, size, background=(1.0, 1.0, 1.0, 1.0), filter=Filter(abi.FILTER_BOX, 0.5))
+    sc.shapes.append(Shape(geom))
+    sc.groups.append(ShapeGroup([0], fill=Solid(colour)))
+    return sc
+
+
+def test_boundary_gradient_of_circle_radius_is_the_area_derivative(oracle):
+    # One opaque circle of colour c over white, d_render = 1 on every channel:
+    # L = sum(image) = W*H*4 - (4 - sum c) * pi r^2 in expectation, so
+    # dL/dr = -(4 - sum c) * 2 pi r and dL/dc_xy = 0 (the circle stays inside
+    # the canvas).  Only boundary sampling (backward/boundary.rs:113-1148)
+    # produces the geometric terms; a wrong pdf, normal orientation or 1/(W H)
+    # normalisation fails this.
+    r, c = 17.3, (0.2, 0.5, 0.8, 1.0)
+    sc = _one_group_scene(Circle((31.7, 32.4), r), c)
+    opts = RenderOptions(samples_x=4, samples_y=4, seed=3, jitter=True, use_prefiltering=False)
+    g = oracle.render_backward(SceneDesc(sc), opts, np.ones(64 * 64 * 4, np.float32))
+    dr = float(g["d_shape_params"][2])
+    want = -(4.0 - sum(c)) * 2.0 * math.pi * r
+    assert abs(dr - want) <= 0.01 * abs(want), (dr, want)
+    # the centre derivative vanishes up to Monte Carlo noise
+    assert abs(g["d_shape_params"][0]) <= 0.01 * abs(want)
+    assert abs(g["d_shape_params"][1]) <= 0.01 * abs(want)
+
+
+def test_boundary_gradient_of_polygon_vertices_is_the_area_derivative(oracle):
+    # A closed 4-segment line path (counter-clockwise quadrilateral) of colour
+    # c over white with d_render = 1: dL/dv_i = -(4 - sum c) dA/dv_i with the
+    # shoelace derivative dA/dv_i = 1/2 (y_{i+1} - y_{i-1}, x_{i-1} - x_{i+1}).
+    pts = np.array([[14.0, 12.0], [50.0, 17.0], [46.0, 52.0], [11.0, 45.0]], np.float32)
+    c = (0.9, 0.1, 0.3, 1.0)
+    sc = _one_group_scene(Path(np.zeros(4, np.uint8), pts.copy(), is_closed=True), c)
+    opts = RenderOptions(samples_x=4, samples_y=4, seed=5, jitter=True, use_prefiltering=False)
+    g = oracle.render_backward(SceneDesc(sc), opts, np.ones(64 * 64 * 4, np.float32))
+    d = g["d_shape_points"].reshape(-1, 2)
+    x, y = pts[:, 0].astype(np.float64), pts[:, 1].astype(np.float64)
+    area = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
+    sgn = 1.0 if area > 0 else -1.0
+    dA = sgn * 0.5 * np.stack([np.roll(y, -1) - np.roll(y, 1), np.roll(x, 1) - np.roll(x, -1)], axis=1)
+    want = -(4.0 - sum(c)) * dA
+    scale = np.abs(want).max()
+    assert np.abs(d - want).max() <= 0.02 * scale, (d, want)
