@@ -295,6 +295,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         return set_err(ctx, DVG_INVALID_SCENE, "too many samples for 1d launch");
     uint32_t tiles_x = (W + dvgk::TILE - 1) / dvgk::TILE, tiles_y = (H + dvgk::TILE - 1) / dvgk::TILE;
     uint32_t ntiles = tiles_x * tiles_y;
+    const size_t ncurves = pk->curve_data ? pk->curve_data_len / 13 : 0;
 
     // Tile range of every group (binning.rs:30-110): its group-local bounds
     // mapped to canvas by shape_to_canvas, floored to 16-px tiles, clamped.
@@ -428,9 +429,10 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     bool plan_ok = bp && make_update_plan(pk, bp, prm, curve_pt, inst_pad);
     size_t i_cpt = addv(plan_ok ? curve_pt.data() : nullptr, plan_ok ? curve_pt.size() * 4 : 0);
     size_t i_pad = addv(plan_ok ? inst_pad.data() : nullptr, plan_ok ? inst_pad.size() * 4 : 0);
-    size_t i_cib = addv(nullptr, 0), i_flag = addv(nullptr, 0);
+    size_t i_cib = addv(nullptr, 0), i_flag = addv(nullptr, 0), i_wind = addv(nullptr, 0);
     A.srcs[i_cib].second = (size_t)std::max(SP, 1u) * 16;
     A.srcs[i_flag].second = 4;
+    A.srcs[i_wind].second = std::max<size_t>(ncurves, 1) * dvgk::WIND_STRIDE * 4;
     {  // offsets again: sizes of sourceless entries changed
         size_t t = 0;
         for (size_t k = 0; k < A.offs.size(); ++k) {
@@ -481,6 +483,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.num_prepared = SP;
     D.shape_data = (const float *)at(i_shape);
     D.curve_data = (const float *)at(i_curve);
+    D.curve_wind = (const float *)at(i_wind);
     D.shape_bounds = (const float *)at(i_sb);
     D.group_data = (const float *)at(i_gd);
     D.group_xform = (const float *)at(i_gx);
@@ -558,6 +561,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
                              npix * 2};
         for (int k = 0; k < 14; ++k) ds->grad_len[k] = (uint32_t)lens[k];
     }
+    dvgk::launch_wind_coef(D.curve_data, (uint32_t)ncurves, (float *)at(i_wind), ctx->stream);
     // tile lists: count, scan, write (sorted by group id by construction)
     if (ntiles > 0) {
         dvgk::launch_bin_count((const float *)at(i_rng), G, tiles_x, tiles_y, (uint32_t *)at(i_tcnt), ctx->stream);

@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "../../include/dvg.h"
+#include "dvg_kernels.h"
 
 namespace dvgk {
 
@@ -181,6 +182,123 @@ DVG_DEV void winding_cubic(float px, float py, float x0, float y0, float x1, flo
                 if (rule == 1u) cr = 1.f - cr;
                 else {
                     float dy = 3.0f * a * t * t + 2.0f * b * t + c;
+                    w += (dy > 0.f) ? 1.f : -1.f;
+                }
+            }
+        }
+    }
+}
+
+// Per-segment terms of the winding tests, computed once per segment
+// (k_wind_coef) with exactly the operations and order winding_quad /
+// winding_cubic / solve_cubic apply to them, so that the per-sample tests
+// below give bit-identical results while doing only the point-dependent part.
+// Record of WIND_STRIDE floats:
+//   cubic: a, b, c, q = (bb^2 - 3 cc) / 9, P = 2 bb^3 - 9 bb cc, q3 = q^3,
+//          sqrt(q3), sqrt(q), bb / 3, 3a, 2b   (bb = b / a, cc = c / a)
+//   quad:  ay, by, -, by^2, 4 ay, 2 ay
+
+DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
+    float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
+    float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
+    float c = -3.0f * y0 + 3.0f * y1;
+    o[0] = a; o[1] = b; o[2] = c;
+    float bb = b / a, cc = c / a;
+    float q = (bb * bb - 3.0f * cc) / 9.0f;
+    o[3] = q;
+    o[4] = 2.0f * bb * bb * bb - 9.0f * bb * cc;
+    float q3 = q * q * q;
+    o[5] = q3;
+    o[6] = sqrtf(q3);
+    o[7] = sqrtf(q);
+    o[8] = bb * (1.0f / 3.0f);
+    o[9] = 3.0f * a;
+    o[10] = 2.0f * b;
+    o[11] = 0.f;
+}
+
+DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
+    float ay = y0 - 2.0f * y1 + y2;
+    float by = -2.0f * y0 + 2.0f * y1;
+    o[0] = ay; o[1] = by; o[2] = 0.f;
+    o[3] = by * by;
+    o[4] = 4.0f * ay;
+    o[5] = 2.0f * ay;
+    for (int k = 6; k < WIND_STRIDE; ++k) o[k] = 0.f;
+}
+
+// winding_quad with the segment's terms from its record (same result).
+DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, float x2, const float *wc,
+                            uint32_t rule, float &w, float &cr) {
+    float ay = wc[0], by = wc[1];
+    float cy = y0 - py;
+    // solve_quadratic(ay, by, cy) with b*b and 4*a taken from the record
+    float disc = wc[3] - wc[4] * cy;
+    if (disc < 0.f) return;
+    float root = sqrtf(disc);
+    float qq = (by < 0.f) ? -0.5f * (by - root) : -0.5f * (by + root);
+    float t0 = qq / ay, t1 = cy / qq;
+    float r[2];
+    if (t0 > t1) { r[0] = t1; r[1] = t0; }
+    else { r[0] = t0; r[1] = t1; }
+    for (int i = 0; i < 2; ++i) {
+        float t = r[i];
+        if (t >= 0.f && t <= 1.f) {
+            float tt = 1.f - t;
+            float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
+            if (x > px) {
+                if (rule == 1u) cr = 1.f - cr;
+                else {
+                    float dy = wc[5] * t + by;
+                    w += (dy > 0.f) ? 1.f : -1.f;
+                }
+            }
+        }
+    }
+}
+
+// winding_cubic with the segment's terms from its record (same result).
+DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, float x2, float x3, const float *wc,
+                             uint32_t rule, float &w, float &cr) {
+    float a = wc[0];
+    float d = y0 - py;
+    float r[3];
+    int n;
+    const float eps = 1.0e-6f;
+    if (absf_(a) < eps) {
+        n = solve_quadratic(wc[1], wc[2], d, r[0], r[1]);
+    } else {
+        // solve_cubic (curves.rs:897-946) from the per-segment terms
+        float dd = d / a;
+        float q = wc[3];
+        float rr = (wc[4] + 27.0f * dd) / 54.0f;
+        float r2 = rr * rr, q3 = wc[5];
+        float off = wc[8];
+        if (r2 < q3) {
+            float theta = acosf(rr / wc[6]);
+            float sq = wc[7];
+            const float third = 1.0f / 3.0f;
+            const float two_pi = 2.0f * 3.14159265f;
+            r[0] = -2.0f * sq * cosf(theta * third) - off;
+            r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
+            r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+            n = 3;
+        } else {
+            float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));
+            float br = (absf_(ar) > eps) ? q / ar : 0.f;
+            r[0] = (ar + br) - off;
+            n = 1;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        float t = r[i];
+        if (t >= 0.f && t <= 1.f) {
+            float tt = 1.f - t;
+            float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+            if (x > px) {
+                if (rule == 1u) cr = 1.f - cr;
+                else {
+                    float dy = wc[9] * t * t + wc[10] * t + wc[2];
                     w += (dy > 0.f) ? 1.f : -1.f;
                 }
             }

@@ -11,6 +11,7 @@ constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.r
 constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
 constexpr int FRAG_WINDOW = 16;     // fragments held per lane in the adjoint walk
 constexpr int LDS_GROUP_CAP = 512;  // tile groups whose paint grads are staged in LDS
+constexpr int WIND_STRIDE = 12;     // floats of a segment's winding-term record
 
 // Device-resident scene: the packed buffers of PreparedScene /
 // PreparedBackwardScene / BoundarySamplingData plus the tile lists.
@@ -29,6 +30,9 @@ struct DevScene {
     const float *bg_image;  // W*H*4 or null
     // forward pack
     uint32_t num_groups, num_prepared;
+    // per-segment winding terms (WIND_STRIDE floats a segment), derived on
+    // the device from curve_data
+    const float *curve_wind;
     const float *shape_data, *curve_data, *shape_bounds, *group_data, *group_xform, *group_shape_xform,
         *group_inv_scale, *group_shapes, *shape_xform, *gradient_data, *stop_offsets, *stop_colors;
     // tiles
@@ -135,6 +139,9 @@ struct UpdatePlan {
 // Rewrites the packed geometry of S from path_points (scene layout; NULL =
 // unchanged) and group colours (8 per group: fill then stroke RGBA; NULL =
 // unchanged), rebuilds tile lists and boundary tables.  Asynchronous.
+// Per-segment winding terms of curve_data (13 floats a segment) into
+// curve_wind; after every change of curve_data.
+void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream);
 void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
                          hipStream_t stream);
 

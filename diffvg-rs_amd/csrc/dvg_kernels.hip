@@ -270,9 +270,10 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         // forward.rs:938-1018: a +x ray only meets segments spanning py with
         // some control point right of px.
         if (py < by0 || py > by1 || px > bx1) continue;
+        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
         if (k == 0) winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
-        else if (k == 1) winding_quad(px, py, x0, y0, x1, y1, x2, y2, rule, lw, lc);
-        else winding_cubic(px, py, x0, y0, x1, y1, x2, y2, x3, y3, rule, lw, lc);
+        else if (k == 1) winding_quad_w(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
+        else winding_cubic_w(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
     }
     if (lmin < c.fill_min) c.fill_min = lmin;
     if (rule == 1u) {
@@ -2649,6 +2650,22 @@ __global__ void k_copy_f32(float *__restrict__ dst, const float *__restrict__ sr
 }
 
 // Control points of every curve from the path points (path.rs:100-173).
+// Winding terms of every segment (the record described with WIND_STRIDE).
+__global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurves, float *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncurves) return;
+    const float *cv = curve_data + (size_t)i * 13u;
+    float *o = out + (size_t)i * WIND_STRIDE;
+    uint32_t k = (uint32_t)cv[0];
+    if (k == 2u) {
+        wind_coef_cubic(cv[2], cv[4], cv[6], cv[8], o);
+    } else if (k == 1u) {
+        wind_coef_quad(cv[2], cv[4], cv[6], o);
+    } else {
+        for (int j = 0; j < WIND_STRIDE; ++j) o[j] = 0.f;
+    }
+}
+
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
                              const float *__restrict__ pts, uint32_t ncurves) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3058,6 +3075,11 @@ void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
     hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, stream, p, n, v);
 }
 
+void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream) {
+    if (ncurves == 0) return;
+    hipLaunchKernelGGL(k_wind_coef, dim3(cdiv(ncurves, 256)), dim3(256), 0, stream, curve_data, ncurves, curve_wind);
+}
+
 void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
                          hipStream_t stream) {
     // the scene's own arena buffers, written in place
@@ -3076,6 +3098,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
     if (P.ncurves)
         hipLaunchKernelGGL(k_upd_curves, dim3(cdiv(P.ncurves, 256)), dim3(256), 0, stream, curve_data, P.curve_pt, pts,
                            P.ncurves);
+    launch_wind_coef(curve_data, P.ncurves, const_cast<float *>(S.curve_wind), stream);
     if (S.num_prepared)
         hipLaunchKernelGGL(k_upd_instances, dim3(cdiv(S.num_prepared, 256)), dim3(256), 0, stream, S, shape_data,
                            shape_bounds, P.inst_pad, P.cpu_inst_bounds);
