@@ -478,55 +478,55 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.filter_radius = prm->filter_radius;
     D.ri = (int)std::fmax(std::ceil(prm->filter_radius), 0.0f);
     for (int c = 0; c < 4; ++c) D.bg[c] = prm->background[c];
-    D.bg_image = bg_src ? (const float *)at(i_bg) : nullptr;
+    D.bg_image = bg_src ? (dvgk::kfloat *)at(i_bg) : nullptr;
     D.num_groups = G;
     D.num_prepared = SP;
-    D.shape_data = (const float *)at(i_shape);
-    D.curve_data = (const float *)at(i_curve);
-    D.curve_wind = (const float *)at(i_wind);
-    D.shape_bounds = (const float *)at(i_sb);
-    D.group_data = (const float *)at(i_gd);
-    D.group_xform = (const float *)at(i_gx);
-    D.group_shape_xform = (const float *)at(i_gsx);
-    D.group_inv_scale = (const float *)at(i_gis);
-    D.group_shapes = (const float *)at(i_gs);
-    D.shape_xform = (const float *)at(i_sx);
-    D.gradient_data = (const float *)at(i_grd);
-    D.stop_offsets = (const float *)at(i_so);
-    D.stop_colors = (const float *)at(i_sc);
+    D.shape_data = (dvgk::kfloat *)at(i_shape);
+    D.curve_data = (dvgk::kfloat *)at(i_curve);
+    D.curve_wind = (dvgk::kfloat *)at(i_wind);
+    D.shape_bounds = (dvgk::kfloat *)at(i_sb);
+    D.group_data = (dvgk::kfloat *)at(i_gd);
+    D.group_xform = (dvgk::kfloat *)at(i_gx);
+    D.group_shape_xform = (dvgk::kfloat *)at(i_gsx);
+    D.group_inv_scale = (dvgk::kfloat *)at(i_gis);
+    D.group_shapes = (dvgk::kfloat *)at(i_gs);
+    D.shape_xform = (dvgk::kfloat *)at(i_sx);
+    D.gradient_data = (dvgk::kfloat *)at(i_grd);
+    D.stop_offsets = (dvgk::kfloat *)at(i_so);
+    D.stop_colors = (dvgk::kfloat *)at(i_sc);
     D.tiles_x = tiles_x;
     D.tiles_y = tiles_y;
-    D.tile_offsets = (const uint32_t *)at(i_toff);
-    D.tile_entries = (const uint32_t *)at(i_tent);
+    D.tile_offsets = (dvgk::kuint *)at(i_toff);
+    D.tile_entries = (dvgk::kuint *)at(i_tent);
     if (bp) {
         D.num_shapes = S;
-        D.prepared_to_scene = (const uint32_t *)at(i_p2s);
-        D.shape_transform = (const float *)at(i_st);
-        D.path_points = (const float *)at(i_pp);
-        D.path_ctrl = (const uint32_t *)at(i_pc);
-        D.path_thickness = (const float *)at(i_pt);
-        D.sp_off = (const uint32_t *)at(i_spo);
-        D.sp_cnt = (const uint32_t *)at(i_spc);
-        D.sc_off = (const uint32_t *)at(i_sco);
-        D.sc_cnt = (const uint32_t *)at(i_scc);
-        D.st_off = (const uint32_t *)at(i_sto);
-        D.st_cnt = (const uint32_t *)at(i_stc);
-        D.s_closed = (const uint32_t *)at(i_scl);
+        D.prepared_to_scene = (dvgk::kuint *)at(i_p2s);
+        D.shape_transform = (dvgk::kfloat *)at(i_st);
+        D.path_points = (dvgk::kfloat *)at(i_pp);
+        D.path_ctrl = (dvgk::kuint *)at(i_pc);
+        D.path_thickness = (dvgk::kfloat *)at(i_pt);
+        D.sp_off = (dvgk::kuint *)at(i_spo);
+        D.sp_cnt = (dvgk::kuint *)at(i_spc);
+        D.sc_off = (dvgk::kuint *)at(i_sco);
+        D.sc_cnt = (dvgk::kuint *)at(i_scc);
+        D.st_off = (dvgk::kuint *)at(i_sto);
+        D.st_cnt = (dvgk::kuint *)at(i_stc);
+        D.s_closed = (dvgk::kuint *)at(i_scl);
         D.n_cdf = n_cdf;
-        D.shape_lengths = (const float *)at(i_sl);
-        D.shape_cdf = (const float *)at(i_cdf);
-        D.shape_pmf = (const float *)at(i_pmf);
-        D.shape_ids = (const uint32_t *)at(i_sid);
-        D.group_ids = (const uint32_t *)at(i_gid);
-        D.path_cdf = (const float *)at(i_pcdf);
-        D.path_pmf = (const float *)at(i_ppmf);
-        D.path_point_ids = (const uint32_t *)at(i_ppid);
-        D.path_cdf_off = (const uint32_t *)at(i_pco);
-        D.path_cdf_cnt = (const uint32_t *)at(i_pcc);
-        D.path_point_off = (const uint32_t *)at(i_ppo);
-        D.scene_to_prepared = (const uint32_t *)at(i_s2p);
+        D.shape_lengths = (dvgk::kfloat *)at(i_sl);
+        D.shape_cdf = (dvgk::kfloat *)at(i_cdf);
+        D.shape_pmf = (dvgk::kfloat *)at(i_pmf);
+        D.shape_ids = (dvgk::kuint *)at(i_sid);
+        D.group_ids = (dvgk::kuint *)at(i_gid);
+        D.path_cdf = (dvgk::kfloat *)at(i_pcdf);
+        D.path_pmf = (dvgk::kfloat *)at(i_ppmf);
+        D.path_point_ids = (dvgk::kuint *)at(i_ppid);
+        D.path_cdf_off = (dvgk::kuint *)at(i_pco);
+        D.path_cdf_cnt = (dvgk::kuint *)at(i_pcc);
+        D.path_point_off = (dvgk::kuint *)at(i_ppo);
+        D.scene_to_prepared = (dvgk::kuint *)at(i_s2p);
     }
-    D.group_cpu_bounds = (const float *)at(i_gcb);
+    D.group_cpu_bounds = (dvgk::kfloat *)at(i_gcb);
     if (plan_ok) {
         dvgk::UpdatePlan &P = ds->plan;
         P.ncurves = (uint32_t)(curve_pt.size() / 4);

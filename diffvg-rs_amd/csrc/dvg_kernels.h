@@ -13,6 +13,20 @@ constexpr int FRAG_WINDOW = 16;     // fragments held per lane in the adjoint wa
 constexpr int LDS_GROUP_CAP = 512;  // tile groups whose paint grads are staged in LDS
 constexpr int WIND_STRIDE = 12;     // floats of a segment's winding-term record
 
+// Scene arrays are read through the constant address space on the device:
+// they do not change while a render or adjoint kernel runs, so loads at
+// wave-uniform addresses (tile lists, group records, curve data of the group
+// in hand) become scalar loads whatever the kernel writes elsewhere.  Kernels
+// that rebuild scene arrays get writable pointers of their own.  The host
+// pass sees plain pointers of the same size (the struct layout is one).
+#if defined(__HIP_DEVICE_COMPILE__)
+using kfloat = const __attribute__((address_space(4))) float;
+using kuint = const __attribute__((address_space(4))) uint32_t;
+#else
+using kfloat = const float;
+using kuint = const uint32_t;
+#endif
+
 // Device-resident scene: the packed buffers of PreparedScene /
 // PreparedBackwardScene / BoundarySamplingData plus the tile lists.
 struct DevScene {
@@ -27,34 +41,34 @@ struct DevScene {
     float filter_radius;
     int ri;
     float bg[4];            // straight RGBA
-    const float *bg_image;  // W*H*4 or null
+    kfloat *bg_image;  // W*H*4 or null
     // forward pack
     uint32_t num_groups, num_prepared;
     // per-segment winding terms (WIND_STRIDE floats a segment), derived on
     // the device from curve_data
-    const float *curve_wind;
-    const float *shape_data, *curve_data, *shape_bounds, *group_data, *group_xform, *group_shape_xform,
+    kfloat *curve_wind;
+    kfloat *shape_data, *curve_data, *shape_bounds, *group_data, *group_xform, *group_shape_xform,
         *group_inv_scale, *group_shapes, *shape_xform, *gradient_data, *stop_offsets, *stop_colors;
     // tiles
     uint32_t tiles_x, tiles_y;
-    const uint32_t *tile_offsets, *tile_entries;
+    kuint *tile_offsets, *tile_entries;
     // backward pack (scene-indexed)
     uint32_t num_shapes;
-    const uint32_t *prepared_to_scene;
-    const float *shape_transform;  // 6*S
-    const float *path_points;
-    const uint32_t *path_ctrl;
-    const float *path_thickness;
-    const uint32_t *sp_off, *sp_cnt, *sc_off, *sc_cnt, *st_off, *st_cnt, *s_closed;
+    kuint *prepared_to_scene;
+    kfloat *shape_transform;  // 6*S
+    kfloat *path_points;
+    kuint *path_ctrl;
+    kfloat *path_thickness;
+    kuint *sp_off, *sp_cnt, *sc_off, *sc_cnt, *st_off, *st_cnt, *s_closed;
     uint32_t n_cdf;
-    const float *shape_lengths, *shape_cdf, *shape_pmf;
-    const uint32_t *shape_ids, *group_ids;
-    const float *path_cdf, *path_pmf;
-    const uint32_t *path_point_ids, *path_cdf_off, *path_cdf_cnt, *path_point_off;
-    const uint32_t *scene_to_prepared;  // S: first prepared instance of each scene shape
+    kfloat *shape_lengths, *shape_cdf, *shape_pmf;
+    kuint *shape_ids, *group_ids;
+    kfloat *path_cdf, *path_pmf;
+    kuint *path_point_ids, *path_cdf_off, *path_cdf_cnt, *path_point_off;
+    kuint *scene_to_prepared;  // S: first prepared instance of each scene shape
     // 4 per group, group-local: the root box of the reference's CPU group BVH
     // (union of distance/shape.rs shape_bounds); min > max when it has none
-    const float *group_cpu_bounds;
+    kfloat *group_cpu_bounds;
 };
 
 struct DevGrads {

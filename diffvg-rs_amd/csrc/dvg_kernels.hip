@@ -3083,14 +3083,14 @@ void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wi
 void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
                          hipStream_t stream) {
     // the scene's own arena buffers, written in place
-    float *pts = const_cast<float *>(S.path_points);
-    float *curve_data = const_cast<float *>(S.curve_data);
-    float *shape_data = const_cast<float *>(S.shape_data);
-    float *shape_bounds = const_cast<float *>(S.shape_bounds);
-    float *group_data = const_cast<float *>(S.group_data);
-    float *cpu_root = const_cast<float *>(S.group_cpu_bounds);
-    uint32_t *offsets = const_cast<uint32_t *>(S.tile_offsets);
-    uint32_t *entries = const_cast<uint32_t *>(S.tile_entries);
+    float *pts = (float *)(S.path_points);
+    float *curve_data = (float *)(S.curve_data);
+    float *shape_data = (float *)(S.shape_data);
+    float *shape_bounds = (float *)(S.shape_bounds);
+    float *group_data = (float *)(S.group_data);
+    float *cpu_root = (float *)(S.group_cpu_bounds);
+    uint32_t *offsets = (uint32_t *)(S.tile_offsets);
+    uint32_t *entries = (uint32_t *)(S.tile_entries);
     uint32_t ntiles = S.tiles_x * S.tiles_y;
     if (path_points && P.n_points)
         hipLaunchKernelGGL(k_copy_f32, dim3(cdiv(2ull * P.n_points, 256)), dim3(256), 0, stream, pts, path_points,
@@ -3098,7 +3098,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
     if (P.ncurves)
         hipLaunchKernelGGL(k_upd_curves, dim3(cdiv(P.ncurves, 256)), dim3(256), 0, stream, curve_data, P.curve_pt, pts,
                            P.ncurves);
-    launch_wind_coef(curve_data, P.ncurves, const_cast<float *>(S.curve_wind), stream);
+    launch_wind_coef(curve_data, P.ncurves, (float *)(S.curve_wind), stream);
     if (S.num_prepared)
         hipLaunchKernelGGL(k_upd_instances, dim3(cdiv(S.num_prepared, 256)), dim3(256), 0, stream, S, shape_data,
                            shape_bounds, P.inst_pad, P.cpu_inst_bounds);
@@ -3116,10 +3116,10 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
     }
     if (S.n_cdf > 0 && !S.prefilter) {
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
-                           const_cast<float *>(S.shape_lengths), const_cast<float *>(S.path_cdf),
-                           const_cast<float *>(S.path_pmf));
-        hipLaunchKernelGGL(k_upd_shape_cdf, dim3(1), dim3(64), 0, stream, S, const_cast<float *>(S.shape_cdf),
-                           const_cast<float *>(S.shape_pmf));
+                           (float *)(S.shape_lengths), (float *)(S.path_cdf),
+                           (float *)(S.path_pmf));
+        hipLaunchKernelGGL(k_upd_shape_cdf, dim3(1), dim3(64), 0, stream, S, (float *)(S.shape_cdf),
+                           (float *)(S.shape_pmf));
     }
 }
 
