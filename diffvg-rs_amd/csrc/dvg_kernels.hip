@@ -21,18 +21,19 @@
 namespace dvgk {
 
 // Waves per SIMD the heavy kernels are compiled for.  The scene walks are
-// VALU-issue bound with long memory waits between groups, so more resident
-// waves beat an unspilled register file: measured on the 1024x1024 /
-// 1000-path scene, 5 waves (<= 96 VGPRs) is best for the forward and the
-// boundary pass, 4 for the interior adjoint (its LDS allows no more).
+// VALU-issue bound with memory waits between groups, so more resident waves
+// beat an unspilled register file: measured on the 1024x1024 / 1000-path
+// scene, 7 waves (<= 72 VGPRs) is best for the forward and the boundary pass
+// (4.5 -> 3.7 ms from 4 to 7, slower again at 8), 4 for the colour adjoint
+// (its LDS allows no more).
 #ifndef OCC_FORWARD
-#define OCC_FORWARD 5
+#define OCC_FORWARD 7
 #endif
 #ifndef OCC_INTERIOR
 #define OCC_INTERIOR 4
 #endif
 #ifndef OCC_BOUNDARY
-#define OCC_BOUNDARY 5
+#define OCC_BOUNDARY 7
 #endif
 
 // ---------------------------------------------------------------------------
@@ -2185,6 +2186,9 @@ DVG_DEV bool sample_boundary_path(const DevScene &S, uint32_t si, float path_len
 
 // Colour of one boundary query point with the EdgeQuery hit tracking of
 // sampling.rs:76-224 (no adjoint).  (qx, qy) is the normalized point.
+DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
+                              uint32_t eq_prep, bool &hit);
+
 DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint32_t eq_group, uint32_t eq_prep,
                           bool &hit) {
     float px = qx * (float)S.W, py = qy * (float)S.H;
@@ -2192,6 +2196,16 @@ DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint
     tx = tx < 0 ? 0 : (tx >= (int)S.tiles_x ? (int)S.tiles_x - 1 : tx);
     ty = ty < 0 ? 0 : (ty >= (int)S.tiles_y ? (int)S.tiles_y - 1 : ty);
     uint32_t tile = (uint32_t)ty * S.tiles_x + (uint32_t)tx;
+    // samples are sorted by pixel block, so the lanes of a wave almost always
+    // query one tile: then the walk runs with the tile in a scalar register
+    // (list, group and curve data through scalar loads)
+    uint32_t t0 = __builtin_amdgcn_readfirstlane(tile);
+    if (__ballot(tile != t0) == 0) return edge_walk_tile(S, t0, px, py, bg, eq_group, eq_prep, hit);
+    return edge_walk_tile(S, tile, px, py, bg, eq_group, eq_prep, hit);
+}
+
+DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
+                              uint32_t eq_prep, bool &hit) {
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     float4 acc = bg;
     uint32_t nfr = 0;
