@@ -9,8 +9,8 @@ namespace dvgk {
 
 constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.rs:21)
 constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
-constexpr int FRAG_WINDOW = 16;     // fragments held per lane in the adjoint walk
-constexpr int LDS_GROUP_CAP = 512;  // tile groups whose paint grads are staged in LDS
+constexpr int FRAG_WINDOW = 8;      // fragments held per lane in the adjoint walk
+constexpr int LDS_GROUP_CAP = 256;  // tile groups whose paint grads are staged in LDS
 constexpr int WIND_STRIDE = 12;     // floats of a segment's winding-term record
 
 // Scene arrays are read through the constant address space on the device:
@@ -91,9 +91,10 @@ void launch_resolve(const DevScene &S, const float *weight, const float *accum, 
 // set (LDS_MASK_GROUPS entries): `slots` workgroup slots of
 // nw = (2 gcap + 31) / 32 words x BLOCK lanes at gmask, and one queue counter.
 // gmask null: no slots (long lists then use fragment windows).
-constexpr uint32_t LDS_MASK_GROUPS = 256;
-// Workgroups of the colour adjoint resident on one CU (4 waves / SIMD).
-constexpr uint32_t INTERIOR_WG_PER_CU = 4;
+constexpr uint32_t LDS_MASK_GROUPS = 128;
+// Workgroups of the colour adjoint resident on one CU (7 waves / SIMD; its
+// 18 KiB of LDS allow 8).
+constexpr uint32_t INTERIOR_WG_PER_CU = 7;
 // Longest list given a global bit-set slot (a slot is then 256 KiB x 4).
 constexpr uint32_t GMASK_MAX_GROUPS = 16384;
 struct InteriorWork {

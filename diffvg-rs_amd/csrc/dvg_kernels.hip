@@ -23,14 +23,14 @@ namespace dvgk {
 // Waves per SIMD the heavy kernels are compiled for.  The scene walks are
 // VALU-issue bound with memory waits between groups, so more resident waves
 // beat an unspilled register file: measured on the 1024x1024 / 1000-path
-// scene, 7 waves (<= 72 VGPRs) is best for the forward and the boundary pass
-// (4.5 -> 3.7 ms from 4 to 7, slower again at 8), 4 for the colour adjoint
-// (its LDS allows no more).
+// scene, 7 waves (<= 72 VGPRs) is best for the forward, the boundary pass
+// (4.5 -> 3.7 ms from 4 to 7, slower again at 8) and the colour adjoint
+// (1.64 -> 1.38 ms from 4 to 7, with its LDS cut to 18 KiB a workgroup).
 #ifndef OCC_FORWARD
 #define OCC_FORWARD 7
 #endif
 #ifndef OCC_INTERIOR
-#define OCC_INTERIOR 4
+#define OCC_INTERIOR 7
 #endif
 #ifndef OCC_BOUNDARY
 #define OCC_BOUNDARY 7
@@ -768,7 +768,7 @@ DVG_DEV uint32_t walk_window(const DevScene &S, const uint32_t *list, uint32_t n
 // stop), allocated by a prefix sum over the entries; an entry whose slot does
 // not fit in STAGE_FLOATS adds to global memory directly.  The slots are
 // flushed with one global atomic per non-zero float at the end of the tile.
-constexpr uint32_t STAGE_FLOATS = 4096;
+constexpr uint32_t STAGE_FLOATS = 2048;
 constexpr uint32_t STAGE_NONE = 0xffffffffu;
 
 template <bool LDS>
