@@ -195,8 +195,18 @@ DVG_DEV void winding_cubic(float px, float py, float x0, float y0, float x1, flo
 // below give bit-identical results while doing only the point-dependent part.
 // Record of WIND_STRIDE floats:
 //   cubic: a, b, c, q = (bb^2 - 3 cc) / 9, P = 2 bb^3 - 9 bb cc, q3 = q^3,
-//          sqrt(q3), sqrt(q), bb / 3, 3a, 2b   (bb = b / a, cc = c / a)
+//          sqrt(q3), sqrt(q), bb / 3, 3a, 2b   (bb = b / a, cc = c / a),
+//          root mask (below)
 //   quad:  ay, by, -, by^2, 4 ay, 2 ay
+//   all:   [12..15] = min y, max y, max x, min x of the control points (the
+//          ray test's box, as the walk would compute it)
+//
+// Root mask: in the three-root case root k is -2 sqrt(q) cos(phi_k) - bb/3
+// with phi_0 in [0, pi/3], phi_1 in [2pi/3, pi], phi_2 in [-2pi/3, -pi/3]
+// (theta = acos(.) in [0, pi]), so cos(phi_k) lies in [0.5, 1], [-1, -0.5],
+// [-0.5, 0.5].  Bit k is clear when that interval, widened by a rounding
+// margin, puts root k outside [0, 1]: the root would be discarded, so its
+// cosine need not be evaluated.  NaN terms keep every bit set.
 
 DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
     float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
@@ -211,10 +221,20 @@ DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
     o[5] = q3;
     o[6] = sqrtf(q3);
     o[7] = sqrtf(q);
-    o[8] = bb * (1.0f / 3.0f);
+    float off = bb * (1.0f / 3.0f), sq = o[7];
+    o[8] = off;
     o[9] = 3.0f * a;
     o[10] = 2.0f * b;
-    o[11] = 0.f;
+    // root k = -2 sq cos(phi_k) - off with cos(phi_k) in [clo_k, chi_k]
+    const float clo[3] = {0.4999990f, -1.0f, -0.5000010f}, chi[3] = {1.0f, -0.4999990f, 0.5000010f};
+    float m = 1.0e-5f * (2.0f * fabsf(sq) + fabsf(off)) + 1.0e-30f;
+    uint32_t mask = 0;
+    for (int k = 0; k < 3; ++k) {
+        float lo = -2.0f * sq * chi[k] - off, hi = -2.0f * sq * clo[k] - off;
+        bool out = (hi < -m) || (lo > 1.0f + m);
+        if (!out) mask |= 1u << k;
+    }
+    o[11] = (float)mask;
 }
 
 DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
@@ -224,7 +244,7 @@ DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
     o[3] = by * by;
     o[4] = 4.0f * ay;
     o[5] = 2.0f * ay;
-    for (int k = 6; k < WIND_STRIDE; ++k) o[k] = 0.f;
+    for (int k = 6; k < 12; ++k) o[k] = 0.f;
 }
 
 // winding_quad with the segment's terms from its record (same result).
@@ -275,13 +295,18 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
         float r2 = rr * rr, q3 = wc[5];
         float off = wc[8];
         if (r2 < q3) {
-            float theta = acosf(rr / wc[6]);
-            float sq = wc[7];
-            const float third = 1.0f / 3.0f;
-            const float two_pi = 2.0f * 3.14159265f;
-            r[0] = -2.0f * sq * cosf(theta * third) - off;
-            r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
-            r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+            // roots the segment's mask rules out keep -1 (discarded below)
+            uint32_t mask = (uint32_t)wc[11];
+            r[0] = r[1] = r[2] = -1.0f;
+            if (mask != 0u) {
+                float theta = acosf(rr / wc[6]);
+                float sq = wc[7];
+                const float third = 1.0f / 3.0f;
+                const float two_pi = 2.0f * 3.14159265f;
+                if (mask & 1u) r[0] = -2.0f * sq * cosf(theta * third) - off;
+                if (mask & 2u) r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
+                if (mask & 4u) r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+            }
             n = 3;
         } else {
             float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));

@@ -11,7 +11,7 @@ constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.r
 constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
 constexpr int FRAG_WINDOW = 8;      // fragments held per lane in the adjoint walk
 constexpr int LDS_GROUP_CAP = 256;  // tile groups whose paint grads are staged in LDS
-constexpr int WIND_STRIDE = 12;     // floats of a segment's winding-term record
+constexpr int WIND_STRIDE = 16;     // floats of a segment's winding-term record
 
 // Scene arrays are read through the constant address space on the device:
 // they do not change while a render or adjoint kernel runs, so loads at

@@ -252,10 +252,11 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
     float lmin = BIG, lw = 0.f, lc = 0.f;
     for (uint32_t s = 0; s < cnt; ++s) {
         const float *cv = S.curve_data + (off + s) * 13u;
+        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
         uint32_t k = (uint32_t)cv[0];
         float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
-        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)), bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
-        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)), by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+        // control-point box, from the segment's record
+        float bx0 = wc[15], bx1 = wc[14], by0 = wc[12], by1 = wc[13];
         if (prefilter) {
             // forward.rs:889-937: segments farther than the running minimum
             // cannot lower it.
@@ -271,7 +272,6 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         // forward.rs:938-1018: a +x ray only meets segments spanning py with
         // some control point right of px.
         if (py < by0 || py > by1 || px > bx1) continue;
-        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
         if (k == 0) winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
         else if (k == 1) winding_quad_w(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
         else winding_cubic_w(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
@@ -2676,8 +2676,13 @@ __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurv
     } else if (k == 1u) {
         wind_coef_quad(cv[2], cv[4], cv[6], o);
     } else {
-        for (int j = 0; j < WIND_STRIDE; ++j) o[j] = 0.f;
+        for (int j = 0; j < 12; ++j) o[j] = 0.f;
     }
+    float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+    o[12] = fminf(fminf(y0, y1), fminf(y2, y3));
+    o[13] = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+    o[14] = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+    o[15] = fminf(fminf(x0, x1), fminf(x2, x3));
 }
 
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
