@@ -35,6 +35,12 @@ namespace dvgk {
 #ifndef OCC_BOUNDARY
 #define OCC_BOUNDARY 7
 #endif
+// The prefiltered adjoint (closest-point adjoints, ~200 VGPRs unconstrained)
+// is latency bound at 2 waves; 3 measured best on the 1024x1024 prefiltered
+// scene (25.9 -> 20.4 ms; 4-6 spill enough to lose it again).
+#ifndef OCC_PREFILTER
+#define OCC_PREFILTER 3
+#endif
 
 // ---------------------------------------------------------------------------
 // Tile binning: group g belongs to tile t iff t lies in g's clamped tile range
@@ -2026,7 +2032,8 @@ DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t unit, const float *__r
     stage_flush(S, G, list, n, s_off, s_stage);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_backward_prefiltered(DevScene S, const float *__restrict__ weight,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_PREFILTER))) void k_backward_prefiltered(
+    DevScene S, const float *__restrict__ weight,
                                                                 const float *__restrict__ d_render, DevGrads G) {
     prefiltered_unit(S, blockIdx.x, weight, d_render, G);
 }
@@ -2948,7 +2955,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
     DVG_QUEUE_LOOP(interior_unit(B.S, w.unit, B.weight, B.d_render, B.G, slot, gcap))
 }
 
-__global__ __launch_bounds__(BLOCK) void k_batch_prefiltered(DVG_QUEUE_ARGS) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_PREFILTER))) void k_batch_prefiltered(
+    DVG_QUEUE_ARGS) {
     DVG_QUEUE_LOOP(prefiltered_unit(B.S, w.unit, B.weight, B.d_render, B.G))
 }
 
