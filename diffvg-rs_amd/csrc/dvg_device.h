@@ -99,6 +99,18 @@ DVG_DEV float cbrt_ref(float x) {
     return 0.f;
 }
 
+// The Cardano term of solve_cubic (curves.rs:938-941):
+//   rr > 0 ? -cbrt(rr + sqrt(r2 - q3)) : cbrt(-rr + sqrt(r2 - q3)).
+// Both arms take the cube root of |rr| + sqrt(r2 - q3) >= 0, so one powf
+// site serves every lane (a wave whose lanes split on the sign of rr would
+// otherwise run two expansions); the value is the same bit for bit.
+DVG_DEV float cardano_a(float rr, float r2, float q3) {
+    const float third = 1.0f / 3.0f;
+    float x = ((rr > 0.f) ? rr : -rr) + sqrtf(r2 - q3);
+    float c = (x > 0.f) ? powf(x, third) : 0.f;
+    return (rr > 0.f) ? -c : c;
+}
+
 // curves.rs:897-946
 DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
     const float eps = 1.0e-6f;
@@ -120,7 +132,7 @@ DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
         r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
         return 3;
     }
-    float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));
+    float ar = cardano_a(rr, r2, q3);
     float br = (absf_(ar) > eps) ? q / ar : 0.f;
     r[0] = (ar + br) - bb * third;
     return 1;
@@ -309,7 +321,7 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
             }
             n = 3;
         } else {
-            float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));
+            float ar = cardano_a(rr, r2, q3);
             float br = (absf_(ar) > eps) ? q / ar : 0.f;
             r[0] = (ar + br) - off;
             n = 1;
