@@ -710,7 +710,7 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
     }
     if (S.n_cdf > 0) {
         size_t ns = npix * S.spp, nt = dvgk::boundary_buckets(S);
-        size_t need = 2 * ns + 3 * nt + 3;
+        size_t need = 3 * ns + 2 * nt + 3;
         if (ctx->bwork_cap < need) {
             if (ctx->bwork) (void)hipFree(ctx->bwork);
             ctx->bwork = nullptr;
@@ -720,10 +720,10 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         }
         dvgk::BoundaryWork bw;
         bw.keys = ctx->bwork;
-        bw.order = bw.keys + ns;
+        bw.ranks = bw.keys + ns;
+        bw.order = bw.ranks + ns;
         bw.counts = bw.order + ns;
         bw.offsets = bw.counts + nt + 1;
-        bw.cursor = bw.offsets + nt + 2;
         {
             KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_SORT);
             dvgk::launch_boundary_sort(S, bw, ctx->stream);

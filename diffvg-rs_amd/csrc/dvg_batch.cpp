@@ -141,7 +141,6 @@ int upload_config(dvg_ctx *ctx, dvg_batch *b) {
         B.S.unit_samples = (uint32_t)std::min<uint64_t>(us, tile_samples);
         B.nb = dvgk::boundary_buckets(B.S);
         B.zn[15] = B.counts ? B.nb + 1 : 0;
-        B.zn[16] = B.cursor ? B.nb : 0;
     }
     int st = build_items(ctx, b);
     if (st) return st;
@@ -224,7 +223,7 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
     // one allocation for every work buffer; buckets sized for the finest
     // boundary blocks the autotuner may pick (2x2 pixels: 64 per tile)
     struct Lay {
-        size_t weight, accum, img, dr, grads, keys, order, counts, offsets, cursor;
+        size_t weight, accum, img, dr, grads, keys, ranks, order, counts, offsets;
         size_t grad_off[14];
         uint32_t nbmax;
     };
@@ -252,10 +251,10 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
         bool bnd = S.n_cdf > 0 && !S.prefilter;
         L.nbmax = S.tiles_x * S.tiles_y * 64u;
         L.keys = bnd ? take(ns * 4) : 0;
+        L.ranks = bnd ? take(ns * 4) : 0;
         L.order = bnd ? take(ns * 4) : 0;
         L.counts = bnd ? take(((size_t)L.nbmax + 1) * 4) : 0;
         L.offsets = bnd ? take(((size_t)L.nbmax + 2) * 4) : 0;
-        L.cursor = bnd ? take((size_t)L.nbmax * 4) : 0;
     }
     hipError_t e = hipMalloc(&b->work, std::max(total, (size_t)256));
     if (e != hipSuccess) return fail(hip_err(ctx, e, "hipMalloc(batch work)"));
@@ -281,10 +280,10 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
         bool bnd = S.n_cdf > 0 && !S.prefilter;
         if (bnd) {
             B.keys = (uint32_t *)(base + L.keys);
+            B.ranks = (uint32_t *)(base + L.ranks);
             B.order = (uint32_t *)(base + L.order);
             B.counts = (uint32_t *)(base + L.counts);
             B.offsets = (uint32_t *)(base + L.offsets);
-            B.cursor = (uint32_t *)(base + L.cursor);
         }
         for (int k = 0; k < 14; ++k) {
             B.zp[k] = gp[k];
@@ -293,7 +292,6 @@ int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, 
         B.zp[14] = B.accum;
         B.zn[14] = (uint32_t)((size_t)S.W * S.H * 4);
         B.zp[15] = (float *)B.counts;
-        B.zp[16] = (float *)B.cursor;
         // tile list lengths, for the heaviest-first order of tile work
         tile_off[i].resize((size_t)S.tiles_x * S.tiles_y + 1);
         e = hipMemcpyAsync(tile_off[i].data(), S.tile_offsets, tile_off[i].size() * 4, hipMemcpyDeviceToHost,

@@ -127,10 +127,10 @@ __host__ __device__ inline uint32_t boundary_buckets(const DevScene &S) {
     uint32_t per_row = (uint32_t)TILE >> S.bnd_log2;
     return S.tiles_x * S.tiles_y * per_row * per_row;
 }
-// Work buffers of the boundary pass: keys/order hold one u32 per sample,
-// counts buckets+1, offsets buckets+2, cursor buckets.
+// Work buffers of the boundary pass: keys/ranks/order hold one u32 per
+// sample, counts buckets+1, offsets buckets+2.
 struct BoundaryWork {
-    uint32_t *keys, *order, *counts, *offsets, *cursor;
+    uint32_t *keys, *ranks, *order, *counts, *offsets;
 };
 // Boundary samples: tile keys, counting sort; then the tile-ordered evaluation.
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream);
@@ -162,12 +162,12 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
 
 // ---- batches: many scenes per launch ---------------------------------------
 // One scene of a batch: its device scene and its work buffers.
-constexpr int BATCH_ZERO_ARRAYS = 17;  // 14 gradient arrays, accum, counts, cursor
+constexpr int BATCH_ZERO_ARRAYS = 16;  // 14 gradient arrays, accum, counts
 struct BatchScene {
     DevScene S;
     float *weight, *accum, *img, *target, *d_render;
     DevGrads G;
-    uint32_t *keys, *order, *counts, *offsets, *cursor;
+    uint32_t *keys, *ranks, *order, *counts, *offsets;
     uint32_t nb;  // boundary buckets
     float *zp[BATCH_ZERO_ARRAYS];
     uint32_t zn[BATCH_ZERO_ARRAYS];

@@ -2389,9 +2389,10 @@ DVG_DEV uint32_t bucket_of(const DevScene &S, uint32_t x, uint32_t y) {
 }
 
 // Pass 1: bucket key of every boundary sample (nb when it contributes
-// nothing) and the histogram of the contributing ones.
-DVG_DEV void bnd_key_unit(const DevScene &S, uint32_t u, uint32_t *__restrict__ keys, uint32_t *__restrict__ counts,
-                          uint32_t nb) {
+// nothing), the histogram of the contributing ones, and each sample's rank
+// in its bucket (the histogram atomic's return value).
+DVG_DEV void bnd_key_unit(const DevScene &S, uint32_t u, uint32_t *__restrict__ keys, uint32_t *__restrict__ ranks,
+                          uint32_t *__restrict__ counts, uint32_t nb) {
     size_t idx = (size_t)u * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
@@ -2401,32 +2402,33 @@ DVG_DEV void bnd_key_unit(const DevScene &S, uint32_t u, uint32_t *__restrict__ 
     keys[idx] = key;
     // samples without a contribution are not counted: a shared counter for
     // them would serialise every such atomic on one address
-    if (key < nb) atomicAdd(counts + key, 1u);
+    if (key < nb) ranks[idx] = atomicAdd(counts + key, 1u);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bnd_key(DevScene S, uint32_t *__restrict__ keys,
-                                                   uint32_t *__restrict__ counts, uint32_t nb) {
-    bnd_key_unit(S, blockIdx.x, keys, counts, nb);
+                                                   uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts,
+                                                   uint32_t nb) {
+    bnd_key_unit(S, blockIdx.x, keys, ranks, counts, nb);
 }
 
-// Pass 2: counting-sort scatter of sample indices by bucket.
+// Pass 2: counting-sort scatter of sample indices by bucket (bucket offset
+// plus the rank from pass 1: no atomics).
 DVG_DEV void bnd_scatter_unit(const DevScene &S, uint32_t u, const uint32_t *__restrict__ keys,
-                              const uint32_t *__restrict__ offsets, uint32_t *__restrict__ cursor,
+                              const uint32_t *__restrict__ ranks, const uint32_t *__restrict__ offsets,
                               uint32_t *__restrict__ order, uint32_t nb) {
     size_t idx = (size_t)u * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
     uint32_t key = keys[idx];
     if (key >= nb) return;
-    uint32_t pos = offsets[key] + atomicAdd(cursor + key, 1u);
-    order[pos] = (uint32_t)idx;
+    order[offsets[key] + ranks[idx]] = (uint32_t)idx;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bnd_scatter(DevScene S, const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ ranks,
                                                        const uint32_t *__restrict__ offsets,
-                                                       uint32_t *__restrict__ cursor, uint32_t *__restrict__ order,
-                                                       uint32_t nb) {
-    bnd_scatter_unit(S, blockIdx.x, keys, offsets, cursor, order, nb);
+                                                       uint32_t *__restrict__ order, uint32_t nb) {
+    bnd_scatter_unit(S, blockIdx.x, keys, ranks, offsets, order, nb);
 }
 
 DVG_DEV float sel3(uint32_t i, float a, float b, float c) { return i == 0u ? a : (i == 1u ? b : c); }
@@ -2961,7 +2963,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_PREFI
 }
 
 __global__ __launch_bounds__(BLOCK) void k_batch_bnd_key(DVG_QUEUE_ARGS) {
-    DVG_QUEUE_LOOP(bnd_key_unit(B.S, w.unit, B.keys, B.counts, B.nb))
+    DVG_QUEUE_LOOP(bnd_key_unit(B.S, w.unit, B.keys, B.ranks, B.counts, B.nb))
 }
 
 __global__ __launch_bounds__(BLOCK) void k_batch_scan(DVG_QUEUE_ARGS) {
@@ -2969,7 +2971,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_scan(DVG_QUEUE_ARGS) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_batch_bnd_scatter(DVG_QUEUE_ARGS) {
-    DVG_QUEUE_LOOP(bnd_scatter_unit(B.S, w.unit, B.keys, B.offsets, B.cursor, B.order, B.nb))
+    DVG_QUEUE_LOOP(bnd_scatter_unit(B.S, w.unit, B.keys, B.ranks, B.offsets, B.order, B.nb))
 }
 
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_batch_bnd_eval(
@@ -3047,13 +3049,13 @@ void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     uint32_t nbk = boundary_buckets(S);
     uint32_t grid = cdiv(total, BLOCK);
-    // counts[nbk + 1] and cursor[nbk] start at zero
-    float *zp[2] = {reinterpret_cast<float *>(bw.counts), reinterpret_cast<float *>(bw.cursor)};
-    uint32_t zn[2] = {nbk + 1, nbk};
-    launch_zero(zp, zn, 2, stream);
-    hipLaunchKernelGGL(k_bnd_key, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.counts, nbk);
+    // counts[nbk + 1] start at zero
+    float *zp[1] = {reinterpret_cast<float *>(bw.counts)};
+    uint32_t zn[1] = {nbk + 1};
+    launch_zero(zp, zn, 1, stream);
+    hipLaunchKernelGGL(k_bnd_key, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.ranks, bw.counts, nbk);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, nbk + 1, bw.offsets);
-    hipLaunchKernelGGL(k_bnd_scatter, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.offsets, bw.cursor, bw.order,
+    hipLaunchKernelGGL(k_bnd_scatter, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.ranks, bw.offsets, bw.order,
                        nbk);
 }
 
