@@ -2058,6 +2058,50 @@ DVG_DEV uint32_t cdf_search(const float *cdf, uint32_t n, float u, float &t_out)
     return idx;
 }
 
+// The shape CDF's block ends in LDS, for a two-level search: block
+// j = [j B, min((j + 1) B, n)) of the table ends at piv[j], so the first block
+// whose end is >= u holds the table's first entry >= u, and a search inside
+// that block finds the same index as the search over the whole table.
+constexpr uint32_t CDF_PIVOTS = 1024;
+struct CdfPivots {
+    float piv[CDF_PIVOTS];
+    uint32_t nblk, bsz;
+};
+
+// All threads of the workgroup call it (it synchronizes).
+DVG_DEV void stage_cdf_pivots(const DevScene &S, CdfPivots *cp) {
+    uint32_t n = S.n_cdf;
+    uint32_t bsz = n > CDF_PIVOTS ? (n + CDF_PIVOTS - 1) / CDF_PIVOTS : 1u;
+    uint32_t nblk = n > 0 ? (n + bsz - 1) / bsz : 0u;
+    for (uint32_t j = threadIdx.x; j < nblk; j += BLOCK) {
+        uint32_t end = min((j + 1u) * bsz, n) - 1u;
+        cp->piv[j] = S.shape_cdf[end];
+    }
+    if (threadIdx.x == 0) { cp->nblk = nblk; cp->bsz = bsz; }
+    __syncthreads();
+}
+
+// cdf_search over the shape CDF through its staged block ends.
+DVG_DEV uint32_t shape_cdf_search(const DevScene &S, const CdfPivots *cp, float u) {
+    uint32_t nblk = cp->nblk, bsz = cp->bsz, n = S.n_cdf;
+    uint32_t lo = 0, hi = nblk;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u > cp->piv[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= nblk) return n - 1;  // no entry >= u: the last one, as cdf_search
+    uint32_t b0 = lo * bsz, b1 = min(b0 + bsz, n);
+    lo = b0;
+    hi = b1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (u > S.shape_cdf[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 DVG_DEV float2 normalize2(float x, float y) {
     float len = sqrtf(x * x + y * y);
     if (len == 0.f) return make_float2(0.f, 0.f);
@@ -2274,13 +2318,13 @@ struct BPoint {
     BData bd;
 };
 
-DVG_DEV bool boundary_point(const DevScene &S, size_t idx, BPoint &P) {
+DVG_DEV bool boundary_point(const DevScene &S, const CdfPivots *cp, size_t idx, BPoint &P) {
     const float PI = 3.14159274101257324219f;
     Pcg32 rng((uint64_t)idx, (uint64_t)S.seed);
     float u = rng.next_f32();
     float t = rng.next_f32();
     float unused;
-    uint32_t sid = cdf_search(S.shape_cdf, S.n_cdf, u, unused);
+    uint32_t sid = cp ? shape_cdf_search(S, cp, u) : cdf_search(S.shape_cdf, S.n_cdf, u, unused);
     uint32_t si = S.shape_ids[sid], gi = S.group_ids[sid];
     float spmf = S.shape_pmf[sid];
     if (spmf <= 0.f) return false;
@@ -2393,12 +2437,14 @@ DVG_DEV uint32_t bucket_of(const DevScene &S, uint32_t x, uint32_t y) {
 // in its bucket (the histogram atomic's return value).
 DVG_DEV void bnd_key_unit(const DevScene &S, uint32_t u, uint32_t *__restrict__ keys, uint32_t *__restrict__ ranks,
                           uint32_t *__restrict__ counts, uint32_t nb) {
+    __shared__ CdfPivots s_cdf;
+    stage_cdf_pivots(S, &s_cdf);
     size_t idx = (size_t)u * BLOCK + threadIdx.x;
     size_t total = (size_t)S.W * S.H * S.spp;
     if (idx >= total) return;
     BPoint P;
     uint32_t key = nb;
-    if (S.n_cdf > 0 && boundary_point(S, idx, P)) key = bucket_of(S, P.pxi, P.pyi);
+    if (S.n_cdf > 0 && boundary_point(S, &s_cdf, idx, P)) key = bucket_of(S, P.pxi, P.pyi);
     keys[idx] = key;
     // samples without a contribution are not counted: a shared counter for
     // them would serialise every such atomic on one address
@@ -2449,6 +2495,7 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
                            const float *__restrict__ d_render, const DevGrads &G, const uint32_t *__restrict__ offsets,
                            const uint32_t *__restrict__ order, uint32_t nb) {
     const float PI = 3.14159274101257324219f;
+    // (no staged CDF here: its LDS would cost a workgroup per CU)
     __shared__ BPoint s_pt[BLOCK];
     uint32_t n_valid = offsets[nb];
     uint32_t pos = u * BLOCK + threadIdx.x;
@@ -2460,7 +2507,7 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
     size_t pix = 0;
     if (pos < n_valid) {
         size_t idx = order[pos];
-        valid = boundary_point(S, idx, P);
+        valid = boundary_point(S, nullptr, idx, P);
         if (valid) {
             pix = (size_t)P.pyi * S.W + P.pxi;
             float4 bgs = background_at(S, P.pxi, P.pyi);
