@@ -101,7 +101,11 @@ struct InteriorWork {
     uint32_t *gmask, *counter;
     uint32_t gcap, slots;
 };
-inline size_t interior_slot_words(uint32_t gcap) { return (size_t)((2u * gcap + 31u) / 32u) * BLOCK; }
+// A slot: the bit set (nw words a lane) then the adjoint's checkpoint ring
+// (40 words a lane), both word-major over the BLOCK lanes.
+__host__ __device__ inline size_t interior_slot_words(uint32_t gcap) {
+    return ((size_t)((2u * gcap + 31u) / 32u) + 40u) * BLOCK;
+}
 // List capacity of the global bit-set slots for a scene whose longest tile
 // list has `list_max` entries (0: the LDS bit set holds every list).  The
 // margin leaves room for lists that grow under device-side updates.

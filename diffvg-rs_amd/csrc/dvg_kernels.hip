@@ -1000,8 +1000,23 @@ constexpr int MASK_WORDS = FRAG_WINDOW;
 constexpr uint32_t MASK_GROUPS = 16u * MASK_WORDS;
 static_assert(MASK_GROUPS == LDS_MASK_GROUPS, "LDS bit set size");
 
+// Fragments are taken back in blocks of ADJ_BLOCK (mask_adjoint).  With a
+// checkpoint ring (global bit-set slots), the walk records for the last
+// ADJ_RING blocks the composite before the block's first fragment and that
+// fragment's bit, so the reverse pass starts those blocks without
+// recompositing everything below them.
+constexpr uint32_t ADJ_BLOCK = 32;
+constexpr uint32_t ADJ_RING = 8;
+static_assert(5 * ADJ_RING == 40, "ring words of interior_slot_words");
+
+DVG_DEV void ring_put(float *ring, uint32_t blk, float4 c, uint32_t bit) {
+    float *r = ring + (size_t)(blk % ADJ_RING) * 5u * BLOCK;
+    r[0] = c.x; r[BLOCK] = c.y; r[2 * BLOCK] = c.z; r[3 * BLOCK] = c.w;
+    r[4 * BLOCK] = __uint_as_float(bit);
+}
+
 DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, uint32_t *mask,
-                           float4 &fin) {
+                           float4 &fin, float *ring) {
     uint32_t nw = (2u * n + 31u) / 32u;
     for (uint32_t w = 0; w < nw; ++w) mask[w * BLOCK] = 0u;
     uint32_t count = 0;
@@ -1013,8 +1028,16 @@ DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, 
         uint32_t bits = (st ? 1u : 0u) | (fi ? 2u : 0u);
         uint32_t bit = 2u * e;
         mask[(bit >> 5) * BLOCK] |= bits << (bit & 31u);
-        if (st) { blend_cpu(fin, frag_color(S, g, true, px, py)); ++count; }
-        if (fi) { blend_cpu(fin, frag_color(S, g, false, px, py)); ++count; }
+        if (st) {
+            if (ring && count % ADJ_BLOCK == 0u) ring_put(ring, count / ADJ_BLOCK, fin, bit);
+            blend_cpu(fin, frag_color(S, g, true, px, py));
+            ++count;
+        }
+        if (fi) {
+            if (ring && count % ADJ_BLOCK == 0u) ring_put(ring, count / ADJ_BLOCK, fin, bit + 1u);
+            blend_cpu(fin, frag_color(S, g, false, px, py));
+            ++count;
+        }
     }
     return count;
 }
@@ -1032,7 +1055,6 @@ DVG_DEV uint32_t walk_mask(const DevScene &S, const uint32_t *list, uint32_t n, 
 // zero every fragment below adds exactly zero (finite paints), so the walk
 // down stops there -- under a deep stack of translucent fragments the f32
 // adjoint reaches zero after a few hundred fragments.
-constexpr uint32_t ADJ_BLOCK = 32;
 
 DVG_DEV bool adjoint_zero(float dr, float dg, float db, float da) {
     return dr == 0.f && dg == 0.f && db == 0.f && da == 0.f;
@@ -1049,8 +1071,9 @@ DVG_DEV uint32_t next_bit(const uint32_t *mask, uint32_t nw, uint32_t &w, uint32
 }
 
 DVG_DEV void mask_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, const uint32_t *s_off,
-                          const uint32_t *list, uint32_t n, const uint32_t *mask, uint32_t nf, float4 bg, float px,
-                          float py, float &dr, float &dg, float &db, float &da, float &dtx, float &dty) {
+                          const uint32_t *list, uint32_t n, const uint32_t *mask, const float *ring, uint32_t nf,
+                          float4 bg, float px, float py, float &dr, float &dg, float &db, float &da, float &dtx,
+                          float &dty) {
     uint32_t nw = (2u * n + 31u) / 32u;
     if (nf == 0u || nw == 0u) return;
     uint32_t nblk = (nf + ADJ_BLOCK - 1u) / ADJ_BLOCK;
@@ -1061,11 +1084,21 @@ DVG_DEV void mask_adjoint(const DevScene &S, const DevGrads &G, float *s_stage, 
         // state at the block's first fragment
         float4 pre = bg;
         uint32_t w0 = 0u, word0 = mask[0];
-        // (the bit set holds exactly nf bits: next_bit never runs out below)
-        for (uint32_t k = 0; k < first; ++k) {
-            uint32_t bit = next_bit(mask, nw, w0, word0);
-            if (bit >= 2u * n) return;
-            blend_cpu(pre, frag_color(S, list[bit >> 1], (bit & 1u) == 0u, px, py));
+        if (jb > 0 && ring && (uint32_t)jb + ADJ_RING >= nblk) {
+            // recorded by the walk
+            const float *r = ring + (size_t)((uint32_t)jb % ADJ_RING) * 5u * BLOCK;
+            pre = make_float4(r[0], r[BLOCK], r[2 * BLOCK], r[3 * BLOCK]);
+            uint32_t b0 = __float_as_uint(r[4 * BLOCK]);
+            if (b0 >= 2u * n) return;
+            w0 = b0 >> 5;
+            word0 = mask[w0 * BLOCK] & (0xffffffffu << (b0 & 31u));
+        } else {
+            // (the bit set holds exactly nf bits: next_bit never runs out below)
+            for (uint32_t k = 0; k < first; ++k) {
+                uint32_t bit = next_bit(mask, nw, w0, word0);
+                if (bit >= 2u * n) return;
+                blend_cpu(pre, frag_color(S, list[bit >> 1], (bit & 1u) == 0u, px, py));
+            }
         }
         for (int i = (int)cnt - 1; i >= 0; --i) {
             float4 pc = pre;
@@ -1108,6 +1141,9 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
     bool use_mask = n <= MASK_GROUPS;
     bool use_gmask = !use_mask && gmask != nullptr && n <= gcap;
     uint32_t *gids = use_gmask ? gmask + threadIdx.x : nullptr;
+    // the lane's checkpoint ring follows the slot's bit-set words
+    float *gring = use_gmask ? reinterpret_cast<float *>(gmask + (size_t)((2u * gcap + 31u) / 32u) * BLOCK) + threadIdx.x
+                             : nullptr;
     stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
     uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
@@ -1126,8 +1162,8 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
         float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
         float4 dcol = gather_d_color(S, weight, d_render, px0, py0);
         float4 pre = bg, fin = bg;
-        uint32_t nf = use_mask    ? walk_mask(S, list, n, px, py, ids, fin)
-                      : use_gmask ? walk_mask(S, list, n, px, py, gids, fin)
+        uint32_t nf = use_mask    ? walk_mask(S, list, n, px, py, ids, fin, nullptr)
+                      : use_gmask ? walk_mask(S, list, n, px, py, gids, fin, gring)
                                   : walk_window(S, list, n, px, py, 0, ids, pre, fin);
         float4 col;
         float dtx = 0.f, dty = 0.f;
@@ -1155,9 +1191,9 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
                 da -= dot / fa;
             }
             if (use_mask) {
-                mask_adjoint(S, G, s_stage, s_off, list, n, ids, nf, bg, px, py, dr, dg, db, da, dtx, dty);
+                mask_adjoint(S, G, s_stage, s_off, list, n, ids, nullptr, nf, bg, px, py, dr, dg, db, da, dtx, dty);
             } else if (use_gmask) {
-                mask_adjoint(S, G, s_stage, s_off, list, n, gids, nf, bg, px, py, dr, dg, db, da, dtx, dty);
+                mask_adjoint(S, G, s_stage, s_off, list, n, gids, gring, nf, bg, px, py, dr, dg, db, da, dtx, dty);
             } else {
                 uint32_t nwin = (nf + FRAG_WINDOW - 1) / FRAG_WINDOW;
                 for (int w = (int)nwin - 1; w >= 0; --w) {
@@ -1228,8 +1264,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTER
     DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
     uint32_t *__restrict__ counter, uint32_t n_units, uint32_t *__restrict__ gmask, uint32_t gcap) {
     __shared__ uint32_t s_unit;
-    uint32_t nw = (2u * gcap + 31u) / 32u;
-    uint32_t *slot = gmask + (size_t)blockIdx.x * nw * BLOCK;
+    uint32_t *slot = gmask + (size_t)blockIdx.x * interior_slot_words(gcap);
     for (;;) {
         if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1u);
         __syncthreads();
@@ -2999,8 +3034,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_resolve(DVG_QUEUE_ARGS, int los
 
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_INTERIOR))) void k_batch_interior(
     DVG_QUEUE_ARGS, uint32_t *__restrict__ gmask, uint32_t gcap) {
-    uint32_t nw = (2u * gcap + 31u) / 32u;
-    uint32_t *slot = gmask ? gmask + (size_t)blockIdx.x * nw * BLOCK : nullptr;
+    uint32_t *slot = gmask ? gmask + (size_t)blockIdx.x * interior_slot_words(gcap) : nullptr;
     DVG_QUEUE_LOOP(interior_unit(B.S, w.unit, B.weight, B.d_render, B.G, slot, gcap))
 }
 
