@@ -159,17 +159,26 @@ DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
     float q = (bb * bb - 3.0f * cc) / 9.0f;
     float rr = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
     float r2 = rr * rr, q3 = q * q * q;
+    // fast transcendentals when the root formulas do not amplify their errors
+    bool fast = absf_(bb * third) < WIND_FAST_COND && q < WIND_FAST_COND * WIND_FAST_COND;
     if (r2 < q3) {
-        float theta = acosf(rr / sqrtf(q3));
         float sq = sqrtf(q);
         const float two_pi = 2.0f * 3.14159265f;
         float off = bb * third;
-        r[0] = -2.0f * sq * cosf(theta * third) - off;
-        r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
-        r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+        if (fast) {
+            float theta = fast_acos(rr / sqrtf(q3));
+            r[0] = -2.0f * sq * __cosf(theta * third) - off;
+            r[1] = -2.0f * sq * __cosf((theta + two_pi) * third) - off;
+            r[2] = -2.0f * sq * __cosf((theta - two_pi) * third) - off;
+        } else {
+            float theta = acosf(rr / sqrtf(q3));
+            r[0] = -2.0f * sq * cosf(theta * third) - off;
+            r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
+            r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+        }
         return 3;
     }
-    float ar = cardano_a(rr, r2, q3);
+    float ar = fast ? cardano_a_fast(rr, r2, q3) : cardano_a(rr, r2, q3);
     float br = (absf_(ar) > eps) ? q / ar : 0.f;
     r[0] = (ar + br) - bb * third;
     return 1;
