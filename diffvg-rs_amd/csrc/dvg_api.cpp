@@ -47,6 +47,20 @@ extern "C" {
 
 const char *dvg_version(void) { return DVG_VERSION_STRING; }
 
+int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *mismatches) {
+    if (!ctx || !mismatches) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    uint32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, sizeof(uint32_t)));
+    dvgk::launch_selftest_winding(n, seed, d, ctx->stream);
+    hipError_t e = hipMemcpyAsync(mismatches, d, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_err(ctx, e, "winding self-test");
+    if (n == 0) *mismatches = 0;
+    return DVG_OK;
+}
+
 dvg_ctx *dvg_ctx_create(int device_ordinal) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) return nullptr;

@@ -158,6 +158,9 @@ struct UpdatePlan {
 // Rewrites the packed geometry of S from path_points (scene layout; NULL =
 // unchanged) and group colours (8 per group: fill then stroke RGBA; NULL =
 // unchanged), rebuilds tile lists and boundary tables.  Asynchronous.
+// Counts into *mismatches (device) the random cubic segment / point pairs whose
+// winding from the per-segment record differs from the libm formula.
+void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hipStream_t stream);
 // Per-segment winding terms of curve_data (13 floats a segment) into
 // curve_wind; after every change of curve_data.
 void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream);

@@ -2755,6 +2755,82 @@ __global__ void k_copy_f32(float *__restrict__ dst, const float *__restrict__ sr
 }
 
 // Control points of every curve from the path points (path.rs:100-173).
+// ---- self-test of the winding solve --------------------------------------
+// The cubic winding test with libm's acos / cos / pow throughout (the
+// formula of curves.rs:170-214 and curves.rs:897-946 as written), the
+// reference the per-segment record and its fast transcendentals are checked
+// against.
+DVG_DEV void winding_cubic_libm(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                                float x3, float y3, float &w) {
+    float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
+    float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
+    float c = -3.0f * y0 + 3.0f * y1;
+    float d = y0 - py;
+    float r[3];
+    int n;
+    const float eps = 1.0e-6f;
+    if (absf_(a) < eps) {
+        n = solve_quadratic(b, c, d, r[0], r[1]);
+    } else {
+        float bb = b / a, cc = c / a, dd = d / a;
+        const float third = 1.0f / 3.0f;
+        float q = (bb * bb - 3.0f * cc) / 9.0f;
+        float rr = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
+        float r2 = rr * rr, q3 = q * q * q;
+        if (r2 < q3) {
+            float theta = acosf(rr / sqrtf(q3));
+            float sq = sqrtf(q);
+            const float two_pi = 2.0f * 3.14159265f;
+            float off = bb * third;
+            r[0] = -2.0f * sq * cosf(theta * third) - off;
+            r[1] = -2.0f * sq * cosf((theta + two_pi) * third) - off;
+            r[2] = -2.0f * sq * cosf((theta - two_pi) * third) - off;
+            n = 3;
+        } else {
+            float ar = (rr > 0.f) ? -cbrt_ref(rr + sqrtf(r2 - q3)) : cbrt_ref(-rr + sqrtf(r2 - q3));
+            float br = (absf_(ar) > eps) ? q / ar : 0.f;
+            r[0] = (ar + br) - bb * third;
+            n = 1;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        float t = r[i];
+        if (t >= 0.f && t <= 1.f) {
+            float tt = 1.f - t;
+            float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+            if (x > px) {
+                float dy = 3.0f * a * t * t + 2.0f * b * t + c;
+                w += (dy > 0.f) ? 1.f : -1.f;
+            }
+        }
+    }
+}
+
+// Random cubic segments (control points within 64 px of a centre in a
+// 128 px square) and points at a random height inside each segment's
+// y-range: the cases the walk sends to the solver.  Counts the pairs whose
+// winding differs between the record path and the libm formula.
+__global__ void k_selftest_winding(uint32_t n, uint32_t seed, uint32_t *__restrict__ mismatches) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Pcg32 rng((uint64_t)i, (uint64_t)seed);
+    float cx = 128.0f * rng.next_f32(), cy = 128.0f * rng.next_f32();
+    float x[4], y[4];
+    for (int k = 0; k < 4; ++k) {
+        x[k] = cx + 128.0f * rng.next_f32() - 64.0f;
+        y[k] = cy + 128.0f * rng.next_f32() - 64.0f;
+    }
+    float ymin = fminf(fminf(y[0], y[1]), fminf(y[2], y[3])), ymax = fmaxf(fmaxf(y[0], y[1]), fmaxf(y[2], y[3]));
+    float px = cx + 128.0f * rng.next_f32() - 64.0f;
+    float py = ymin + (ymax - ymin) * rng.next_f32();
+    float rec[WIND_STRIDE];
+    wind_coef_cubic(y[0], y[1], y[2], y[3], rec);
+    float w0 = 0.f, w1 = 0.f, cr = 0.f;
+    winding_cubic_libm(px, py, x[0], y[0], x[1], y[1], x[2], y[2], x[3], y[3], w0);
+    winding_cubic_w(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w1, cr);
+    if (w0 != w1) atomicAdd(mismatches, 1u);
+}
+
 // Winding terms of every segment (the record described with WIND_STRIDE).
 __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurves, float *__restrict__ out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3183,6 +3259,12 @@ void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *ite
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, stream, p, n, v);
+}
+
+void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, mismatches, 1u, 0u);
+    hipLaunchKernelGGL(k_selftest_winding, dim3(cdiv(n, 256)), dim3(256), 0, stream, n, seed, mismatches);
 }
 
 void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream) {

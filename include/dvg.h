@@ -474,6 +474,15 @@ int32_t dvg_batch_set_config(dvg_ctx *ctx, dvg_batch *batch, const dvg_batch_con
  * coordinate search) and keeps the fastest; best may be NULL. */
 int32_t dvg_batch_autotune(dvg_ctx *ctx, dvg_batch *batch, uint32_t flags, int reps, dvg_batch_config *best);
 
+/* ---- diagnostics (no counterpart in the reference) ------------------------ */
+
+/* Self-test of the cubic winding solve: n random segment / point pairs (PCG32
+ * stream `seed`, points inside each segment's y-range) are evaluated through
+ * the per-segment record with its fast transcendentals and through the
+ * curves.rs formula with libm's acos / cos / pow; *mismatches receives the
+ * number of pairs whose winding differs.  Synchronous. */
+int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *mismatches);
+
 /* Version string of the library build. */
 const char *dvg_version(void);
 
