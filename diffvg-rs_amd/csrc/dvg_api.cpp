@@ -61,6 +61,19 @@ int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *
     return DVG_OK;
 }
 
+int32_t dvg_selftest_inside(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *counts) {
+    if (!ctx || !counts) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    uint32_t *d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, 8 * sizeof(uint32_t)));
+    dvgk::launch_selftest_inside(n, seed, d, ctx->stream);
+    hipError_t e = hipMemcpyAsync(counts, d, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_err(ctx, e, "inside-test self-test");
+    return DVG_OK;
+}
+
 dvg_ctx *dvg_ctx_create(int device_ordinal) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) return nullptr;

@@ -62,7 +62,7 @@ int build_items(dvg_ctx *ctx, dvg_batch *b) {
         uint64_t cost;
         WorkItem w;
     };
-    std::vector<Costed> fwd_units, bwd_units, pre_units;
+    std::vector<Costed> fwd_units, fwd_pf_units, bwd_units, pre_units;
     for (uint32_t i = 0; i < b->n; ++i) {
         const BatchScene &B = b->bs[i];
         const dvgk::DevScene &S = B.S;
@@ -87,7 +87,7 @@ int build_items(dvg_ctx *ctx, dvg_batch *b) {
             for (uint32_t k = 0; k < upt; ++k) {
                 uint32_t ns_unit = std::min(S.unit_samples, tile_samples - k * S.unit_samples);
                 Costed c = {len * ns_unit, {i, t * upt + k}};
-                fwd_units.push_back(c);
+                (S.prefilter ? fwd_pf_units : fwd_units).push_back(c);
                 (S.prefilter ? pre_units : bwd_units).push_back(c);
             }
         }
@@ -106,6 +106,7 @@ int build_items(dvg_ctx *ctx, dvg_batch *b) {
         for (const Costed &c : v) outv.push_back(c.w);
     };
     heavy_first(fwd_units, b->items[dvgk::PH_FORWARD]);
+    heavy_first(fwd_pf_units, b->items[dvgk::PH_FORWARD_PF]);
     heavy_first(bwd_units, b->items[dvgk::PH_INTERIOR]);
     heavy_first(pre_units, b->items[dvgk::PH_PREFILTERED]);
     size_t n_items = 0;
@@ -171,6 +172,7 @@ void launch_step(dvg_ctx *ctx, dvg_batch *b, uint32_t flags) {
     if (fwd) {
         run(dvgk::PH_WEIGHTS_FWD);
         run(dvgk::PH_FORWARD);
+        run(dvgk::PH_FORWARD_PF);
         run(dvgk::PH_RESOLVE, loss);
     }
     if (bwd) {

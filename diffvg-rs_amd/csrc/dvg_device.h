@@ -114,6 +114,16 @@ DVG_DEV float cbrt_ref(float x) {
 // cos, which the WGSL specification bounds only loosely (cos: 2^-11
 // absolute).
 constexpr float WIND_FAST_COND = 16.0f;
+// Segments whose sqrt|q| or |b / 3a| reach WIND_PIECE_COND (record bit 4)
+// lose their roots in the f32 Cardano solve: a small next to b and c inflates
+// both terms, whose difference is the root.  The backward's inside tests, which
+// follow the reference CPU's f64 solve, find their roots on the monotone
+// pieces of y(t) instead (winding_cubic_pieces).  On the 1024^2 scene of 1000
+// random closed cubic paths they are 1 % of the segments; solved by Cardano in
+// f32 they cause 6 858 of the 6 879 inside decisions (over 11.8M sample /
+// shape pairs) that differ from the f64 solve.  The 21 left are samples
+// within rounding distance of an edge.
+constexpr float WIND_PIECE_COND = 16.0f;
 
 DVG_DEV float fast_acos(float x) {
     float ax = fabsf(x);
@@ -148,7 +158,21 @@ DVG_DEV float cardano_a(float rr, float r2, float q3) {
     return (rr > 0.f) ? -c : c;
 }
 
-// curves.rs:897-946
+// Correctly rounded f32 acos / cos / pow: the f64 function rounded once,
+// which is the f32 value nearest the exact one unless that lies within f64
+// rounding of a tie.  The prefiltered forward's closest-point solves use them:
+// its Newton search stops at |f| < 1e-5, so the distance it returns moves with
+// the last bit of the root-isolating transcendentals, and a last-bit change
+// shifts a pixel by up to 7e-5 (1 % of the pixels of the 1024^2 prefiltered
+// scene beyond 1e-5 with the f32 library functions, none with these).
+DVG_DEV float acos_cr(float x) { return (float)acos((double)x); }
+DVG_DEV float cos_cr(float x) { return (float)cos((double)x); }
+DVG_DEV float pow_cr(float x, float y) { return (float)pow((double)x, (double)y); }
+
+// curves.rs:897-946.  CR: correctly rounded transcendentals (above);
+// otherwise fast ones on well-conditioned cubics and the f32 library's
+// elsewhere.
+template <bool CR = false>
 DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
     const float eps = 1.0e-6f;
     if (absf_(a) < eps) {
@@ -159,6 +183,23 @@ DVG_DEV int solve_cubic(float a, float b, float c, float d, float r[3]) {
     float q = (bb * bb - 3.0f * cc) / 9.0f;
     float rr = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
     float r2 = rr * rr, q3 = q * q * q;
+    if (CR) {
+        if (r2 < q3) {
+            float theta = acos_cr(rr / sqrtf(q3));
+            float sq = sqrtf(q);
+            const float two_pi = 2.0f * 3.14159265f;
+            r[0] = -2.0f * sq * cos_cr(theta * third) - bb * third;
+            r[1] = -2.0f * sq * cos_cr((theta + two_pi) * third) - bb * third;
+            r[2] = -2.0f * sq * cos_cr((theta - two_pi) * third) - bb * third;
+            return 3;
+        }
+        float x = ((rr > 0.f) ? rr : -rr) + sqrtf(r2 - q3);
+        float cb = (x > 0.f) ? pow_cr(x, third) : 0.f;
+        float ar = (rr > 0.f) ? -cb : cb;
+        float br = (absf_(ar) > eps) ? q / ar : 0.f;
+        r[0] = (ar + br) - bb * third;
+        return 1;
+    }
     // fast transcendentals when the root formulas do not amplify their errors
     bool fast = absf_(bb * third) < WIND_FAST_COND && q < WIND_FAST_COND * WIND_FAST_COND;
     if (r2 < q3) {
@@ -194,6 +235,22 @@ DVG_DEV void winding_line(float px, float py, float x0, float y0, float x1, floa
         if (xi > px) {
             if (rule == 1u) cr = 1.f - cr;
             else w += (y1 > y0) ? 1.f : -1.f;
+        }
+    }
+}
+
+// The reference CPU's line winding (distance/winding.rs:120-133): t in
+// [0, 1] closed, curve at or right of px, horizontal lines skipped.
+DVG_DEV void winding_line_cpu(float px, float py, float x0, float y0, float x1, float y1, uint32_t rule, float &w,
+                              float &cr) {
+    float dy = y1 - y0;
+    if (fabsf(dy) <= 1.0e-8f) return;
+    float t = (py - y0) / dy;
+    if (t >= 0.f && t <= 1.f) {
+        float tp = x0 - px + t * (x1 - x0);
+        if (tp >= 0.f) {
+            if (rule == 1u) cr = 1.f - cr;
+            else w += (dy > 0.f) ? 1.f : -1.f;
         }
     }
 }
@@ -258,6 +315,9 @@ DVG_DEV void winding_cubic(float px, float py, float x0, float y0, float x1, flo
 //   quad:  ay, by, -, by^2, 4 ay, 2 ay
 //   all:   [12..15] = min y, max y, max x, min x of the control points (the
 //          ray test's box, as the walk would compute it)
+//   cubic: [16..20] = the monotone pieces of y(t) - y0 = a t^3 + b t^2 + c t
+//          on [0, 1]: its critical points t1 <= t2 in (0, 1) (1 when absent)
+//          and its values Y(t1), Y(t2), Y(1), computed in f64 and rounded
 //
 // Root mask: in the three-root case root k is -2 sqrt(q) cos(phi_k) - bb/3
 // with phi_0 in [0, pi/3], phi_1 in [2pi/3, pi], phi_2 in [-2pi/3, -pi/3]
@@ -294,7 +354,39 @@ DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
         if (!out) mask |= 1u << k;
     }
     if (fabsf(sq) < WIND_FAST_COND && fabsf(off) < WIND_FAST_COND) mask |= 8u;  // well conditioned
+    // Bit 4: the f32 solve loses the roots' position (a small next to b and c
+    // inflates sqrt|q| and b / 3a, whose difference is the root), or a is
+    // below the reference's quadratic cut-off.  The backward's inside tests
+    // then solve in f64 as the reference CPU does.
+    float sqa = sqrtf(fabsf(q));
+    if (!(sqa < WIND_PIECE_COND && fabsf(off) < WIND_PIECE_COND) || !(fabsf(a) >= 1.0e-6f)) mask |= 16u;
     o[11] = (float)mask;
+    // monotone pieces: roots of Y'(t) = 3a t^2 + 2b t + c in (0, 1)
+    double A = a, B = b, C = c;
+    double r0 = 2.0, r1 = 2.0;
+    if (A == 0.0) {
+        if (B != 0.0) r0 = -C / (2.0 * B);
+    } else {
+        double disc = 4.0 * B * B - 12.0 * A * C;
+        if (disc >= 0.0) {
+            double sd = sqrt(disc);
+            double qq = -0.5 * (2.0 * B + (B < 0.0 ? -sd : sd));
+            if (qq != 0.0) {
+                r0 = qq / (3.0 * A);
+                r1 = C / qq;
+            } else {
+                r0 = -B / (3.0 * A);
+            }
+        }
+    }
+    if (!(r0 > 0.0 && r0 < 1.0)) r0 = 1.0;
+    if (!(r1 > 0.0 && r1 < 1.0)) r1 = 1.0;
+    double t1 = r0 < r1 ? r0 : r1, t2 = r0 < r1 ? r1 : r0;
+    o[16] = (float)t1;
+    o[17] = (float)t2;
+    o[18] = (float)(((A * t1 + B) * t1 + C) * t1);
+    o[19] = (float)(((A * t2 + B) * t2 + C) * t2);
+    o[20] = (float)(A + B + C);
 }
 
 DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
@@ -305,6 +397,7 @@ DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
     o[4] = 4.0f * ay;
     o[5] = 2.0f * ay;
     for (int k = 6; k < 12; ++k) o[k] = 0.f;
+    for (int k = 16; k < WIND_STRIDE; ++k) o[k] = 0.f;
 }
 
 // winding_quad with the segment's terms from its record (same result).
@@ -338,6 +431,11 @@ DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, fl
 }
 
 // winding_cubic with the segment's terms from its record (same result).
+// FAST: the fast transcendentals on well-conditioned segments (record bit 3).
+// The forward keeps the f32 library's: the fast ones move crossings by up to
+// 1e-3 px, which flips about 0.1 % of the pixels of the 2048^2 scene of
+// 10000 paths; the backward's inside tests take them.
+template <bool FAST>
 DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, float x2, float x3, const float *wc,
                              uint32_t rule, float &w, float &cr) {
     float a = wc[0];
@@ -362,7 +460,7 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
                 float sq = wc[7];
                 const float third = 1.0f / 3.0f;
                 const float two_pi = 2.0f * 3.14159265f;
-                if (mask & 8u) {
+                if (FAST && (mask & 8u)) {
                     float theta = fast_acos(rr / wc[6]);
                     if (mask & 1u) r[0] = -2.0f * sq * __cosf(theta * third) - off;
                     if (mask & 2u) r[1] = -2.0f * sq * __cosf((theta + two_pi) * third) - off;
@@ -376,7 +474,7 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
             }
             n = 3;
         } else {
-            float ar = ((uint32_t)wc[11] & 8u) ? cardano_a_fast(rr, r2, q3) : cardano_a(rr, r2, q3);
+            float ar = (FAST && ((uint32_t)wc[11] & 8u)) ? cardano_a_fast(rr, r2, q3) : cardano_a(rr, r2, q3);
             float br = (absf_(ar) > eps) ? q / ar : 0.f;
             r[0] = (ar + br) - off;
             n = 1;
@@ -395,6 +493,106 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
                 }
             }
         }
+    }
+}
+
+// ---- the backward's cubic inside test on ill-conditioned segments -------------
+// The reference CPU (distance/winding.rs:136-188) counts the roots t in [0, 1]
+// of P(t) = a t^3 + b t^2 + c t + (y0 - py), solved in f64
+// (distance/utils.rs:82-112), whose curve point lies strictly right of px,
+// signed by P'(t).  On a segment with record bit 4 the same roots are found
+// in f32 without Cardano's cancellation: each monotone piece of P (record
+// slots 16..20) holds a root exactly when P changes sign over it, and the
+// root is bracketed there, so safeguarded Newton steps on the unnormalised
+// polynomial converge to it; the piece's direction is the sign of P'.  A
+// cubic with |a| < 1e-6 is the CPU's quadratic b t^2 + c t + d
+// (utils.rs:44-79), whose double roots count twice.
+DVG_DEV float bezier_x(float t, float x0, float x1, float x2, float x3) {
+    float tt = 1.f - t;
+    return (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+}
+
+DVG_DEV float piece_root(float a, float b, float c, float d, float lo, float hi, float vlo, float vhi) {
+    // regula falsi start, then Newton kept inside the bracket [lo, hi]
+    float t = lo + (hi - lo) * (vlo / (vlo - vhi));
+    bool neg_lo = vlo < 0.f;
+    for (int it = 0; it < 12; ++it) {
+        float p = ((a * t + b) * t + c) * t + d;
+        if (p == 0.f) break;
+        float dp = (3.0f * a * t + 2.0f * b) * t + c;
+        if ((p < 0.f) == neg_lo) lo = t;
+        else hi = t;
+        float tn = t - p / dp;
+        if (!(tn >= lo && tn <= hi)) tn = 0.5f * (lo + hi);
+        bool done = fabsf(tn - t) <= 1.0e-7f;
+        t = tn;
+        if (done) break;
+    }
+    return t;
+}
+
+DVG_DEV int winding_cubic_pieces(float px, float py, float x0, float y0, float x1, float x2, float x3,
+                                 const float *wc) {
+    float a = wc[0], b = wc[1], c = wc[2];
+    float d = y0 - py;
+    int w = 0;
+    if (!(fabsf(a) >= 1.0e-6f)) {
+        float r0, r1;
+        int n = 0;
+        if (fabsf(b) < 1.0e-12f) {
+            if (fabsf(c) >= 1.0e-12f) { r0 = r1 = -d / c; n = 2; }
+        } else {
+            float disc = c * c - 4.0f * b * d;
+            if (disc >= 0.f) {
+                float root = sqrtf(disc);
+                n = 2;
+                if (root < 1.0e-12f) {
+                    r0 = r1 = -0.5f * c / b;
+                } else {
+                    float q = (c < 0.f) ? -0.5f * (c - root) : -0.5f * (c + root);
+                    if (fabsf(q) < 1.0e-12f) {
+                        float inv = 0.5f / b;
+                        r0 = (-c - root) * inv;
+                        r1 = (-c + root) * inv;
+                    } else {
+                        r0 = q / b;
+                        r1 = d / q;
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            float t = i == 0 ? r0 : r1;
+            if (t >= 0.f && t <= 1.f && bezier_x(t, x0, x1, x2, x3) > px)
+                w += ((3.0f * a * t + 2.0f * b) * t + c > 0.f) ? 1 : -1;
+        }
+        return w;
+    }
+    float tk[4] = {0.f, wc[16], wc[17], 1.f};
+    float vk[4] = {d, wc[18] + d, wc[19] + d, wc[20] + d};
+    for (int k = 0; k < 3; ++k) {
+        float va = vk[k], vb = vk[k + 1];
+        if ((va < 0.f) == (vb < 0.f) || !(tk[k] < tk[k + 1])) continue;
+        float t = piece_root(a, b, c, d, tk[k], tk[k + 1], va, vb);
+        if (bezier_x(t, x0, x1, x2, x3) > px) w += (vb > va) ? 1 : -1;
+    }
+    return w;
+}
+
+// The backward's cubic inside test: the f32 record solve, or the monotone
+// pieces on segments whose Cardano solve is ill-conditioned (record bit 4).
+DVG_DEV void winding_cubic_bwd(float px, float py, float x0, float y0, float x1, float x2, float x3,
+                               const float *wc, uint32_t rule, float &w, float &cr) {
+    if ((uint32_t)wc[11] & 16u) {
+        // the even-odd rule toggles per crossing: the parity of the signed sum
+        int n = winding_cubic_pieces(px, py, x0, y0, x1, x2, x3, wc);
+        if (rule == 1u) {
+            if (n & 1) cr = 1.f - cr;
+        } else {
+            w += (float)n;
+        }
+    } else {
+        winding_cubic_w<true>(px, py, x0, y0, x1, x2, x3, wc, rule, w, cr);
     }
 }
 
@@ -434,6 +632,7 @@ DVG_DEV void quad_closest_approx(float px, float py, float x0, float y0, float x
 
 // curves.rs:218-279 (closest_point_quadratic_with_t; distance_to_quadratic is
 // the same arithmetic)
+template <bool CR = false>
 DVG_DEV void quad_closest(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
                           bool approx, float &dist, float &tout) {
     float best_t = 0.f;
@@ -454,7 +653,7 @@ DVG_DEV void quad_closest(float px, float py, float x0, float y0, float x1, floa
         float c = 2.0f * (bx * bx + by * by) + (ax * cx + ay * cy);
         float d = bx * cx + by * cy;
         float r[3];
-        int n = solve_cubic(a, b, c, d, r);
+        int n = solve_cubic<CR>(a, b, c, d, r);
         for (int i = 0; i < n; ++i) {
             float t = r[i];
             if (t >= 0.f && t <= 1.f) {
@@ -501,6 +700,7 @@ DVG_DEV void cubic_closest_approx(float px, float py, float x0, float y0, float 
 
 // curves.rs:561-731 (closest_point_cubic_with_t; distance_to_cubic,
 // curves.rs:341-505, is the same arithmetic)
+template <bool CR = false>
 DVG_DEV void cubic_closest(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2, float x3,
                            float y3, bool approx, float &dist, float &tout) {
     if (approx) {
@@ -532,7 +732,7 @@ DVG_DEV void cubic_closest(float px, float py, float x0, float y0, float x1, flo
         float p1d = f - b * e / 25.0f;
         float q_root = -b / 5.0f;
         float pr[3];
-        int num_p = solve_cubic(p1a, p1b, p1c, p1d, pr);
+        int num_p = solve_cubic<CR>(p1a, p1b, p1c, p1d, pr);
         float iv[4];
         int ni = 0;
         if (q_root >= 0.f && q_root <= 1.f) iv[ni++] = q_root;

@@ -11,7 +11,7 @@ constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.r
 constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
 constexpr int FRAG_WINDOW = 8;      // fragments held per lane in the adjoint walk
 constexpr int LDS_GROUP_CAP = 256;  // tile groups whose paint grads are staged in LDS
-constexpr int WIND_STRIDE = 16;     // floats of a segment's winding-term record
+constexpr int WIND_STRIDE = 24;     // floats of a segment's winding-term record
 
 // Scene arrays are read through the constant address space on the device:
 // they do not change while a render or adjoint kernel runs, so loads at
@@ -161,6 +161,10 @@ struct UpdatePlan {
 // Counts into *mismatches (device) the random cubic segment / point pairs whose
 // winding from the per-segment record differs from the libm formula.
 void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hipStream_t stream);
+// counts[0..8): per family of random pairs, the backward inside tests that
+// differ from the reference CPU's f64 winding, then the pairs solved on
+// monotone pieces.
+void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStream_t stream);
 // Per-segment winding terms of curve_data (13 floats a segment) into
 // curve_wind; after every change of curve_data.
 void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream);
@@ -189,6 +193,7 @@ enum BatchPhase : int {
     PH_ZERO = 0,
     PH_WEIGHTS_FWD,
     PH_FORWARD,
+    PH_FORWARD_PF,  // the forward of prefiltered scenes
     PH_RESOLVE,
     PH_WEIGHTS_BWD,
     PH_INTERIOR,

@@ -227,7 +227,11 @@ DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, f
 }
 
 // accumulate_shape_fill, forward.rs:409-560 (distances only when prefiltering:
-// the non-prefiltered blend reads the winding alone).
+// the non-prefiltered blend reads the winding alone).  With CPU = true the
+// winding follows the reference CPU's winding_number_segment
+// (distance/winding.rs:118-192): the closed-interval line rule and the f64
+// cubic solve on ill-conditioned segments.
+template <bool CPU>
 DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool prefilter, float px, float py,
                         Cov &c) {
     uint32_t kind = (uint32_t)sd[0];
@@ -270,17 +274,24 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
             if (sqrtf(ddx * ddx + ddy * ddy) <= lmin) {
                 float d, t;
                 if (k == 0) d = seg_dist(px, py, x0, y0, x1, y1);
-                else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
-                else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+                else if (k == 1) quad_closest<!CPU>(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+                else cubic_closest<!CPU>(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
                 if (d < lmin) lmin = d;
             }
         }
         // forward.rs:938-1018: a +x ray only meets segments spanning py with
         // some control point right of px.
         if (py < by0 || py > by1 || px > bx1) continue;
-        if (k == 0) winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
-        else if (k == 1) winding_quad_w(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
-        else winding_cubic_w(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
+        if (k == 0) {
+            if (CPU) winding_line_cpu(px, py, x0, y0, x1, y1, rule, lw, lc);
+            else winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
+        } else if (k == 1) {
+            winding_quad_w(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
+        } else if (CPU) {
+            winding_cubic_bwd(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
+        } else {
+            winding_cubic_w<false>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
+        }
     }
     if (lmin < c.fill_min) c.fill_min = lmin;
     if (rule == 1u) {
@@ -352,8 +363,14 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
         }
         float d, t;
         if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
-        else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
-        else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        else if (!CPU && prefilter) {
+            if (k == 1) quad_closest<true>(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+            else cubic_closest<true>(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        } else if (k == 1) {
+            quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+        } else {
+            cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        }
         float r = seg_radius(k, t, r0, r1, r2, r3);
         if (prefilter) {
             if (d < lmin) { lmin = d; lr = r; }
@@ -388,7 +405,7 @@ DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx,
                 const float *xf = S.shape_xform + si * 6u;
                 float spx = xf[0] * lx + xf[1] * ly + xf[2];
                 float spy = xf[3] * lx + xf[4] * ly + xf[5];
-                shape_fill(S, S.shape_data + si * 15u, rule, false, spx, spy, c);
+                shape_fill<true>(S, S.shape_data + si * 15u, rule, false, spx, spy, c);
             }
         }
         do_fill = false;
@@ -403,7 +420,7 @@ DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx,
         float spx = xf[0] * lx + xf[1] * ly + xf[2];
         float spy = xf[3] * lx + xf[4] * ly + xf[5];
         const float *sd = S.shape_data + si * 15u;
-        if (do_fill) shape_fill(S, sd, rule, prefilter, spx, spy, c);
+        if (do_fill) shape_fill<false>(S, sd, rule, prefilter, spx, spy, c);
         if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<CPU>(S, sd, prefilter, spx, spy, c);
     }
 }
@@ -423,9 +440,12 @@ DVG_DEV void blend(float4 &o, float4 c, float a) {
 }
 
 // eval_scene_tiled, forward.rs:1133-1290: premultiplied RGBA at (px, py).
+// PF = S.prefilter, a template argument so that the analytic-coverage code
+// (with its correctly rounded solves) stays out of the point-sampled kernel.
+template <bool PF>
 DVG_DEV float4 eval_forward(const DevScene &S, const uint32_t *list, uint32_t n, float px, float py, float4 bg) {
     float4 out = make_float4(bg.x * bg.w, bg.y * bg.w, bg.z * bg.w, bg.w);
-    bool prefilter = S.prefilter != 0;
+    const bool prefilter = PF;
     for (uint32_t e = 0; e < n; ++e) {
         uint32_t g = list[e];
         const float *gd = S.group_data + g * 16u;
@@ -515,6 +535,7 @@ __global__ void k_weights(DevScene S, float *__restrict__ weight) {
 constexpr int SPLAT_HALO = 4;
 constexpr int SPLAT_EDGE = TILE + 2 * SPLAT_HALO;
 
+template <bool PF>
 DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restrict__ weight,
                           float *__restrict__ accum) {
     __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
@@ -539,7 +560,7 @@ DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restr
         if (x >= S.W || y >= S.H) continue;
         float px, py;
         sample_point<false>(S, x, y, sx, sy, px, py);
-        float4 col = eval_forward(S, list, n, px, py, background_at(S, x, y));
+        float4 col = eval_forward<PF>(S, list, n, px, py, background_at(S, x, y));
         for (int dy = -ri; dy <= ri; ++dy)
             for (int dx = -ri; dx <= ri; ++dx) {
                 int xx = (int)x + dx, yy = (int)y + dy;
@@ -582,9 +603,10 @@ DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restr
     }
 }
 
+template <bool PF>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_forward(
     DevScene S, const float *__restrict__ weight, float *__restrict__ accum) {
-    forward_unit(S, blockIdx.x, weight, accum);
+    forward_unit<PF>(S, blockIdx.x, weight, accum);
 }
 
 // resolve_splat_f32 with the premultiplied background of renderer.rs:499-515.
@@ -2318,7 +2340,7 @@ DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float 
                 const float *sd = S.shape_data + eq_prep * 15u;
                 Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
                 if (sk != DVG_PAINT_NONE) shape_stroke<true>(S, sd, false, spx, spy, c);
-                if (fk != DVG_PAINT_NONE) shape_fill(S, sd, rule, false, spx, spy, c);
+                if (fk != DVG_PAINT_NONE) shape_fill<true>(S, sd, rule, false, spx, spy, c);
                 if (c.stroke_hit > 0.f) edge_test = true;
                 if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
                     if (fk != DVG_PAINT_NONE) edge_test = true;
@@ -2827,8 +2849,138 @@ __global__ void k_selftest_winding(uint32_t n, uint32_t seed, uint32_t *__restri
     wind_coef_cubic(y[0], y[1], y[2], y[3], rec);
     float w0 = 0.f, w1 = 0.f, cr = 0.f;
     winding_cubic_libm(px, py, x[0], y[0], x[1], y[1], x[2], y[2], x[3], y[3], w0);
-    winding_cubic_w(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w1, cr);
+    winding_cubic_w<true>(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w1, cr);
     if (w0 != w1) atomicAdd(mismatches, 1u);
+}
+
+// ---- self-test of the backward's inside test ------------------------------
+// The reference CPU's cubic winding (distance/winding.rs:136-188 over
+// distance/utils.rs:44-123) restated in f64 with the same steps, the checker
+// of winding_cubic_bwd.
+DVG_DEV double cbrt_f64(double x) {  // utils.rs:115-123
+    if (x > 0.0) return pow(x, 1.0 / 3.0);
+    if (x < 0.0) return -pow(-x, 1.0 / 3.0);
+    return 0.0;
+}
+
+DVG_DEV int quadratic_roots_f64(double a, double b, double c, double t[2]) {  // utils.rs:44-79
+    const double eps = 1.0e-12;
+    if (fabs(a) < eps) {
+        if (fabs(b) < eps) return 0;
+        t[0] = t[1] = -c / b;
+        return 2;
+    }
+    double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) return 0;
+    double root = sqrt(disc);
+    if (fabs(root) < eps) {
+        t[0] = t[1] = -0.5 * b / a;
+        return 2;
+    }
+    double q = (b < 0.0) ? -0.5 * (b - root) : -0.5 * (b + root);
+    double lo, hi;
+    if (fabs(q) < eps) {
+        double inv = 0.5 / a;
+        lo = (-b - root) * inv;
+        hi = (-b + root) * inv;
+    } else {
+        lo = q / a;
+        hi = c / q;
+    }
+    t[0] = lo < hi ? lo : hi;
+    t[1] = lo < hi ? hi : lo;
+    return 2;
+}
+
+DVG_DEV int cubic_roots_f64(double a, double b, double c, double d, double t[3]) {  // utils.rs:82-112
+    if (fabs(a) < 1.0e-6) return quadratic_roots_f64(b, c, d, t);
+    double bn = b / a, cn = c / a, dn = d / a;
+    double q = (bn * bn - 3.0 * cn) / 9.0;
+    double r = (2.0 * bn * bn * bn - 9.0 * bn * cn + 27.0 * dn) / 54.0;
+    double q3 = q * q * q;
+    if (r * r < q3) {
+        const double pi = 3.14159265358979323846;
+        double theta = acos(r / sqrt(q3));
+        double s = -2.0 * sqrt(q);
+        t[0] = s * cos(theta / 3.0) - bn / 3.0;
+        t[1] = s * cos((theta + 2.0 * pi) / 3.0) - bn / 3.0;
+        t[2] = s * cos((theta - 2.0 * pi) / 3.0) - bn / 3.0;
+        return 3;
+    }
+    double disc = sqrt(r * r - q3);
+    double ar = (r > 0.0) ? -cbrt_f64(r + disc) : cbrt_f64(-r + disc);
+    double br = (fabs(ar) > 1.0e-6) ? q / ar : 0.0;
+    t[0] = (ar + br) - bn / 3.0;
+    return 1;
+}
+
+DVG_DEV int winding_cubic_f64(float px, float py, const float *x, const float *y) {  // winding.rs:136-188
+    double a = (double)(-y[0] + 3.0f * y[1] - 3.0f * y[2] + y[3]);
+    double b = (double)(3.0f * y[0] - 6.0f * y[1] + 3.0f * y[2]);
+    double c = (double)(-3.0f * y[0] + 3.0f * y[1]);
+    double d = (double)(y[0] - py);
+    double t[3];
+    int n = cubic_roots_f64(a, b, c, d, t);
+    double ax = (double)(-x[0] + 3.0f * x[1] - 3.0f * x[2] + x[3]);
+    double bx = (double)(3.0f * x[0] - 6.0f * x[1] + 3.0f * x[2]);
+    double cx = (double)(-3.0f * x[0] + 3.0f * x[1]);
+    double dx = (double)(x[0] - px);
+    int w = 0;
+    for (int i = 0; i < n; ++i) {
+        double s = t[i];
+        if (s >= 0.0 && s <= 1.0) {
+            double tp = ax * s * s * s + bx * s * s + cx * s + dx;
+            if (tp > 0.0) w += (3.0 * a * s * s + 2.0 * b * s + c > 0.0) ? 1 : -1;
+        }
+    }
+    return w;
+}
+
+// Random segment / point pairs in four families (pair i is family i % 4):
+// 0 random control points in a 128 px square, py in the segment's y-range;
+// 1 near-quadratic cubics (|a| <= 0.25 against control points 128 px apart),
+// the ill-conditioned case; 2 py within 0.05 px of a y-extremum of the curve,
+// where two roots merge; 3 points within 0.01 px of the curve.  counts[f]
+// receives the pairs of family f whose winding differs between
+// winding_cubic_bwd and the f64 restatement, counts[4 + f] the pairs that
+// took the monotone-piece path.
+__global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restrict__ counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t fam = i & 3u;
+    Pcg32 rng((uint64_t)i, (uint64_t)seed);
+    float cx = 128.0f * rng.next_f32(), cy = 128.0f * rng.next_f32();
+    float x[4], y[4];
+    for (int k = 0; k < 4; ++k) {
+        x[k] = cx + 128.0f * rng.next_f32() - 64.0f;
+        y[k] = cy + 128.0f * rng.next_f32() - 64.0f;
+    }
+    if (fam == 1u) y[3] = y[0] - 3.0f * y[1] + 3.0f * y[2] + 0.5f * (rng.next_f32() - 0.5f);
+    float ymin = fminf(fminf(y[0], y[1]), fminf(y[2], y[3])), ymax = fmaxf(fmaxf(y[0], y[1]), fmaxf(y[2], y[3]));
+    float px = cx + 128.0f * rng.next_f32() - 64.0f;
+    float py = ymin + (ymax - ymin) * rng.next_f32();
+    float rec[WIND_STRIDE];
+    wind_coef_cubic(y[0], y[1], y[2], y[3], rec);
+    if (fam == 2u) {
+        // a y-extremum of the curve: record slot 16 or 17 when inside (0, 1)
+        float tc = rec[16] < 1.f ? rec[16] : (rec[17] < 1.f ? rec[17] : 0.5f);
+        float tt = 1.f - tc;
+        float yc = (tt * tt * tt) * y[0] + (3.0f * tt * tt * tc) * y[1] + (3.0f * tt * tc * tc) * y[2] +
+                   (tc * tc * tc) * y[3];
+        py = yc + 0.1f * (rng.next_f32() - 0.5f);
+    } else if (fam == 3u) {
+        float t = rng.next_f32(), tt = 1.f - t;
+        float bx = (tt * tt * tt) * x[0] + (3.0f * tt * tt * t) * x[1] + (3.0f * tt * t * t) * x[2] + (t * t * t) * x[3];
+        float by = (tt * tt * tt) * y[0] + (3.0f * tt * tt * t) * y[1] + (3.0f * tt * t * t) * y[2] + (t * t * t) * y[3];
+        px = bx + 0.02f * (rng.next_f32() - 0.5f);
+        py = by + 0.02f * (rng.next_f32() - 0.5f);
+    }
+    if (py < ymin || py > ymax) return;  // the walks cull those (no root either way)
+    float w = 0.f, cr = 0.f;
+    winding_cubic_bwd(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w, cr);
+    int w64 = winding_cubic_f64(px, py, x, y);
+    if ((int)w != w64) atomicAdd(counts + fam, 1u);
+    if ((uint32_t)rec[11] & 16u) atomicAdd(counts + 4 + fam, 1u);
 }
 
 // Winding terms of every segment (the record described with WIND_STRIDE).
@@ -3098,9 +3250,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_weights(DVG_QUEUE_ARGS) {
     DVG_QUEUE_LOOP(weights_unit<CPU>(B.S, w.unit, B.weight))
 }
 
+template <bool PF>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_batch_forward(
     DVG_QUEUE_ARGS) {
-    DVG_QUEUE_LOOP(forward_unit(B.S, w.unit, B.weight, B.accum))
+    DVG_QUEUE_LOOP(forward_unit<PF>(B.S, w.unit, B.weight, B.accum))
 }
 
 __global__ __launch_bounds__(BLOCK) void k_batch_resolve(DVG_QUEUE_ARGS, int loss) {
@@ -3180,8 +3333,9 @@ void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stre
     else hipLaunchKernelGGL(k_weights<false>, dim3(nb), dim3(256), 0, stream, S, weight);
 }
 void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream) {
-    hipLaunchKernelGGL(k_forward, dim3(S.tiles_x * S.tiles_y * units_per_tile(S)), dim3(BLOCK), 0, stream, S, weight,
-                       accum);
+    dim3 grid(S.tiles_x * S.tiles_y * units_per_tile(S));
+    if (S.prefilter) hipLaunchKernelGGL(k_forward<true>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
+    else hipLaunchKernelGGL(k_forward<false>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
 }
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream) {
     hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)S.W * S.H, 256)), dim3(256), 0, stream, S, weight, accum, out);
@@ -3233,7 +3387,12 @@ void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *ite
         case PH_WEIGHTS_FWD:
             hipLaunchKernelGGL(k_batch_weights<false>, g, b, 0, stream, scenes, items, n_items, counter);
             break;
-        case PH_FORWARD: hipLaunchKernelGGL(k_batch_forward, g, b, 0, stream, scenes, items, n_items, counter); break;
+        case PH_FORWARD:
+            hipLaunchKernelGGL(k_batch_forward<false>, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
+        case PH_FORWARD_PF:
+            hipLaunchKernelGGL(k_batch_forward<true>, g, b, 0, stream, scenes, items, n_items, counter);
+            break;
         case PH_RESOLVE:
             hipLaunchKernelGGL(k_batch_resolve, g, b, 0, stream, scenes, items, n_items, counter, loss ? 1 : 0);
             break;
@@ -3265,6 +3424,12 @@ void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hi
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, mismatches, 1u, 0u);
     hipLaunchKernelGGL(k_selftest_winding, dim3(cdiv(n, 256)), dim3(256), 0, stream, n, seed, mismatches);
+}
+
+void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, counts, 8u, 0u);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_selftest_inside, dim3(cdiv(n, 256)), dim3(256), 0, stream, n, seed, counts);
 }
 
 void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream) {

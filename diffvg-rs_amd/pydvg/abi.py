@@ -164,6 +164,7 @@ SIGNATURES = {
     "dvg_batch_autotune": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_int, C.POINTER(BatchConfigC)]),
     "dvg_version": (C.c_char_p, []),
     "dvg_selftest_winding": (C.c_int32, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "dvg_selftest_inside": (C.c_int32, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
 }
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

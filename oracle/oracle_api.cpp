@@ -41,15 +41,26 @@ int oracle_render(const dvg_scene_desc *desc, const dvg_render_options *opt, flo
     return render_forward(sc, *opt, out, sample_colors, &err);
 }
 
-// Reference CPU render_backward into the flat gradient layout.
-int oracle_render_backward(const dvg_scene_desc *desc, const dvg_render_options *opt,
-                           const dvg_backward_options *bopt, const float *d_render, const dvg_grads *out) {
+// Pixel rows [y0, y1) of the forward render into out ((y1 - y0) * W * 4),
+// bit-identical to the same rows of oracle_render.
+int oracle_render_rows(const dvg_scene_desc *desc, const dvg_render_options *opt, uint32_t y0, uint32_t y1,
+                       float *out) {
+    Scene sc = read_scene(desc);
+    const char *err = nullptr;
+    return render_forward(sc, *opt, out, nullptr, &err, y0, y1);
+}
+
+// Reference CPU render_backward into the flat gradient layout, for a d_render
+// that is zero outside pixel rows [y0, y1) (the whole image: 0, UINT32_MAX).
+int oracle_render_backward_rows(const dvg_scene_desc *desc, const dvg_render_options *opt,
+                                const dvg_backward_options *bopt, const float *d_render, const dvg_grads *out,
+                                uint32_t y0, uint32_t y1) {
     Scene sc = read_scene(desc);
     Grads gr;
     bool tr = bopt && bopt->compute_translation;
     grads_zero(sc, tr, gr);
     const char *err = nullptr;
-    int st = render_backward(sc, *opt, tr, d_render, gr, &err);
+    int st = render_backward(sc, *opt, tr, d_render, gr, &err, y0, y1);
     if (st != DVG_OK) return st;
     fill_out(gr.d_shape_params, out->d_shape_params);
     fill_out(gr.d_shape_points, out->d_shape_points);
@@ -66,6 +77,12 @@ int oracle_render_backward(const dvg_scene_desc *desc, const dvg_render_options 
     fill_out(gr.d_background_image, out->d_background_image);
     fill_out(gr.d_translation, out->d_translation);
     return DVG_OK;
+}
+
+// Reference CPU render_backward into the flat gradient layout.
+int oracle_render_backward(const dvg_scene_desc *desc, const dvg_render_options *opt,
+                           const dvg_backward_options *bopt, const float *d_render, const dvg_grads *out) {
+    return oracle_render_backward_rows(desc, opt, bopt, d_render, out, 0, 0xffffffffu);
 }
 
 // ---- primitives, for pinning the restatement against closed forms ----------

@@ -1455,7 +1455,7 @@ static void accumulate_boundary_gradient(const Scene &sc, uint32_t si, uint32_t 
 
 void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, const float *d_render,
                        const std::vector<float> &wimg, Grads &gr, bool compute_translation, size_t i0,
-                       size_t i1) {  // boundary.rs:113-197, over sample indices [i0, i1)
+                       size_t i1, long row_lo, long row_hi) {  // boundary.rs:113-197, over sample indices [i0, i1)
     const Scene &sc = *cx.sc;
     if (sc.groups.empty()) return;
     std::vector<float> L = compute_shape_lengths(cx);
@@ -1497,6 +1497,7 @@ void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed,
         if (W <= 0 || H <= 0 || pdf <= 0.f) continue;
         int bx = (int)(bpt.x * (float)W), by = (int)(bpt.y * (float)H);
         if (bx < 0 || bx >= W || by < 0 || by >= H) continue;
+        if (by < row_lo || by >= row_hi) continue;
         size_t pix = (size_t)by * W + bx;
         V4 bg = sample_background(sc, pix);
         EdgeQuery iq{gi, si, false}, oq{gi, si, false};
@@ -1618,7 +1619,7 @@ std::vector<float> weight_image(const Scene &sc, const dvg_render_options &opt) 
 
 // render_backward, backward/render.rs:35-210 (d_render present, d_sdf absent).
 int render_backward(const Scene &sc, const dvg_render_options &opt, bool compute_translation, const float *d_render,
-                    Grads &gr, const char **err) {
+                    Grads &gr, const char **err, uint32_t y0, uint32_t y1) {
     using namespace cpu;
     if (sc.has_bg_image && sc.bg_image.size() != (size_t)sc.width * sc.height * 4) {
         *err = "background image size mismatch";
@@ -1630,10 +1631,15 @@ int render_backward(const Scene &sc, const dvg_render_options &opt, bool compute
     Ctx cx;
     make_ctx(sc, cx);
     std::vector<float> wimg = weight_image(sc, opt);
-    backward_samples(cx, opt, compute_translation, d_render, wimg, gr, 0, total);
+    long ri = (long)rmax(std::ceil(sc.filter_radius), 0.0f);
+    size_t row = (size_t)sc.width * nsx * nsy;  // samples of a pixel row
+    y1 = std::min(y1, sc.height);
+    size_t ys = (size_t)std::max(0L, (long)y0 - ri), ye = std::min<size_t>(sc.height, (size_t)y1 + ri);
+    if (ys < ye) backward_samples(cx, opt, compute_translation, d_render, wimg, gr, ys * row, ye * row);
     // boundary sampling only without prefiltering (render.rs:192-206)
     if (!opt.use_prefiltering)
-        boundary_sampling(cx, nsx, nsy, opt.seed, d_render, wimg, gr, compute_translation, 0, total);
+        boundary_sampling(cx, nsx, nsy, opt.seed, d_render, wimg, gr, compute_translation, 0, total,
+                          (long)y0 - ri - 1, (long)y1 + ri + 1);
     finalize_background_gradients(sc, gr);
     return DVG_OK;
 }

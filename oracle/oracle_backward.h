@@ -64,15 +64,21 @@ V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 
                             size_t pix);
 void boundary_query_points(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, size_t i0, size_t i1,
                            std::vector<float> &pts);
+// Samples whose pixel row lies outside [row_lo, row_hi) are skipped.
 void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed, const float *d_render,
-                       const std::vector<float> &wimg, Grads &gr, bool compute_translation, size_t i0, size_t i1);
+                       const std::vector<float> &wimg, Grads &gr, bool compute_translation, size_t i0, size_t i1,
+                       long row_lo = -1, long row_hi = 1L << 40);
 
 }  // namespace cpu
 
 // render_backward (backward/render.rs:35-210) with d_render present.
 // Gradients are finalized (background.rs:29-61).
+// With a row window [y0, y1) d_render must be zero outside those pixel rows:
+// the samples that then gather a zero adjoint (interior samples of rows
+// outside [y0 - ri, y1 + ri), boundary samples whose pixel row is more than
+// ri + 1 rows away, ri = ceil(filter radius)) add exact zeros and are skipped.
 int render_backward(const Scene &sc, const dvg_render_options &opt, bool compute_translation, const float *d_render,
-                    Grads &gr, const char **err);
+                    Grads &gr, const char **err, uint32_t y0 = 0, uint32_t y1 = 0xffffffffu);
 
 void grads_zero(const Scene &sc, bool with_translation, Grads &gr);
 void make_ctx(const Scene &sc, Ctx &cx);

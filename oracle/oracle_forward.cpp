@@ -834,22 +834,29 @@ void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, 
 }
 
 int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_img, float *sample_colors,
-                   const char **err) {
+                   const char **err, uint32_t y0, uint32_t y1) {
     uint32_t W = sc.width, H = sc.height;
+    y1 = std::min(y1, H);
+    if (y0 >= y1) return DVG_OK;
+    if (sample_colors && (y0 != 0 || y1 != H)) {
+        *err = "sample colours need the whole image";
+        return DVG_INVALID_SCENE;
+    }
     if (sc.has_bg_image && sc.bg_image.size() != (size_t)W * H * 4) {
         *err = "background image size mismatch";
         return DVG_INVALID_SCENE;
     }
     if (W == 0 || H == 0) return DVG_OK;
     if (sc.groups.empty()) {
-        for (size_t i = 0; i < (size_t)W * H; ++i) {
+        for (size_t i = (size_t)y0 * W; i < (size_t)y1 * W; ++i) {
+            float *o = out_img + (i - (size_t)y0 * W) * 4;
             if (sc.has_bg_image) {
-                for (int c = 0; c < 4; ++c) out_img[i * 4 + c] = sc.bg_image[i * 4 + c];
+                for (int c = 0; c < 4; ++c) o[c] = sc.bg_image[i * 4 + c];
             } else {
-                out_img[i * 4 + 0] = sc.background.x;
-                out_img[i * 4 + 1] = sc.background.y;
-                out_img[i * 4 + 2] = sc.background.z;
-                out_img[i * 4 + 3] = sc.background.w;
+                o[0] = sc.background.x;
+                o[1] = sc.background.y;
+                o[2] = sc.background.z;
+                o[3] = sc.background.w;
             }
         }
         return DVG_OK;
@@ -863,8 +870,11 @@ int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_im
     float inv_sx = 1.0f / (float)nsx, inv_sy = 1.0f / (float)nsy;
     size_t npix = (size_t)W * H;
     std::vector<float> weight(npix, 0.f), color(npix * 4, 0.f);
+    // sample rows whose filter footprint reaches rows [y0, y1)
+    uint32_t ys = y0 > (uint32_t)ri ? y0 - (uint32_t)ri : 0u;
+    uint32_t ye = std::min(H, y1 + (uint32_t)ri);
     // rasterize_weights_f32
-    for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t y = ys; y < ye; ++y)
         for (uint32_t x = 0; x < W; ++x)
             for (uint32_t sy = 0; sy < nsy; ++sy)
                 for (uint32_t sx = 0; sx < nsx; ++sx) {
@@ -883,7 +893,7 @@ int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_im
                 }
     // rasterize_splat_f32
     uint32_t spp = nsx * nsy;
-    for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t y = ys; y < ye; ++y)
         for (uint32_t x = 0; x < W; ++x)
             for (uint32_t sy = 0; sy < nsy; ++sy)
                 for (uint32_t sx = 0; sx < nsx; ++sx) {
@@ -921,13 +931,14 @@ int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_im
     // resolve_splat_f32 with the premultiplied background of renderer.rs:499-515
     float bgp[4] = {sc.background.x * sc.background.w, sc.background.y * sc.background.w,
                     sc.background.z * sc.background.w, sc.background.w};
-    for (size_t i = 0; i < npix; ++i) {
+    for (size_t i = (size_t)y0 * W; i < (size_t)y1 * W; ++i) {
         float r = bgp[0], g = bgp[1], b = bgp[2], a = bgp[3];
         if (weight[i] > 0.f) {
             r = color[i * 4]; g = color[i * 4 + 1]; b = color[i * 4 + 2]; a = color[i * 4 + 3];
         }
         if (a > 1.0e-8f) { r = r / a; g = g / a; b = b / a; }
-        out_img[i * 4] = r; out_img[i * 4 + 1] = g; out_img[i * 4 + 2] = b; out_img[i * 4 + 3] = a;
+        float *o = out_img + (i - (size_t)y0 * W) * 4;
+        o[0] = r; o[1] = g; o[2] = b; o[3] = a;
     }
     return DVG_OK;
 }

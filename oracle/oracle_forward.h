@@ -26,10 +26,14 @@ void jitter_xy(uint32_t x, uint32_t y, uint32_t sx, uint32_t sy, uint32_t width,
                uint32_t seed, bool jitter, float &rx, float &ry);
 }  // namespace gpu
 
-// Full forward render; sample_colors (optional) receives the premultiplied
-// per-sample colours in (pixel, sy, sx) order.
 OpCount count_eval_points(const Scene &sc, const dvg_render_options &opt, const float *pts, size_t n);
 void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, size_t i1);
+// Forward render; sample_colors (optional, whole image only) receives the
+// premultiplied per-sample colours in (pixel, sy, sx) order.  Pixel rows
+// [y0, y1) are resolved into out_img (row y0 first) from the samples of rows
+// [y0 - ri, y1 + ri), ri = ceil(filter radius): every sample whose filter
+// reaches those rows, visited in the whole-image order, so the rows are
+// bit-identical to those of the whole-image render.
 int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_img, float *sample_colors,
-                   const char **err);
+                   const char **err, uint32_t y0 = 0, uint32_t y1 = 0xffffffffu);
 }  // namespace oracle

@@ -31,6 +31,13 @@ def lib():
         L.oracle_render_backward.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions),
                                              C.POINTER(abi.BackwardOptions), f32p, C.POINTER(abi.GradsC)]
         L.oracle_render_backward.restype = C.c_int
+        L.oracle_render_rows.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), C.c_uint32,
+                                         C.c_uint32, f32p]
+        L.oracle_render_rows.restype = C.c_int
+        L.oracle_render_backward_rows.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions),
+                                                  C.POINTER(abi.BackwardOptions), f32p, C.POINTER(abi.GradsC),
+                                                  C.c_uint32, C.c_uint32]
+        L.oracle_render_backward_rows.restype = C.c_int
         L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u32p, f32p]
         L.oracle_solve_cubic_f64.argtypes = [C.c_double] * 4 + [C.POINTER(C.c_double)]
         L.oracle_solve_cubic_f64.restype = C.c_int
@@ -67,6 +74,33 @@ def render(desc, opts, want_samples=False):
     if st != 0:
         raise RuntimeError(f"oracle_render status {st}")
     return (out, samples) if want_samples else out
+
+
+def render_rows(desc, opts, y0, y1):
+    """Pixel rows [y0, y1) of the forward image ((y1 - y0) * W * 4), equal to
+    the same rows of render(): only the samples whose filter reaches them are
+    evaluated."""
+    sc = desc.scene
+    y1 = min(y1, sc.height)
+    out = np.zeros(max(y1 - y0, 0) * sc.width * 4, np.float32)
+    st = lib().oracle_render_rows(desc.ref(), C.byref(opts.to_c()), y0, y1, _fp(out))
+    if st != 0:
+        raise RuntimeError(f"oracle_render_rows status {st}")
+    return out
+
+
+def render_backward_rows(desc, opts, d_render, y0, y1, translation=False):
+    """render_backward for a d_render that is zero outside pixel rows [y0, y1):
+    the samples that gather a zero adjoint are skipped (they add exact zeros)."""
+    from pydvg.scene import BackwardOptions, FlatGrads
+    g = FlatGrads(desc, translation)
+    d_render = np.ascontiguousarray(d_render, np.float32)
+    st = lib().oracle_render_backward_rows(desc.ref(), C.byref(opts.to_c()),
+                                           C.byref(BackwardOptions(translation).to_c()), _fp(d_render), g.ref(),
+                                           y0, y1)
+    if st != 0:
+        raise RuntimeError(f"oracle_render_backward_rows status {st}")
+    return g
 
 
 def prefiltered_objective(desc, opts, d_render):

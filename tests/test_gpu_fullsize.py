@@ -1,0 +1,128 @@
+"""GPU parity at the full workload sizes.
+
+The scenes here are the benchmark workloads themselves: 1024^2 with 1000
+cubic paths (2x2 jittered samples), 512^2 with 512 mixed shapes (tent
+filter), 1024^2 prefiltered strokes and polygons, 2048^2 with 10000 cubic
+paths.  The oracle needs minutes for one of these images, so it is asked for
+a band of pixel rows only (oracle_render_rows; its band equals the
+whole-image result bit for bit, which the CPU tests check): the GPU renders
+the whole image and its band is compared.  For the gradients d_render is
+the L2 loss adjoint against a jittered copy of the scene, kept inside the
+band and zero elsewhere; the GPU runs its whole backward pass (every interior
+and boundary sample) and the oracle only the samples that can see the band.
+
+Size-independent properties complete the picture where the oracle is too
+slow (the 2048^2 backward): the backward pass is linear in d_render, and a
+render repeats itself.
+
+Tolerances are those of the small-scene parity tests: image channels within
+1e-5 except at most 0.1 % of the pixels, gradient relative L2 <= 1e-3.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from parity_util import grad_report, image_report
+from pydvg import scenes
+from pydvg.scene import GRAD_FIELDS, SceneDesc
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+FLIP_FRACTION = 1e-3
+GRAD_REL = 1e-3
+
+
+def _scene(name):
+    return {
+        "paths1024": scenes.headline,
+        "mixed512": scenes.config2,
+        "prefiltered1024": scenes.config3,
+        "paths2048": scenes.config5,
+    }[name]()
+
+
+# (scene, forward bands, backward band or None); the bands are sized so that
+# the oracle finishes each in seconds (the prefiltered CPU backward costs
+# about 8 ms a sample, the 2048^2 CPU backward over a minute whatever the band)
+CASES = [
+    ("paths1024", [(0, 8), (504, 520), (1016, 1024)], (504, 520)),
+    ("mixed512", [(0, 6), (248, 264), (506, 512)], (248, 264)),
+    ("prefiltered1024", [(500, 516)], (511, 512)),
+    ("paths2048", [(1023, 1025)], None),
+]
+
+
+def _band_adjoint(img, tgt, W, H, band):
+    y0, y1 = band
+    dr = np.zeros((H, W, 4), np.float32)
+    dr[y0:y1] = 2.0 * (img.reshape(H, W, 4)[y0:y1] - tgt.reshape(H, W, 4)[y0:y1])
+    return dr.reshape(-1)
+
+
+@pytest.mark.parametrize("name,bands,bwd_band", CASES, ids=[c[0] for c in CASES])
+def test_fullsize_band_matches_oracle(renderer, oracle, name, bands, bwd_band):
+    sc, opts = _scene(name)
+    W, H = sc.width, sc.height
+    print(f"case {name} {W}x{H} {len(sc.shapes)} shapes", flush=True)
+    img = renderer.render(sc, opts).reshape(H, W, 4)
+    desc = SceneDesc(sc)
+    for y0, y1 in bands:
+        t = time.time()
+        ref = oracle.render_rows(desc, opts, y0, y1)
+        frac, mx = image_report(img[y0:y1], ref, IMG_TOL)
+        print(f"{name} rows {y0}..{y1}: pixels beyond {IMG_TOL}: {frac:.2e}, max |diff| {mx:.3g} "
+              f"(oracle {time.time() - t:.1f} s)", flush=True)
+        assert frac <= FLIP_FRACTION
+    if bwd_band is None:
+        return
+    tgt = renderer.render(scenes.jittered(sc, 2.0, seed=9), opts)
+    d_render = _band_adjoint(img, tgt, W, H, bwd_band)
+    assert np.abs(d_render).max() > 0
+    g = renderer.render_backward(sc, opts, d_render)
+    t = time.time()
+    gref = oracle.render_backward_rows(desc, opts, d_render, *bwd_band)
+    total, per = grad_report(g, gref)
+    print(f"{name} backward rows {bwd_band}: gradient rel L2 {total:.3g} (oracle {time.time() - t:.1f} s); "
+          f"per field {per}", flush=True)
+    assert total <= GRAD_REL
+
+
+@pytest.mark.parametrize("name", ["paths1024", "paths2048"])
+def test_fullsize_backward_is_linear(renderer, name):
+    """grad(d1 + 2 d2) = grad(d1) + 2 grad(d2) for the whole backward pass
+    (interior and boundary samples); only the order of the float atomic sums
+    differs between the runs."""
+    sc, opts = _scene(name)
+    W, H = sc.width, sc.height
+    print(f"case {name} linearity", flush=True)
+    rng = np.random.default_rng(5)
+    img = renderer.render(sc, opts)
+    tgt = renderer.render(scenes.jittered(sc, 2.0, seed=9), opts)
+    d1 = (2.0 * (img - tgt)).astype(np.float32)
+    d2 = rng.normal(0.0, 0.05, size=W * H * 4).astype(np.float32)
+    g1 = renderer.render_backward(sc, opts, d1)
+    g2 = renderer.render_backward(sc, opts, d2)
+    g12 = renderer.render_backward(sc, opts, (d1 + 2.0 * d2).astype(np.float32))
+    comb = {k: g1[k] + 2.0 * g2[k] for k in GRAD_FIELDS}
+    total, per = grad_report(g12, comb)
+    print(f"{name}: linearity rel L2 {total:.3g}", flush=True)
+    assert total <= 1e-4
+    for k in GRAD_FIELDS:
+        assert np.all(np.isfinite(g12[k])), k
+
+
+@pytest.mark.parametrize("name", ["paths1024", "paths2048", "prefiltered1024"])
+def test_fullsize_render_repeats(renderer, name):
+    """Two renders of one scene agree (the splat adds by float atomics: up to
+    rounding) and every pixel is an opaque colour in [0, 1] over the white
+    background."""
+    sc, opts = _scene(name)
+    print(f"case {name} repeat", flush=True)
+    a = renderer.render(sc, opts).reshape(-1, 4)
+    b = renderer.render(sc, opts).reshape(-1, 4)
+    assert np.abs(a - b).max() <= 1e-5
+    assert np.all(np.isfinite(a))
+    assert a.min() >= -1e-6 and a.max() <= 1.0 + 1e-5
+    assert np.allclose(a[:, 3], 1.0, atol=1e-5)
