@@ -497,7 +497,7 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
 }
 
 // ---- the backward's cubic inside test on ill-conditioned segments -------------
-// The reference CPU (distance/winding.rs:136-188) counts the roots t in [0, 1]
+// The reference CPU (distance/winding.rs:166-188) counts the roots t in [0, 1]
 // of P(t) = a t^3 + b t^2 + c t + (y0 - py), solved in f64
 // (distance/utils.rs:82-112), whose curve point lies strictly right of px,
 // signed by P'(t).  On a segment with record bit 4 the same roots are found

@@ -2854,7 +2854,7 @@ __global__ void k_selftest_winding(uint32_t n, uint32_t seed, uint32_t *__restri
 }
 
 // ---- self-test of the backward's inside test ------------------------------
-// The reference CPU's cubic winding (distance/winding.rs:136-188 over
+// The reference CPU's cubic winding (distance/winding.rs:166-188 over
 // distance/utils.rs:44-123) restated in f64 with the same steps, the checker
 // of winding_cubic_bwd.
 DVG_DEV double cbrt_f64(double x) {  // utils.rs:115-123
@@ -2914,7 +2914,7 @@ DVG_DEV int cubic_roots_f64(double a, double b, double c, double d, double t[3])
     return 1;
 }
 
-DVG_DEV int winding_cubic_f64(float px, float py, const float *x, const float *y) {  // winding.rs:136-188
+DVG_DEV int winding_cubic_f64(float px, float py, const float *x, const float *y) {  // winding.rs:166-188
     double a = (double)(-y[0] + 3.0f * y[1] - 3.0f * y[2] + y[3]);
     double b = (double)(3.0f * y[0] - 6.0f * y[1] + 3.0f * y[2]);
     double c = (double)(-3.0f * y[0] + 3.0f * y[1]);

@@ -484,7 +484,7 @@ int32_t dvg_batch_autotune(dvg_ctx *ctx, dvg_batch *batch, uint32_t flags, int r
 int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *mismatches);
 
 /* Self-test of the backward's cubic inside test, which follows the
- * reference CPU's f64 winding (distance/winding.rs:136-188): n random
+ * reference CPU's f64 winding (distance/winding.rs:166-188): n random
  * segment / point pairs in four families (random; near-quadratic cubics;
  * points near a y-extremum; points within 0.01 px of the curve) against an
  * f64 restatement of that formula.  counts[f] (f < 4) receives the pairs of

@@ -37,7 +37,7 @@ INSIDE_BOUNDS = (2e-6, 2e-6, 1e-4, 2e-3)
 def test_backward_inside_test_follows_f64_winding(renderer, seed):
     """The backward's cubic inside test (the record solve, or the monotone
     pieces on ill-conditioned segments) against the reference CPU's f64
-    winding (distance/winding.rs:136-188), restated on the device."""
+    winding (distance/winding.rs:166-188), restated on the device."""
     n = 1 << 24
     counts = (C.c_uint32 * 8)()
     print(f"case inside self-test seed {seed}", flush=True)
