@@ -26,6 +26,9 @@ namespace dvgk {
 // scene, 7 waves (<= 72 VGPRs) is best for the forward, the boundary pass
 // (4.5 -> 3.7 ms from 4 to 7, slower again at 8) and the colour adjoint
 // (1.64 -> 1.38 ms from 4 to 7, with its LDS cut to 18 KiB a workgroup).
+#ifndef OCC_FORWARD_PF
+#define OCC_FORWARD_PF 7
+#endif
 #ifndef OCC_FORWARD
 #define OCC_FORWARD 7
 #endif
@@ -604,7 +607,7 @@ DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restr
 }
 
 template <bool PF>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_forward(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PF ? OCC_FORWARD_PF : OCC_FORWARD))) void k_forward(
     DevScene S, const float *__restrict__ weight, float *__restrict__ accum) {
     forward_unit<PF>(S, blockIdx.x, weight, accum);
 }
@@ -3251,7 +3254,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch_weights(DVG_QUEUE_ARGS) {
 }
 
 template <bool PF>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_FORWARD))) void k_batch_forward(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PF ? OCC_FORWARD_PF : OCC_FORWARD))) void k_batch_forward(
     DVG_QUEUE_ARGS) {
     DVG_QUEUE_LOOP(forward_unit<PF>(B.S, w.unit, B.weight, B.accum))
 }
