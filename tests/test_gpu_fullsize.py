@@ -1,10 +1,11 @@
 """GPU parity at the full workload sizes.
 
 The scenes here are the benchmark workloads themselves: 1024^2 with 1000
-cubic paths (2x2 jittered samples), 512^2 with 512 mixed shapes (tent
-filter), 1024^2 prefiltered strokes and polygons, 2048^2 with 10000 cubic
-paths.  The oracle needs minutes for one of these images, so it is asked for
-a band of pixel rows only (oracle_render_rows; its band equals the
+cubic paths (2x2 jittered samples), 256^2 with 64 canvas-wide cubic paths
+(4x4 jittered samples), 512^2 with 512 mixed shapes (tent filter), 1024^2
+prefiltered strokes and polygons, 2048^2 with 10000 cubic paths.  The
+oracle needs minutes for one of these images, so it is asked for a band of
+pixel rows only (oracle_render_rows; its band equals the
 whole-image result bit for bit, which the CPU tests check): the GPU renders
 the whole image and its band is compared.  For the gradients d_render is
 the L2 loss adjoint against a jittered copy of the scene, kept inside the
@@ -37,6 +38,7 @@ GRAD_REL = 1e-3
 def _scene(name):
     return {
         "paths1024": scenes.headline,
+        "paths256_spp16": scenes.config1,
         "mixed512": scenes.config2,
         "prefiltered1024": scenes.config3,
         "paths2048": scenes.config5,
@@ -48,6 +50,7 @@ def _scene(name):
 # about 8 ms a sample, the 2048^2 CPU backward over a minute whatever the band)
 CASES = [
     ("paths1024", [(0, 8), (504, 520), (1016, 1024)], (504, 520)),
+    ("paths256_spp16", [(0, 8), (120, 136), (248, 256)], (120, 136)),
     ("mixed512", [(0, 6), (248, 264), (506, 512)], (248, 264)),
     ("prefiltered1024", [(500, 516)], (511, 512)),
     ("paths2048", [(1023, 1025)], None),
