@@ -255,59 +255,11 @@ DVG_DEV void winding_line_cpu(float px, float py, float x0, float y0, float x1, 
     }
 }
 
-// curves.rs:127-166
-DVG_DEV void winding_quad(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
-                          uint32_t rule, float &w, float &cr) {
-    float ay = y0 - 2.0f * y1 + y2;
-    float by = -2.0f * y0 + 2.0f * y1;
-    float cy = y0 - py;
-    float r[2];
-    int n = solve_quadratic(ay, by, cy, r[0], r[1]);
-    for (int i = 0; i < n; ++i) {
-        float t = r[i];
-        if (t >= 0.f && t <= 1.f) {
-            float tt = 1.f - t;
-            float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
-            if (x > px) {
-                if (rule == 1u) cr = 1.f - cr;
-                else {
-                    float dy = 2.0f * ay * t + by;
-                    w += (dy > 0.f) ? 1.f : -1.f;
-                }
-            }
-        }
-    }
-}
-
-// curves.rs:170-214
-DVG_DEV void winding_cubic(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
-                           float x3, float y3, uint32_t rule, float &w, float &cr) {
-    float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
-    float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
-    float c = -3.0f * y0 + 3.0f * y1;
-    float d = y0 - py;
-    float r[3];
-    int n = solve_cubic(a, b, c, d, r);
-    for (int i = 0; i < n; ++i) {
-        float t = r[i];
-        if (t >= 0.f && t <= 1.f) {
-            float tt = 1.f - t;
-            float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
-            if (x > px) {
-                if (rule == 1u) cr = 1.f - cr;
-                else {
-                    float dy = 3.0f * a * t * t + 2.0f * b * t + c;
-                    w += (dy > 0.f) ? 1.f : -1.f;
-                }
-            }
-        }
-    }
-}
-
-// Per-segment terms of the winding tests, computed once per segment
-// (k_wind_coef) with exactly the operations and order winding_quad /
-// winding_cubic / solve_cubic apply to them, so that the per-sample tests
-// below give bit-identical results while doing only the point-dependent part.
+// Per-segment terms of the winding tests of curves.rs:127-214 (quadratic and
+// cubic segments, with solve_cubic of curves.rs:897-946), computed once per
+// segment (k_wind_coef) with exactly the operations and order the reference
+// applies to them, so that the per-sample tests below give bit-identical
+// results while doing only the point-dependent part.
 // Record of WIND_STRIDE floats:
 //   cubic: a, b, c, q = (bb^2 - 3 cc) / 9, P = 2 bb^3 - 9 bb cc, q3 = q^3,
 //          sqrt(q3), sqrt(q), bb / 3, 3a, 2b   (bb = b / a, cc = c / a),
@@ -400,7 +352,8 @@ DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
     for (int k = 16; k < WIND_STRIDE; ++k) o[k] = 0.f;
 }
 
-// winding_quad with the segment's terms from its record (same result).
+// The quadratic winding test of curves.rs:127-166 with the segment's terms
+// from its record.
 DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, float x2, const float *wc,
                             uint32_t rule, float &w, float &cr) {
     float ay = wc[0], by = wc[1];
@@ -430,7 +383,8 @@ DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, fl
     }
 }
 
-// winding_cubic with the segment's terms from its record (same result).
+// The cubic winding test of curves.rs:170-214 with the segment's terms from
+// its record.
 // FAST: the fast transcendentals on well-conditioned segments (record bit 3).
 // The forward keeps the f32 library's: the fast ones move crossings by up to
 // 1e-3 px, which flips about 0.1 % of the pixels of the 2048^2 scene of
