@@ -60,7 +60,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
+    ap.add_argument("--exact-boundary", action="store_true",
+                    help="exact boundary mode: uncertain boundary samples again in f64 (single-scene workloads)")
     args = ap.parse_args()
+    if args.exact_boundary and args.workload == "c4":
+        print(json.dumps({"error": "the exact boundary mode covers single-scene passes, not the batch (c4)"}))
+        sys.exit(2)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -78,6 +83,8 @@ def main():
     torch.cuda.set_stream(stream)
     r = Renderer(0)
     r.set_stream(stream.cuda_stream)
+    if args.exact_boundary:
+        r.set_exact_boundary(True)
 
     if args.workload == "c4":
         return bench_batch(args, r, stream, dev)
@@ -216,7 +223,8 @@ def main():
         "metric": METRIC if headline else f"fwd+bwd pixel-samples/s ({args.workload})", "value": value, "unit": "pixel-samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": workload_desc, "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu"},
+        "config": {"workload": workload_desc, "global_batch": 1, "samples_per_step": n_samples, "parallelism": "single-gpu",
+                   "exact_boundary": args.exact_boundary},
         "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None, "adam": adam,
         "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
         "roofline": roofline, "cpu_baseline": cpu,

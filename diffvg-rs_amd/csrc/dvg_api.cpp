@@ -61,13 +61,26 @@ int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *
     return DVG_OK;
 }
 
+int32_t dvg_ctx_set_exact_boundary(dvg_ctx *ctx, int enable) {
+    if (!ctx) return DVG_INVALID_SCENE;
+    if (enable && !ctx->side) {
+        // the redo's stream (default priority) and its fork / join events
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        if (!ctx->ev_fork) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        if (!ctx->ev_join) HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    }
+    ctx->exact_boundary = enable != 0;
+    return DVG_OK;
+}
+
 int32_t dvg_selftest_inside(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *counts) {
     if (!ctx || !counts) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     uint32_t *d = nullptr;
-    HIP_TRY(ctx, hipMalloc(&d, 8 * sizeof(uint32_t)));
+    HIP_TRY(ctx, hipMalloc(&d, 12 * sizeof(uint32_t)));
     dvgk::launch_selftest_inside(n, seed, d, ctx->stream);
-    hipError_t e = hipMemcpyAsync(counts, d, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e = hipMemcpyAsync(counts, d, 12 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return hip_err(ctx, e, "inside-test self-test");
@@ -106,6 +119,12 @@ void dvg_ctx_destroy(dvg_ctx *ctx) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
@@ -731,13 +750,18 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         }
         iw = {ctx->imask + 1, ctx->imask, ds->gcap, slots};
     }
-    {
+    // exact boundary mode: the boundary pass first, its uncertain samples
+    // again in f64 on the side stream beside the colour adjoint
+    bool exact = ctx->exact_boundary && S.n_cdf > 0;
+    if (!exact) {
         KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
         dvgk::launch_backward_interior(S, ctx->weight, d_render, G, iw, ctx->stream);
     }
+    bool redo = false;
     if (S.n_cdf > 0) {
         size_t ns = npix * S.spp, nt = dvgk::boundary_buckets(S);
-        size_t need = 3 * ns + 2 * nt + 3;
+        size_t units = (ns + dvgk::BLOCK - 1) / dvgk::BLOCK;
+        size_t need = 3 * ns + 2 * nt + 3 + (exact ? units * (1 + dvgk::FLAG_UNIT_SLOTS) : 0);
         if (ctx->bwork_cap < need) {
             if (ctx->bwork) (void)hipFree(ctx->bwork);
             ctx->bwork = nullptr;
@@ -751,6 +775,8 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         bw.order = bw.ranks + ns;
         bw.counts = bw.order + ns;
         bw.offsets = bw.counts + nt + 1;
+        bw.flag_count = exact ? bw.offsets + nt + 2 : nullptr;
+        bw.flag_slots = exact ? bw.flag_count + units : nullptr;
         {
             KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_SORT);
             dvgk::launch_boundary_sort(S, bw, ctx->stream);
@@ -759,7 +785,21 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
             KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_EVAL);
             dvgk::launch_boundary_eval(S, ctx->weight, d_render, G, bw, ctx->stream);
         }
+        if (exact) {
+            // a few hundred latency-bound waves: they run beside the colour
+            // adjoint instead of after it
+            HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            dvgk::launch_boundary_redo(S, ctx->weight, d_render, G, bw, ctx->side);
+            HIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+            redo = true;
+        }
     }
+    if (exact) {
+        KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
+        dvgk::launch_backward_interior(S, ctx->weight, d_render, G, iw, ctx->stream);
+    }
+    if (redo) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }

@@ -269,7 +269,9 @@ DVG_DEV void winding_line_cpu(float px, float py, float x0, float y0, float x1, 
 //          ray test's box, as the walk would compute it)
 //   cubic: [16..20] = the monotone pieces of y(t) - y0 = a t^3 + b t^2 + c t
 //          on [0, 1]: its critical points t1 <= t2 in (0, 1) (1 when absent)
-//          and its values Y(t1), Y(t2), Y(1), computed in f64 and rounded
+//          and its values Y(t1), Y(t2), Y(1), computed in f64 and rounded;
+//          [21] = 1 / (54 sqrt(q3)), [22] = 2 sqrt(q) + |bb / 3| + 1 (error
+//          bounds of the backward's checked solve)
 //
 // Root mask: in the three-root case root k is -2 sqrt(q) cos(phi_k) - bb/3
 // with phi_0 in [0, pi/3], phi_1 in [2pi/3, pi], phi_2 in [-2pi/3, -pi/3]
@@ -339,6 +341,10 @@ DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
     o[18] = (float)(((A * t1 + B) * t1 + C) * t1);
     o[19] = (float)(((A * t2 + B) * t2 + C) * t2);
     o[20] = (float)(A + B + C);
+    // the checked solve's per-segment terms: 1 / (54 sqrt(q3)), 2 sqrt(q) + |b / 3a| + 1
+    o[21] = 1.0f / (54.0f * o[6]);
+    o[22] = 2.0f * fabsf(sq) + fabsf(off) + 1.0f;
+    o[23] = 0.f;
 }
 
 DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
@@ -353,13 +359,17 @@ DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
 }
 
 // The quadratic winding test of curves.rs:127-166 with the segment's terms
-// from its record.
+// from its record.  CHK: raise `uns` where the reference CPU's f64 solve
+// (winding.rs:135-165) could decide differently.
+template <bool CHK = false>
 DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, float x2, const float *wc,
-                            uint32_t rule, float &w, float &cr) {
+                            uint32_t rule, float &w, float &cr, float *uns = nullptr) {
     float ay = wc[0], by = wc[1];
     float cy = y0 - py;
     // solve_quadratic(ay, by, cy) with b*b and 4*a taken from the record
     float disc = wc[3] - wc[4] * cy;
+    if (CHK && !(absf_(disc) > 1.0e-5f * (wc[3] + absf_(wc[4] * cy)))) *uns = 1.f;
+    if (CHK && !(absf_(ay) >= 1.0e-6f)) *uns = 1.f;
     if (disc < 0.f) return;
     float root = sqrtf(disc);
     float qq = (by < 0.f) ? -0.5f * (by - root) : -0.5f * (by + root);
@@ -369,9 +379,11 @@ DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, fl
     else { r[0] = t0; r[1] = t1; }
     for (int i = 0; i < 2; ++i) {
         float t = r[i];
+        if (CHK && (absf_(t) <= 1.0e-5f || absf_(t - 1.f) <= 1.0e-5f)) *uns = 1.f;
         if (t >= 0.f && t <= 1.f) {
             float tt = 1.f - t;
             float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
+            if (CHK && absf_(x - px) <= 1.0e-3f) *uns = 1.f;
             if (x > px) {
                 if (rule == 1u) cr = 1.f - cr;
                 else {
@@ -389,16 +401,43 @@ DVG_DEV void winding_quad_w(float px, float py, float x0, float y0, float x1, fl
 // The forward keeps the f32 library's: the fast ones move crossings by up to
 // 1e-3 px, which flips about 0.1 % of the pixels of the 2048^2 scene of
 // 10000 paths; the backward's inside tests take them.
-template <bool FAST>
+//
+// CHK: also raise `uns` when the reference CPU's f64 solve could decide
+// differently (a root within its f32 error bound of 0 or 1, a crossing
+// within its error bound of px, two roots about to merge).  The bound is the
+// root formula's error, KT (below) per unit of 2 sqrt(q) + |b / 3a| + 1, with
+// the acos argument's error amplified by 1 / sqrt(1 - rho^2) near a double
+// root.
+constexpr float WIND_KT_FAST = 3.0e-7f, WIND_KT_LIBM = 1.0e-7f;
+
+DVG_DEV float bezier_x(float t, float x0, float x1, float x2, float x3) {
+    float tt = 1.f - t;
+    return (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+}
+
+DVG_DEV float bezier_dx(float t, float x0, float x1, float x2, float x3) {
+    float tt = 1.f - t;
+    return 3.0f * ((tt * tt) * (x1 - x0) + (2.0f * tt * t) * (x2 - x1) + (t * t) * (x3 - x2));
+}
+
+// CHK also raises *pieces where Cardano's f32 roots cannot settle the count
+// (two roots about to merge, a root at a segment end): the caller then takes
+// the monotone-piece evaluation, whose sign tests at the piece ends are exact.
+template <bool FAST, bool CHK = false>
 DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, float x2, float x3, const float *wc,
-                             uint32_t rule, float &w, float &cr) {
+                             uint32_t rule, float &w, float &cr, float *uns = nullptr, bool *pieces = nullptr) {
     float a = wc[0];
     float d = y0 - py;
     float r[3];
     int n;
+    // CHK: the roots' error bound is et0 + amp (amp: the double-root term),
+    // evaluated only for roots near 0 or 1 or crossings near px
+    float et0 = 0.f, amp = 0.f;
+    bool risky = false;  // roots close to merging: no cheap pre-filter
     const float eps = 1.0e-6f;
     if (absf_(a) < eps) {
         n = solve_quadratic(wc[1], wc[2], d, r[0], r[1]);
+        if (CHK) *uns = 1.f;  // the backward takes these on the piece path
     } else {
         // solve_cubic (curves.rs:897-946) from the per-segment terms
         float dd = d / a;
@@ -414,6 +453,16 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
                 float sq = wc[7];
                 const float third = 1.0f / 3.0f;
                 const float two_pi = 2.0f * 3.14159265f;
+                if (CHK) {
+                    float gap = q3 - r2;  // q3 (1 - rho^2)
+                    float kt = (FAST && (mask & 8u)) ? WIND_KT_FAST : WIND_KT_LIBM;
+                    et0 = kt * wc[22];
+                    if (!(gap > 1.0e-5f * q3)) *pieces = true;  // two roots about to merge
+                    // acos argument error, amplified by 1 / sqrt(1 - rho^2)
+                    float erho = 2.4e-7f * ((absf_(wc[4]) + 27.0f * absf_(dd)) * wc[21] + 1.0f);
+                    amp = 0.667f * sq * erho * wc[6] * __builtin_amdgcn_rsqf(maxf_(gap, 1.0e-30f));
+                    risky = gap < 1.0e-3f * q3 || et0 + amp > 1.0e-4f;
+                }
                 if (FAST && (mask & 8u)) {
                     float theta = fast_acos(rr / wc[6]);
                     if (mask & 1u) r[0] = -2.0f * sq * __cosf(theta * third) - off;
@@ -432,13 +481,33 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
             float br = (absf_(ar) > eps) ? q / ar : 0.f;
             r[0] = (ar + br) - off;
             n = 1;
+            if (CHK) {
+                float gap = r2 - q3;
+                float kt = (FAST && ((uint32_t)wc[11] & 8u)) ? WIND_KT_FAST : WIND_KT_LIBM;
+                et0 = 4.0f * kt * (absf_(ar) + absf_(br) + absf_(off) + 1.0f);
+                if (!(gap > 1.0e-5f * r2)) *pieces = true;  // a complex pair about to turn real
+                // sqrt(r2 - q3)'s error near the three-root boundary
+                amp = 1.2e-7f * (absf_(ar) + absf_(br)) * absf_(rr) * __builtin_amdgcn_rsqf(maxf_(gap, 1.0e-30f));
+                risky = gap < 1.0e-3f * r2 || et0 + amp > 1.0e-4f;
+                if (absf_(absf_(ar) - eps) <= 4.0f * kt * absf_(ar)) *uns = 1.f;
+            }
         }
     }
     for (int i = 0; i < n; ++i) {
         float t = r[i];
+        if (CHK && (risky || absf_(t) <= 1.0e-2f || absf_(t - 1.f) <= 1.0e-2f)) {
+            float et = et0 + amp;
+            if (absf_(t) <= et || absf_(t - 1.f) <= et) *pieces = true;  // a root at a segment end
+        }
         if (t >= 0.f && t <= 1.f) {
             float tt = 1.f - t;
             float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
+            // (pre-filter: with et <= 1e-4 and |x'| <= 3 x 2048 the bound stays below 1 px)
+            if (CHK && (risky || absf_(x - px) <= 1.0f)) {
+                float xm = maxf_(maxf_(absf_(x0), absf_(x1)), maxf_(absf_(x2), absf_(x3)));
+                float ex = absf_(bezier_dx(t, x0, x1, x2, x3)) * (et0 + amp) + 1.2e-7f * (xm + absf_(px));
+                if (absf_(x - px) <= ex) *uns = 1.f;
+            }
             if (x > px) {
                 if (rule == 1u) cr = 1.f - cr;
                 else {
@@ -461,10 +530,6 @@ DVG_DEV void winding_cubic_w(float px, float py, float x0, float y0, float x1, f
 // polynomial converge to it; the piece's direction is the sign of P'.  A
 // cubic with |a| < 1e-6 is the CPU's quadratic b t^2 + c t + d
 // (utils.rs:44-79), whose double roots count twice.
-DVG_DEV float bezier_x(float t, float x0, float x1, float x2, float x3) {
-    float tt = 1.f - t;
-    return (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
-}
 
 DVG_DEV float piece_root(float a, float b, float c, float d, float lo, float hi, float vlo, float vhi) {
     // regula falsi start, then Newton kept inside the bracket [lo, hi]
@@ -485,12 +550,15 @@ DVG_DEV float piece_root(float a, float b, float c, float d, float lo, float hi,
     return t;
 }
 
+template <bool CHK>
 DVG_DEV int winding_cubic_pieces(float px, float py, float x0, float y0, float x1, float x2, float x3,
-                                 const float *wc) {
+                                 const float *wc, float &uns) {
     float a = wc[0], b = wc[1], c = wc[2];
     float d = y0 - py;
     int w = 0;
+    float xs = 1.2e-7f * (fabsf(x0) + fabsf(x1) + fabsf(x2) + fabsf(x3) + fabsf(px)) + 2.0e-4f;
     if (!(fabsf(a) >= 1.0e-6f)) {
+        if (CHK) uns = 1.f;  // the CPU's quadratic branch: rare, always taken exactly
         float r0, r1;
         int n = 0;
         if (fabsf(b) < 1.0e-12f) {
@@ -524,30 +592,165 @@ DVG_DEV int winding_cubic_pieces(float px, float py, float x0, float y0, float x
     }
     float tk[4] = {0.f, wc[16], wc[17], 1.f};
     float vk[4] = {d, wc[18] + d, wc[19] + d, wc[20] + d};
+    float ys = 1.0e-6f * (fabsf(wc[18]) + fabsf(wc[19]) + fabsf(wc[20]) + fabsf(d)) + 1.0e-6f;
+    if (CHK)
+        for (int k = 0; k < 4; ++k)
+            if (fabsf(vk[k]) <= ys) uns = 1.f;  // a root at a piece end: the CPU's rounding decides
     for (int k = 0; k < 3; ++k) {
         float va = vk[k], vb = vk[k + 1];
         if ((va < 0.f) == (vb < 0.f) || !(tk[k] < tk[k + 1])) continue;
         float t = piece_root(a, b, c, d, tk[k], tk[k + 1], va, vb);
-        if (bezier_x(t, x0, x1, x2, x3) > px) w += (vb > va) ? 1 : -1;
+        float x = bezier_x(t, x0, x1, x2, x3);
+        if (CHK && fabsf(x - px) <= xs) uns = 1.f;
+        if (x > px) w += (vb > va) ? 1 : -1;
     }
     return w;
 }
 
 // The backward's cubic inside test: the f32 record solve, or the monotone
 // pieces on segments whose Cardano solve is ill-conditioned (record bit 4).
+// CHK: `uns` is raised where the f64 solve could decide differently.
+template <bool CHK>
 DVG_DEV void winding_cubic_bwd(float px, float py, float x0, float y0, float x1, float x2, float x3,
-                               const float *wc, uint32_t rule, float &w, float &cr) {
+                               const float *wc, uint32_t rule, float &w, float &cr, float &uns) {
     if ((uint32_t)wc[11] & 16u) {
         // the even-odd rule toggles per crossing: the parity of the signed sum
-        int n = winding_cubic_pieces(px, py, x0, y0, x1, x2, x3, wc);
+        int n = winding_cubic_pieces<CHK>(px, py, x0, y0, x1, x2, x3, wc, uns);
         if (rule == 1u) {
             if (n & 1) cr = 1.f - cr;
         } else {
             w += (float)n;
         }
-    } else {
+    } else if (!CHK) {
         winding_cubic_w<true>(px, py, x0, y0, x1, x2, x3, wc, rule, w, cr);
+    } else {
+        float tw = 0.f, tc = 0.f, tu = 0.f;
+        bool pieces = false;
+        winding_cubic_w<true, true>(px, py, x0, y0, x1, x2, x3, wc, rule, tw, tc, &tu, &pieces);
+        if (pieces) {
+            int n = winding_cubic_pieces<true>(px, py, x0, y0, x1, x2, x3, wc, uns);
+            tw = (float)n;
+            tc = (n & 1) ? 1.f : 0.f;
+        } else if (tu > 0.f) {
+            uns = 1.f;
+        }
+        if (rule == 1u) {
+            if (tc > 0.f) cr = 1.f - cr;
+        } else {
+            w += tw;
+        }
     }
+}
+
+// ---- the reference CPU's winding in f64 -----------------------------------------
+// distance/winding.rs:118-192 over distance/utils.rs:44-123, restated in f64
+// with the same steps (no contraction: the library is built with
+// -ffp-contract=off).  The boundary samples whose f32 inside tests are
+// uncertain are evaluated again with these, and the self-test checks the f32
+// tests against them.
+DVG_DEV double cbrt_f64(double x) {  // utils.rs:115-123
+    if (x > 0.0) return pow(x, 1.0 / 3.0);
+    if (x < 0.0) return -pow(-x, 1.0 / 3.0);
+    return 0.0;
+}
+
+DVG_DEV int quadratic_roots_f64(double a, double b, double c, double t[2]) {  // utils.rs:44-79
+    const double eps = 1.0e-12;
+    if (fabs(a) < eps) {
+        if (fabs(b) < eps) return 0;
+        t[0] = t[1] = -c / b;
+        return 2;
+    }
+    double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) return 0;
+    double root = sqrt(disc);
+    if (fabs(root) < eps) {
+        t[0] = t[1] = -0.5 * b / a;
+        return 2;
+    }
+    double q = (b < 0.0) ? -0.5 * (b - root) : -0.5 * (b + root);
+    double lo, hi;
+    if (fabs(q) < eps) {
+        double inv = 0.5 / a;
+        lo = (-b - root) * inv;
+        hi = (-b + root) * inv;
+    } else {
+        lo = q / a;
+        hi = c / q;
+    }
+    t[0] = lo < hi ? lo : hi;
+    t[1] = lo < hi ? hi : lo;
+    return 2;
+}
+
+DVG_DEV int cubic_roots_f64(double a, double b, double c, double d, double t[3]) {  // utils.rs:82-112
+    if (fabs(a) < 1.0e-6) return quadratic_roots_f64(b, c, d, t);
+    double bn = b / a, cn = c / a, dn = d / a;
+    double q = (bn * bn - 3.0 * cn) / 9.0;
+    double r = (2.0 * bn * bn * bn - 9.0 * bn * cn + 27.0 * dn) / 54.0;
+    double q3 = q * q * q;
+    if (r * r < q3) {
+        const double pi = 3.14159265358979323846;
+        double theta = acos(r / sqrt(q3));
+        double s = -2.0 * sqrt(q);
+        t[0] = s * cos(theta / 3.0) - bn / 3.0;
+        t[1] = s * cos((theta + 2.0 * pi) / 3.0) - bn / 3.0;
+        t[2] = s * cos((theta - 2.0 * pi) / 3.0) - bn / 3.0;
+        return 3;
+    }
+    double disc = sqrt(r * r - q3);
+    double ar = (r > 0.0) ? -cbrt_f64(r + disc) : cbrt_f64(-r + disc);
+    double br = (fabs(ar) > 1.0e-6) ? q / ar : 0.0;
+    t[0] = (ar + br) - bn / 3.0;
+    return 1;
+}
+
+DVG_DEV int winding_cubic_f64(float px, float py, const float *x, const float *y) {  // winding.rs:166-188
+    double a = (double)(-y[0] + 3.0f * y[1] - 3.0f * y[2] + y[3]);
+    double b = (double)(3.0f * y[0] - 6.0f * y[1] + 3.0f * y[2]);
+    double c = (double)(-3.0f * y[0] + 3.0f * y[1]);
+    double d = (double)(y[0] - py);
+    double t[3];
+    int n = cubic_roots_f64(a, b, c, d, t);
+    double ax = (double)(-x[0] + 3.0f * x[1] - 3.0f * x[2] + x[3]);
+    double bx = (double)(3.0f * x[0] - 6.0f * x[1] + 3.0f * x[2]);
+    double cx = (double)(-3.0f * x[0] + 3.0f * x[1]);
+    double dx = (double)(x[0] - px);
+    int w = 0;
+    for (int i = 0; i < n; ++i) {
+        double s = t[i];
+        if (s >= 0.0 && s <= 1.0) {
+            double tp = ax * s * s * s + bx * s * s + cx * s + dx;
+            if (tp > 0.0) w += (3.0 * a * s * s + 2.0 * b * s + c > 0.0) ? 1 : -1;
+        }
+    }
+    return w;
+}
+
+// winding.rs:135-165 (quadratic segment)
+DVG_DEV int winding_quad_f64(float px, float py, const float *x, const float *y) {
+    double a = (double)(y[0] - 2.0f * y[1] + y[2]);
+    double b = (double)(-2.0f * y[0] + 2.0f * y[1]);
+    double c = (double)(y[0] - py);
+    double t[2] = {0.0, 0.0};
+    int n = 0;
+    if (fabs(a) < 1.0e-12) {
+        if (fabs(b) > 1.0e-12) { t[0] = -c / b; n = 1; }
+    } else {
+        n = quadratic_roots_f64(a, b, c, t);
+    }
+    double ax = (double)(x[0] - 2.0f * x[1] + x[2]);
+    double bx = (double)(-2.0f * x[0] + 2.0f * x[1]);
+    double cx = (double)(x[0] - px);
+    int w = 0;
+    for (int i = 0; i < n; ++i) {
+        double s = t[i];
+        if (s >= 0.0 && s <= 1.0) {
+            double tp = ax * s * s + bx * s + cx;
+            if (tp > 0.0) w += (2.0 * a * s + b > 0.0) ? 1 : -1;
+        }
+    }
+    return w;
 }
 
 DVG_DEV float det2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }

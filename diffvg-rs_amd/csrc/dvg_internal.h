@@ -41,6 +41,11 @@ struct dvg_ctx {
     uint32_t *imask = nullptr;
     size_t imask_cap = 0;
     uint32_t n_cu = 0;
+    // second stream (default priority) for the boundary redo, which runs
+    // beside the colour adjoint; fork / join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool exact_boundary = false;  // dvg_ctx_set_exact_boundary
 };
 
 struct dvg_device_scene {

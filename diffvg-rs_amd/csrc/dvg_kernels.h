@@ -135,10 +135,20 @@ __host__ __device__ inline uint32_t boundary_buckets(const DevScene &S) {
 // sample, counts buckets+1, offsets buckets+2.
 struct BoundaryWork {
     uint32_t *keys, *ranks, *order, *counts, *offsets;
+    // Samples whose f32 inside tests are uncertain (a query point within
+    // rounding distance of an edge), left by the pass to a redo with f64
+    // windings: per unit of BLOCK bucket-ordered samples, a count
+    // (flag_count[unit]) and FLAG_UNIT_SLOTS sample indices.
+    uint32_t *flag_count, *flag_slots;
 };
+constexpr uint32_t FLAG_UNIT_SLOTS = 32;
 // Boundary samples: tile keys, counting sort; then the tile-ordered evaluation.
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream);
 void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                          const BoundaryWork &bw, hipStream_t stream);
+// The samples launch_boundary_eval left uncertain, with f64 windings (after
+// it, in stream order).
+void launch_boundary_redo(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                           const BoundaryWork &bw, hipStream_t stream);
 // Zero `count` (<= 14) device float arrays of the given lengths.
 void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_t stream);
@@ -161,9 +171,9 @@ struct UpdatePlan {
 // Counts into *mismatches (device) the random cubic segment / point pairs whose
 // winding from the per-segment record differs from the libm formula.
 void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hipStream_t stream);
-// counts[0..8): per family of random pairs, the backward inside tests that
-// differ from the reference CPU's f64 winding, then the pairs solved on
-// monotone pieces.
+// counts[0..12): per family of random pairs, the checked backward inside
+// tests that differ from the reference CPU's f64 winding without a flag, the
+// pairs flagged, and the unchecked tests that differ.
 void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStream_t stream);
 // Per-segment winding terms of curve_data (13 floats a segment) into
 // curve_wind; after every change of curve_data.

@@ -218,9 +218,18 @@ DVG_DEV float4 paint_cpu(const DevScene &S, uint32_t kind, uint32_t gidx, const 
 }
 
 // ---- per-shape coverage --------------------------------------------------------
+// `unsure` (backward inside tests of the boundary pass): raised where the
+// reference CPU's f64 solve could decide differently.
 struct Cov {
-    float fill_min, winding, crossings, stroke_min, stroke_r, stroke_hit;
+    float fill_min, winding, crossings, stroke_min, stroke_r, stroke_hit, unsure;
 };
+
+// Which inside tests a coverage walk runs (template argument M):
+//   COV_FWD   the reference GPU forward's (forward.rs);
+//   COV_CPU   the reference CPU backward's (distance/*.rs) in f32;
+//   COV_CHK   COV_CPU raising Cov::unsure where f64 could decide otherwise;
+//   COV_EXACT COV_CPU with the fill windings in f64 as the CPU computes them.
+enum : int { COV_FWD = 0, COV_CPU = 1, COV_CHK = 2, COV_EXACT = 3 };
 
 DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, float r3) {
     if (kind == 0) return r0 + t * (r1 - r0);
@@ -234,9 +243,10 @@ DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, f
 // winding follows the reference CPU's winding_number_segment
 // (distance/winding.rs:118-192): the closed-interval line rule and the f64
 // cubic solve on ill-conditioned segments.
-template <bool CPU>
+template <int M>
 DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool prefilter, float px, float py,
                         Cov &c) {
+    constexpr bool CPU = M != COV_FWD;
     uint32_t kind = (uint32_t)sd[0];
     float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
     if (kind != DVG_SHAPE_PATH) {
@@ -288,10 +298,27 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         if (k == 0) {
             if (CPU) winding_line_cpu(px, py, x0, y0, x1, y1, rule, lw, lc);
             else winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
+        } else if (M == COV_EXACT) {
+            // the checked f32 test, and the f64 one where it is uncertain
+            float tw = 0.f, tc = 0.f, tu = 0.f;
+            if (k == 1) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, tw, tc, &tu);
+            else winding_cubic_bwd<true>(px, py, x0, y0, x1, x2, x3, wc, rule, tw, tc, tu);
+            int n = (int)tw;
+            bool odd = tc > 0.f;
+            if (tu > 0.f) {
+                float xs[4] = {x0, x1, x2, x3}, ys[4] = {y0, y1, y2, y3};
+                n = (k == 1) ? winding_quad_f64(px, py, xs, ys) : winding_cubic_f64(px, py, xs, ys);
+                odd = (n & 1) != 0;
+            }
+            if (rule == 1u) {
+                if (odd) lc = 1.f - lc;
+            } else {
+                lw += (float)n;
+            }
         } else if (k == 1) {
-            winding_quad_w(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
+            winding_quad_w<M == COV_CHK>(px, py, x0, y0, x1, x2, wc, rule, lw, lc, &c.unsure);
         } else if (CPU) {
-            winding_cubic_bwd(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
+            winding_cubic_bwd<M == COV_CHK>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc, c.unsure);
         } else {
             winding_cubic_w<false>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
         }
@@ -308,8 +335,9 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
 // uses the exact closest point of src/distance/shape.rs:117-164 and a shape
 // needs stroke_width > 0, as the CPU query within_distance_bvh does
 // (distance/api.rs:79-135).
-template <bool CPU>
+template <int M>
 DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, float px, float py, Cov &c) {
+    constexpr bool CPU = M != COV_FWD;
     uint32_t kind = (uint32_t)sd[0];
     float sw = sd[3];
     uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
@@ -395,9 +423,10 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
 // With CPU = true the fill follows is_inside_bvh (distance/api.rs:259-309):
 // the group's root box gates the test and then every shape's winding counts
 // (an open path can wind outside its own box).
-template <bool CPU>
+template <int M>
 DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx, float ly, bool do_fill,
                        bool do_stroke, uint32_t rule, bool prefilter, Cov &c) {
+    constexpr bool CPU = M != COV_FWD;
     uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
     if (CPU && do_fill) {
         const float *rb = S.group_cpu_bounds + g * 4u;
@@ -408,7 +437,7 @@ DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx,
                 const float *xf = S.shape_xform + si * 6u;
                 float spx = xf[0] * lx + xf[1] * ly + xf[2];
                 float spy = xf[3] * lx + xf[4] * ly + xf[5];
-                shape_fill<true>(S, S.shape_data + si * 15u, rule, false, spx, spy, c);
+                shape_fill<M>(S, S.shape_data + si * 15u, rule, false, spx, spy, c);
             }
         }
         do_fill = false;
@@ -423,8 +452,8 @@ DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx,
         float spx = xf[0] * lx + xf[1] * ly + xf[2];
         float spy = xf[3] * lx + xf[4] * ly + xf[5];
         const float *sd = S.shape_data + si * 15u;
-        if (do_fill) shape_fill<false>(S, sd, rule, prefilter, spx, spy, c);
-        if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<CPU>(S, sd, prefilter, spx, spy, c);
+        if (do_fill) shape_fill<COV_FWD>(S, sd, rule, prefilter, spx, spy, c);
+        if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<M>(S, sd, prefilter, spx, spy, c);
     }
 }
 
@@ -456,8 +485,8 @@ DVG_DEV float4 eval_forward(const DevScene &S, const uint32_t *list, uint32_t n,
         if (fk == DVG_PAINT_NONE && sk == DVG_PAINT_NONE) continue;
         float lx, ly;
         group_local(S, g, px, py, lx, ly);
-        Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-        group_cov<false>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
+        Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+        group_cov<COV_FWD>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
         float inv_scale = S.group_inv_scale[g];
         float gscale = inv_scale > 0.f ? 1.f / inv_scale : 1.f;
         // blend_group, forward.rs:339-404: stroke first, then fill
@@ -743,17 +772,19 @@ DVG_DEV float filter_grad(const DevScene &S, const float *weight, const float *d
 
 // Non-prefiltered fragment tests of one group (sampling.rs:95-118 with the
 // CPU queries): stroke hit, then fill inside.
-DVG_DEV void group_frags(const DevScene &S, uint32_t g, float px, float py, bool &stroke, bool &fill) {
+template <int M = COV_CPU>
+DVG_DEV void group_frags(const DevScene &S, uint32_t g, float px, float py, bool &stroke, bool &fill, bool *uns = nullptr) {
     const float *gd = S.group_data + g * 16u;
     uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
     stroke = fill = false;
     if (fk == DVG_PAINT_NONE && sk == DVG_PAINT_NONE) return;
     float lx, ly;
     group_local(S, g, px, py, lx, ly);
-    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-    group_cov<true>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, false, c);
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+    group_cov<M>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, false, c);
     stroke = sk != DVG_PAINT_NONE && c.stroke_hit > 0.f;
     fill = fk != DVG_PAINT_NONE && ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f);
+    if (M == COV_CHK && c.unsure > 0.f) *uns = true;
 }
 
 DVG_DEV float4 frag_color(const DevScene &S, uint32_t g, bool stroke, float px, float py) {
@@ -1490,8 +1521,8 @@ DVG_DEV bool group_inside(const DevScene &S, uint32_t g, float px, float py) {
     uint32_t rule = (uint32_t)gd[7];
     float lx, ly;
     group_local(S, g, px, py, lx, ly);
-    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-    group_cov<true>(S, g, gd, lx, ly, true, false, rule, false, c);
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+    group_cov<COV_CPU>(S, g, gd, lx, ly, true, false, rule, false, c);
     return (rule == 1u) ? c.crossings > 0.f : c.winding != 0.f;
 }
 
@@ -2297,11 +2328,48 @@ DVG_DEV bool sample_boundary_path(const DevScene &S, uint32_t si, float path_len
 
 // Colour of one boundary query point with the EdgeQuery hit tracking of
 // sampling.rs:76-224 (no adjoint).  (qx, qy) is the normalized point.
-DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
-                              uint32_t eq_prep, bool &hit);
+// The tests one tile-list entry g makes for a boundary query: its stroke and
+// fill fragments and, for the sampled group, the EdgeQuery hit test on the
+// sampled shape (within_distance_edge / is_inside_edge, sampling.rs:485-549;
+// for a group of one shape that is the group's own test).  `uns`: a COV_CHK
+// test was uncertain.
+template <int M>
+DVG_DEV void group_edge_test(const DevScene &S, uint32_t g, float px, float py, uint32_t eq_group,
+                             uint32_t eq_prep, bool &st, bool &fi, bool &et, bool &uns) {
+    group_frags<M>(S, g, px, py, st, fi, &uns);
+    et = false;
+    if (g != eq_group) return;
+    const float *gd = S.group_data + g * 16u;
+    if ((uint32_t)gd[1] == 1u) {
+        et = st || fi;
+        return;
+    }
+    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+    float lx, ly;
+    group_local(S, g, px, py, lx, ly);
+    const float *xf = S.shape_xform + eq_prep * 6u;
+    float spx = xf[0] * lx + xf[1] * ly + xf[2];
+    float spy = xf[3] * lx + xf[4] * ly + xf[5];
+    const float *sd = S.shape_data + eq_prep * 15u;
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+    if (sk != DVG_PAINT_NONE) shape_stroke<M>(S, sd, false, spx, spy, c);
+    if (fk != DVG_PAINT_NONE) shape_fill<M>(S, sd, rule, false, spx, spy, c);
+    if (M == COV_CHK && c.unsure > 0.f) uns = true;
+    if (c.stroke_hit > 0.f) et = true;
+    if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
+        if (fk != DVG_PAINT_NONE) et = true;
+    }
+}
 
+template <int M>
+DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
+                              uint32_t eq_prep, bool &hit, bool &uns);
+
+// M: COV_CHK (raises `uns` where an f64 evaluation could decide otherwise),
+// COV_CPU or COV_EXACT.
+template <int M>
 DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint32_t eq_group, uint32_t eq_prep,
-                          bool &hit) {
+                          bool &hit, bool &uns) {
     float px = qx * (float)S.W, py = qy * (float)S.H;
     int tx = (int)floorf(px / (float)TILE), ty = (int)floorf(py / (float)TILE);
     tx = tx < 0 ? 0 : (tx >= (int)S.tiles_x ? (int)S.tiles_x - 1 : tx);
@@ -2311,12 +2379,13 @@ DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint
     // query one tile: then the walk runs with the tile in a scalar register
     // (list, group and curve data through scalar loads)
     uint32_t t0 = __builtin_amdgcn_readfirstlane(tile);
-    if (__ballot(tile != t0) == 0) return edge_walk_tile(S, t0, px, py, bg, eq_group, eq_prep, hit);
-    return edge_walk_tile(S, tile, px, py, bg, eq_group, eq_prep, hit);
+    if (__ballot(tile != t0) == 0) return edge_walk_tile<M>(S, t0, px, py, bg, eq_group, eq_prep, hit, uns);
+    return edge_walk_tile<M>(S, tile, px, py, bg, eq_group, eq_prep, hit, uns);
 }
 
+template <int M>
 DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
-                              uint32_t eq_prep, bool &hit) {
+                              uint32_t eq_prep, bool &hit, bool &uns) {
     uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
     float4 acc = bg;
     uint32_t nfr = 0;
@@ -2324,32 +2393,9 @@ DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float 
     bool h0 = false, h1 = true;  // compositing-loop hit for an initial false / true
     for (uint32_t e = lb; e < le; ++e) {
         uint32_t g = S.tile_entries[e];
-        bool st, fi;
-        group_frags(S, g, px, py, st, fi);
-        if (g == eq_group) {
-            // within_distance_edge / is_inside_edge (sampling.rs:485-549) on
-            // the sampled shape alone; for a group of one shape that is the
-            // group's own stroke / fill test above (bounds are conservative)
-            const float *gd = S.group_data + g * 16u;
-            if ((uint32_t)gd[1] == 1u) {
-                if (st || fi) edge_test = true;
-            } else {
-                uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-                float lx, ly;
-                group_local(S, g, px, py, lx, ly);
-                const float *xf = S.shape_xform + eq_prep * 6u;
-                float spx = xf[0] * lx + xf[1] * ly + xf[2];
-                float spy = xf[3] * lx + xf[4] * ly + xf[5];
-                const float *sd = S.shape_data + eq_prep * 15u;
-                Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f};
-                if (sk != DVG_PAINT_NONE) shape_stroke<true>(S, sd, false, spx, spy, c);
-                if (fk != DVG_PAINT_NONE) shape_fill<true>(S, sd, rule, false, spx, spy, c);
-                if (c.stroke_hit > 0.f) edge_test = true;
-                if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
-                    if (fk != DVG_PAINT_NONE) edge_test = true;
-                }
-            }
-        }
+        bool st, fi, et;
+        group_edge_test<M>(S, g, px, py, eq_group, eq_prep, st, fi, et, uns);
+        if (et) edge_test = true;
         for (int k = 0; k < 2; ++k) {
             bool is_stroke = k == 0;
             if (!(is_stroke ? st : fi)) continue;
@@ -2551,22 +2597,45 @@ DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0
 // Pass 3: boundary samples in tile order.  Consecutive lanes sit on the same
 // few shapes of one tile, so the tile's group list is shared by the wave and
 // the shape gradients are reduced per shape instance before the atomics.
+// Uncertain boundary samples (a query point within rounding distance of an
+// edge), per unit of the bucket-ordered pass: up to FLAG_SLOTS sample indices
+// a unit and their count (beyond it the f32 result stands).  The redo pass
+// reads a unit's slots together, so its lanes share the unit's tile and the
+// walks stay uniform.
+constexpr uint32_t FLAG_SLOTS = FLAG_UNIT_SLOTS;
+struct FlagSlots {
+    uint32_t *count, *slots;
+};
+
+// LIST = false, M = COV_CHK: the samples of unit u in bucket order; those
+// with uncertain tests are left to the redo (slots of unit u).  LIST = true:
+// the samples in unit u's slots, with the tests of M.
+template <int M, bool LIST>
 DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restrict__ weight,
                            const float *__restrict__ d_render, const DevGrads &G, const uint32_t *__restrict__ offsets,
-                           const uint32_t *__restrict__ order, uint32_t nb) {
+                           const uint32_t *__restrict__ order, uint32_t nb, FlagSlots fs) {
     const float PI = 3.14159274101257324219f;
     // (no staged CDF here: its LDS would cost a workgroup per CU)
     __shared__ BPoint s_pt[BLOCK];
+    __shared__ uint32_t s_nflag;
     uint32_t n_valid = offsets[nb];
     uint32_t pos = u * BLOCK + threadIdx.x;
     uint32_t lane = threadIdx.x & 63u;
     if (u * BLOCK >= n_valid) return;  // whole workgroup past the end
+    if (LIST) {
+        n_valid = min(fs.count[u], FLAG_SLOTS);
+        if (n_valid == 0) return;
+        pos = threadIdx.x;
+    } else if (M == COV_CHK) {
+        if (threadIdx.x == 0) s_nflag = 0;
+        __syncthreads();
+    }
     BPoint P;
     bool valid = false;
     float contrib = 0.f, nnx = 0.f, nny = 0.f;
     size_t pix = 0;
     if (pos < n_valid) {
-        size_t idx = order[pos];
+        size_t idx = LIST ? fs.slots[(size_t)u * FLAG_SLOTS + pos] : order[pos];
         valid = boundary_point(S, nullptr, idx, P);
         if (valid) {
             pix = (size_t)P.pyi * S.W + P.pxi;
@@ -2580,11 +2649,19 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
             // walks (the empty asm is a compiler barrier: P is reloaded)
             s_pt[threadIdx.x] = P;
             asm volatile("" ::: "memory");
-            float4 cin = edge_color(S, qix, qiy, bg, gi, prep, ih);
-            float4 cout = edge_color(S, qox, qoy, bg, gi, prep, oh);
+            bool uns = false;
+            float4 cin = edge_color<M>(S, qix, qiy, bg, gi, prep, ih, uns);
+            float4 cout = edge_color<M>(S, qox, qoy, bg, gi, prep, oh, uns);
             asm volatile("" ::: "memory");
             P = s_pt[threadIdx.x];
             valid = ih || oh;
+            if (M == COV_CHK && !LIST && uns) {
+                uint32_t slot = atomicAdd(&s_nflag, 1u);
+                if (slot < FLAG_SLOTS) {
+                    fs.slots[(size_t)u * FLAG_SLOTS + slot] = (uint32_t)idx;
+                    valid = false;
+                }
+            }
             nnx = P.ncx;
             nny = P.ncy;
             if (valid && !ih) {
@@ -2744,12 +2821,25 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
         }
         pending &= ~__ballot(mine);
     }
+    if (M == COV_CHK && !LIST) {
+        __syncthreads();
+        if (threadIdx.x == 0) fs.count[u] = min(s_nflag, FLAG_SLOTS);
+    }
 }
 
+template <int M>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval(
     DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
-    const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ order, uint32_t nb) {
-    bnd_eval_unit(S, blockIdx.x, weight, d_render, G, offsets, order, nb);
+    const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ order, uint32_t nb, FlagSlots fs) {
+    bnd_eval_unit<M, false>(S, blockIdx.x, weight, d_render, G, offsets, order, nb, fs);
+}
+
+// The samples k_bnd_eval left to the redo, unit by unit, with the f64
+// windings where the f32 ones are uncertain.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_redo(DevScene S, const float *__restrict__ weight,
+                                                    const float *__restrict__ d_render, DevGrads G,
+                                                    const uint32_t *__restrict__ offsets, uint32_t nb, FlagSlots fs) {
+    bnd_eval_unit<COV_EXACT, true>(S, blockIdx.x, weight, d_render, G, offsets, nullptr, nb, fs);
 }
 
 // Zero up to 14 float arrays in one launch (used instead of memset nodes so
@@ -2857,96 +2947,15 @@ __global__ void k_selftest_winding(uint32_t n, uint32_t seed, uint32_t *__restri
 }
 
 // ---- self-test of the backward's inside test ------------------------------
-// The reference CPU's cubic winding (distance/winding.rs:166-188 over
-// distance/utils.rs:44-123) restated in f64 with the same steps, the checker
-// of winding_cubic_bwd.
-DVG_DEV double cbrt_f64(double x) {  // utils.rs:115-123
-    if (x > 0.0) return pow(x, 1.0 / 3.0);
-    if (x < 0.0) return -pow(-x, 1.0 / 3.0);
-    return 0.0;
-}
-
-DVG_DEV int quadratic_roots_f64(double a, double b, double c, double t[2]) {  // utils.rs:44-79
-    const double eps = 1.0e-12;
-    if (fabs(a) < eps) {
-        if (fabs(b) < eps) return 0;
-        t[0] = t[1] = -c / b;
-        return 2;
-    }
-    double disc = b * b - 4.0 * a * c;
-    if (disc < 0.0) return 0;
-    double root = sqrt(disc);
-    if (fabs(root) < eps) {
-        t[0] = t[1] = -0.5 * b / a;
-        return 2;
-    }
-    double q = (b < 0.0) ? -0.5 * (b - root) : -0.5 * (b + root);
-    double lo, hi;
-    if (fabs(q) < eps) {
-        double inv = 0.5 / a;
-        lo = (-b - root) * inv;
-        hi = (-b + root) * inv;
-    } else {
-        lo = q / a;
-        hi = c / q;
-    }
-    t[0] = lo < hi ? lo : hi;
-    t[1] = lo < hi ? hi : lo;
-    return 2;
-}
-
-DVG_DEV int cubic_roots_f64(double a, double b, double c, double d, double t[3]) {  // utils.rs:82-112
-    if (fabs(a) < 1.0e-6) return quadratic_roots_f64(b, c, d, t);
-    double bn = b / a, cn = c / a, dn = d / a;
-    double q = (bn * bn - 3.0 * cn) / 9.0;
-    double r = (2.0 * bn * bn * bn - 9.0 * bn * cn + 27.0 * dn) / 54.0;
-    double q3 = q * q * q;
-    if (r * r < q3) {
-        const double pi = 3.14159265358979323846;
-        double theta = acos(r / sqrt(q3));
-        double s = -2.0 * sqrt(q);
-        t[0] = s * cos(theta / 3.0) - bn / 3.0;
-        t[1] = s * cos((theta + 2.0 * pi) / 3.0) - bn / 3.0;
-        t[2] = s * cos((theta - 2.0 * pi) / 3.0) - bn / 3.0;
-        return 3;
-    }
-    double disc = sqrt(r * r - q3);
-    double ar = (r > 0.0) ? -cbrt_f64(r + disc) : cbrt_f64(-r + disc);
-    double br = (fabs(ar) > 1.0e-6) ? q / ar : 0.0;
-    t[0] = (ar + br) - bn / 3.0;
-    return 1;
-}
-
-DVG_DEV int winding_cubic_f64(float px, float py, const float *x, const float *y) {  // winding.rs:166-188
-    double a = (double)(-y[0] + 3.0f * y[1] - 3.0f * y[2] + y[3]);
-    double b = (double)(3.0f * y[0] - 6.0f * y[1] + 3.0f * y[2]);
-    double c = (double)(-3.0f * y[0] + 3.0f * y[1]);
-    double d = (double)(y[0] - py);
-    double t[3];
-    int n = cubic_roots_f64(a, b, c, d, t);
-    double ax = (double)(-x[0] + 3.0f * x[1] - 3.0f * x[2] + x[3]);
-    double bx = (double)(3.0f * x[0] - 6.0f * x[1] + 3.0f * x[2]);
-    double cx = (double)(-3.0f * x[0] + 3.0f * x[1]);
-    double dx = (double)(x[0] - px);
-    int w = 0;
-    for (int i = 0; i < n; ++i) {
-        double s = t[i];
-        if (s >= 0.0 && s <= 1.0) {
-            double tp = ax * s * s * s + bx * s * s + cx * s + dx;
-            if (tp > 0.0) w += (3.0 * a * s * s + 2.0 * b * s + c > 0.0) ? 1 : -1;
-        }
-    }
-    return w;
-}
-
 // Random segment / point pairs in four families (pair i is family i % 4):
 // 0 random control points in a 128 px square, py in the segment's y-range;
 // 1 near-quadratic cubics (|a| <= 0.25 against control points 128 px apart),
 // the ill-conditioned case; 2 py within 0.05 px of a y-extremum of the curve,
 // where two roots merge; 3 points within 0.01 px of the curve.  counts[f]
 // receives the pairs of family f whose winding differs between
-// winding_cubic_bwd and the f64 restatement, counts[4 + f] the pairs that
-// took the monotone-piece path.
+// winding_cubic_bwd and the f64 restatement although winding_cubic_bwd did
+// not flag them as uncertain, counts[4 + f] the pairs it flagged, counts[8 + f]
+// the pairs where the unchecked test (the default boundary pass's) differs.
 __global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restrict__ counts) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -2979,11 +2988,14 @@ __global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restric
         py = by + 0.02f * (rng.next_f32() - 0.5f);
     }
     if (py < ymin || py > ymax) return;  // the walks cull those (no root either way)
-    float w = 0.f, cr = 0.f;
-    winding_cubic_bwd(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w, cr);
+    float w = 0.f, cr = 0.f, uns = 0.f;
+    winding_cubic_bwd<true>(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w, cr, uns);
     int w64 = winding_cubic_f64(px, py, x, y);
-    if ((int)w != w64) atomicAdd(counts + fam, 1u);
-    if ((uint32_t)rec[11] & 16u) atomicAdd(counts + 4 + fam, 1u);
+    if (uns > 0.f) atomicAdd(counts + 4 + fam, 1u);
+    else if ((int)w != w64) atomicAdd(counts + fam, 1u);
+    float wd = 0.f, crd = 0.f, ud = 0.f;
+    winding_cubic_bwd<false>(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, wd, crd, ud);
+    if ((int)wd != w64) atomicAdd(counts + 8 + fam, 1u);
 }
 
 // Winding terms of every segment (the record described with WIND_STRIDE).
@@ -3289,8 +3301,10 @@ __global__ __launch_bounds__(BLOCK) void k_batch_bnd_scatter(DVG_QUEUE_ARGS) {
 
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_batch_bnd_eval(
     DVG_QUEUE_ARGS) {
-    DVG_QUEUE_LOOP(bnd_eval_unit(B.S, w.unit, B.weight, B.d_render, B.G, B.offsets, B.order, B.nb))
+    DVG_QUEUE_LOOP(bnd_eval_unit<COV_CPU, false>(B.S, w.unit, B.weight, B.d_render, B.G, B.offsets, B.order, B.nb,
+                                                   FlagSlots{nullptr, nullptr}))
 }
+
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3376,8 +3390,21 @@ void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t
 void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
                           const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
-    hipLaunchKernelGGL(k_bnd_eval, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
-                       bw.offsets, bw.order, boundary_buckets(S));
+    FlagSlots fs = {bw.flag_count, bw.flag_slots};
+    if (fs.count)
+        hipLaunchKernelGGL(k_bnd_eval<COV_CHK>, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight,
+                           d_render, G, bw.offsets, bw.order, boundary_buckets(S), fs);
+    else
+        hipLaunchKernelGGL(k_bnd_eval<COV_CPU>, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight,
+                           d_render, G, bw.offsets, bw.order, boundary_buckets(S), fs);
+}
+
+void launch_boundary_redo(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
+                          const BoundaryWork &bw, hipStream_t stream) {
+    uint64_t total = (uint64_t)S.W * S.H * S.spp;
+    FlagSlots fs = {bw.flag_count, bw.flag_slots};
+    hipLaunchKernelGGL(k_bnd_redo, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
+                       bw.offsets, boundary_buckets(S), fs);
 }
 
 void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,
@@ -3430,7 +3457,7 @@ void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hi
 }
 
 void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, counts, 8u, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, stream, counts, 12u, 0u);
     if (n == 0) return;
     hipLaunchKernelGGL(k_selftest_inside, dim3(cdiv(n, 256)), dim3(256), 0, stream, n, seed, counts);
 }

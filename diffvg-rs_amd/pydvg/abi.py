@@ -150,6 +150,7 @@ SIGNATURES = {
     "dvg_device_render_backward": (C.c_int32, [_vp, _vp, _vp, C.POINTER(GradsC)]),
     "dvg_sync": (C.c_int32, [_vp]),
     "dvg_ctx_set_timing": (C.c_int32, [_vp, C.c_int]),
+    "dvg_ctx_set_exact_boundary": (C.c_int32, [_vp, C.c_int]),
     "dvg_kernel_times": (C.c_int32, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int, C.c_int]),
     "dvg_batch_create": (C.c_int32, [_vp, C.c_uint32, C.POINTER(ScenePackC), C.POINTER(BackwardPackC),
                                      C.POINTER(RenderParamsC), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

@@ -101,6 +101,11 @@ class Renderer:
     def sync(self):
         _check(self.lib, self.ctx, self.lib.dvg_sync(self.ctx))
 
+    def set_exact_boundary(self, enable: bool):
+        """Boundary samples whose f32 inside test is uncertain are evaluated
+        again in f64 (single-scene backward passes only)."""
+        _check(self.lib, self.ctx, self.lib.dvg_ctx_set_exact_boundary(self.ctx, int(enable)))
+
     def set_timing(self, enable: bool):
         self.lib.dvg_ctx_set_timing(self.ctx, int(enable))
 

@@ -324,6 +324,15 @@ const char *dvg_last_error(const dvg_ctx *ctx);
  * as void*; NULL is the device's null stream).  Until called, the context
  * uses a non-blocking stream of its own. */
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *hip_stream);
+/* Exact boundary mode (no counterpart in the reference; off by default).
+ * When enabled, the single-scene backward passes (dvg_render_backward, the
+ * packed and device-scene forms; not the batch) flag every boundary sample
+ * whose f32 inside test the f64 winding of distance/winding.rs could
+ * disagree with, and evaluate those samples again in f64 on a second stream
+ * of the context beside the colour adjoint.  The boundary gradients then
+ * follow the reference CPU sample for sample; the pass costs more (about a
+ * quarter on 1024^2 with 1000 cubic paths). */
+int32_t dvg_ctx_set_exact_boundary(dvg_ctx *ctx, int enable);
 
 /* ---- scene API ------------------------------------------------------------ */
 
@@ -488,8 +497,10 @@ int32_t dvg_selftest_winding(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *
  * segment / point pairs in four families (random; near-quadratic cubics;
  * points near a y-extremum; points within 0.01 px of the curve) against an
  * f64 restatement of that formula.  counts[f] (f < 4) receives the pairs of
- * family f that differ, counts[4 + f] those solved on monotone pieces.
- * Synchronous. */
+ * family f that differ without being flagged uncertain, counts[4 + f] the
+ * pairs flagged (the exact boundary mode evaluates those again in f64),
+ * counts[8 + f] the pairs where the default boundary pass's test differs
+ * (12 counts).  Synchronous. */
 int32_t dvg_selftest_inside(dvg_ctx *ctx, uint32_t n, uint32_t seed, uint32_t *counts);
 
 /* Version string of the library build. */

@@ -43,7 +43,8 @@ def setup(renderer):
 
     items = _items()
     targets, refs = [], []
-    for sc, opts in items:
+    for i, (sc, opts) in enumerate(items):
+        print(f"setup scene {i}: {sc.width}x{sc.height}, {len(sc.shapes)} shapes", flush=True)
         tds = DeviceScene(renderer, PreparedScene(scenes.jittered(sc, 2.0, seed=9), opts))
         tgt = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device="cuda")
         tds.render(tgt)
@@ -61,6 +62,7 @@ def setup(renderer):
         ds.render_backward(dr, bufs, sizes)
         renderer.sync()
         refs.append((img.cpu().numpy(), {k: bufs[k].cpu().numpy()[:sizes[k]] for k in bufs}))
+    print("setup: batch of the scenes", flush=True)
     batch = Batch(renderer, items, targets)
     return batch, refs
 

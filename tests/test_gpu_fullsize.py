@@ -129,3 +129,35 @@ def test_fullsize_render_repeats(renderer, name):
     assert np.all(np.isfinite(a))
     assert a.min() >= -1e-6 and a.max() <= 1.0 + 1e-5
     assert np.allclose(a[:, 3], 1.0, atol=1e-5)
+
+
+# Exact boundary mode (dvg_ctx_set_exact_boundary): boundary samples whose f32
+# inside tests are uncertain are evaluated again with the reference CPU's f64
+# windings, so the band gradients follow the oracle more closely than the
+# default pass's 1e-3.  Measured: 2.7e-7 (1024^2, 1000 cubic paths; the
+# default pass 2.0e-4), 9.4e-7 (256^2, 4x4 samples).
+EXACT_GRAD_REL = 1e-5
+
+
+@pytest.mark.parametrize("name", ["paths1024", "paths256_spp16", "mixed512"])
+def test_fullsize_exact_boundary_band(renderer, oracle, name):
+    sc, opts = _scene(name)
+    W, H = sc.width, sc.height
+    bwd_band = dict((c[0], c[2]) for c in CASES)[name]
+    print(f"case {name} exact boundary, rows {bwd_band}", flush=True)
+    img = renderer.render(sc, opts)
+    tgt = renderer.render(scenes.jittered(sc, 2.0, seed=9), opts)
+    d_render = _band_adjoint(img, tgt, W, H, bwd_band)
+    g_fast = renderer.render_backward(sc, opts, d_render)
+    renderer.set_exact_boundary(True)
+    try:
+        g = renderer.render_backward(sc, opts, d_render)
+    finally:
+        renderer.set_exact_boundary(False)
+    gref = oracle.render_backward_rows(SceneDesc(sc), opts, d_render, *bwd_band)
+    total, per = grad_report(g, gref)
+    fast, _ = grad_report(g_fast, gref)
+    print(f"{name} exact boundary rows {bwd_band}: gradient rel L2 {total:.3g} (default pass {fast:.3g}); "
+          f"per field {per}", flush=True)
+    assert total <= EXACT_GRAD_REL
+    assert total <= fast + 1e-6  # (float atomics reorder sums between runs)
