@@ -316,9 +316,12 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
                 lw += (float)n;
             }
         } else if (k == 1) {
-            winding_quad_w<M == COV_CHK>(px, py, x0, y0, x1, x2, wc, rule, lw, lc, &c.unsure);
+            // (the unchecked tests get no pointer into c: it would keep c in memory)
+            if (M == COV_CHK) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, lw, lc, &c.unsure);
+            else winding_quad_w<false>(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
         } else if (CPU) {
-            winding_cubic_bwd<M == COV_CHK>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc, c.unsure);
+            float u = 0.f;
+            winding_cubic_bwd<M == COV_CHK>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc, M == COV_CHK ? c.unsure : u);
         } else {
             winding_cubic_w<false>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
         }
@@ -2328,39 +2331,6 @@ DVG_DEV bool sample_boundary_path(const DevScene &S, uint32_t si, float path_len
 
 // Colour of one boundary query point with the EdgeQuery hit tracking of
 // sampling.rs:76-224 (no adjoint).  (qx, qy) is the normalized point.
-// The tests one tile-list entry g makes for a boundary query: its stroke and
-// fill fragments and, for the sampled group, the EdgeQuery hit test on the
-// sampled shape (within_distance_edge / is_inside_edge, sampling.rs:485-549;
-// for a group of one shape that is the group's own test).  `uns`: a COV_CHK
-// test was uncertain.
-template <int M>
-DVG_DEV void group_edge_test(const DevScene &S, uint32_t g, float px, float py, uint32_t eq_group,
-                             uint32_t eq_prep, bool &st, bool &fi, bool &et, bool &uns) {
-    group_frags<M>(S, g, px, py, st, fi, &uns);
-    et = false;
-    if (g != eq_group) return;
-    const float *gd = S.group_data + g * 16u;
-    if ((uint32_t)gd[1] == 1u) {
-        et = st || fi;
-        return;
-    }
-    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-    float lx, ly;
-    group_local(S, g, px, py, lx, ly);
-    const float *xf = S.shape_xform + eq_prep * 6u;
-    float spx = xf[0] * lx + xf[1] * ly + xf[2];
-    float spy = xf[3] * lx + xf[4] * ly + xf[5];
-    const float *sd = S.shape_data + eq_prep * 15u;
-    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
-    if (sk != DVG_PAINT_NONE) shape_stroke<M>(S, sd, false, spx, spy, c);
-    if (fk != DVG_PAINT_NONE) shape_fill<M>(S, sd, rule, false, spx, spy, c);
-    if (M == COV_CHK && c.unsure > 0.f) uns = true;
-    if (c.stroke_hit > 0.f) et = true;
-    if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
-        if (fk != DVG_PAINT_NONE) et = true;
-    }
-}
-
 template <int M>
 DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
                               uint32_t eq_prep, bool &hit, bool &uns);
@@ -2393,9 +2363,33 @@ DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float 
     bool h0 = false, h1 = true;  // compositing-loop hit for an initial false / true
     for (uint32_t e = lb; e < le; ++e) {
         uint32_t g = S.tile_entries[e];
-        bool st, fi, et;
-        group_edge_test<M>(S, g, px, py, eq_group, eq_prep, st, fi, et, uns);
-        if (et) edge_test = true;
+        bool st, fi;
+        group_frags<M>(S, g, px, py, st, fi, &uns);
+        if (g == eq_group) {
+            // within_distance_edge / is_inside_edge (sampling.rs:485-549) on
+            // the sampled shape alone; for a group of one shape that is the
+            // group's own stroke / fill test above (bounds are conservative)
+            const float *gd = S.group_data + g * 16u;
+            if ((uint32_t)gd[1] == 1u) {
+                if (st || fi) edge_test = true;
+            } else {
+                uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+                float lx, ly;
+                group_local(S, g, px, py, lx, ly);
+                const float *xf = S.shape_xform + eq_prep * 6u;
+                float spx = xf[0] * lx + xf[1] * ly + xf[2];
+                float spy = xf[3] * lx + xf[4] * ly + xf[5];
+                const float *sd = S.shape_data + eq_prep * 15u;
+                Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+                if (sk != DVG_PAINT_NONE) shape_stroke<M>(S, sd, false, spx, spy, c);
+                if (fk != DVG_PAINT_NONE) shape_fill<M>(S, sd, rule, false, spx, spy, c);
+                if (M == COV_CHK && c.unsure > 0.f) uns = true;
+                if (c.stroke_hit > 0.f) edge_test = true;
+                if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
+                    if (fk != DVG_PAINT_NONE) edge_test = true;
+                }
+            }
+        }
         for (int k = 0; k < 2; ++k) {
             bool is_stroke = k == 0;
             if (!(is_stroke ? st : fi)) continue;
@@ -2827,11 +2821,17 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
     }
 }
 
-template <int M>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval(
     DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
+    const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ order, uint32_t nb) {
+    bnd_eval_unit<COV_CPU, false>(S, blockIdx.x, weight, d_render, G, offsets, order, nb, FlagSlots{nullptr, nullptr});
+}
+
+// The exact boundary mode's pass: uncertain samples go to the redo's slots.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_eval_chk(
+    DevScene S, const float *__restrict__ weight, const float *__restrict__ d_render, DevGrads G,
     const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ order, uint32_t nb, FlagSlots fs) {
-    bnd_eval_unit<M, false>(S, blockIdx.x, weight, d_render, G, offsets, order, nb, fs);
+    bnd_eval_unit<COV_CHK, false>(S, blockIdx.x, weight, d_render, G, offsets, order, nb, fs);
 }
 
 // The samples k_bnd_eval left to the redo, unit by unit, with the f64
@@ -3392,11 +3392,11 @@ void launch_boundary_eval(const DevScene &S, const float *weight, const float *d
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     FlagSlots fs = {bw.flag_count, bw.flag_slots};
     if (fs.count)
-        hipLaunchKernelGGL(k_bnd_eval<COV_CHK>, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight,
-                           d_render, G, bw.offsets, bw.order, boundary_buckets(S), fs);
+        hipLaunchKernelGGL(k_bnd_eval_chk, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
+                           bw.offsets, bw.order, boundary_buckets(S), fs);
     else
-        hipLaunchKernelGGL(k_bnd_eval<COV_CPU>, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight,
-                           d_render, G, bw.offsets, bw.order, boundary_buckets(S), fs);
+        hipLaunchKernelGGL(k_bnd_eval, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
+                           bw.offsets, bw.order, boundary_buckets(S));
 }
 
 void launch_boundary_redo(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,
