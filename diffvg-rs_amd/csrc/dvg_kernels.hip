@@ -2594,8 +2594,8 @@ DVG_DEV float masked_wave_sum(bool mine, float v) { return wave_sum(mine ? v : 0
 // Uncertain boundary samples (a query point within rounding distance of an
 // edge), per unit of the bucket-ordered pass: up to FLAG_SLOTS sample indices
 // a unit and their count (beyond it the f32 result stands).  The redo pass
-// reads a unit's slots together, so its lanes share the unit's tile and the
-// walks stay uniform.
+// gives each unit FLAG_SLOTS adjacent lanes, so a wave holds two neighbouring
+// units of the bucket order and its walks stay close to uniform.
 constexpr uint32_t FLAG_SLOTS = FLAG_UNIT_SLOTS;
 struct FlagSlots {
     uint32_t *count, *slots;
@@ -2603,7 +2603,9 @@ struct FlagSlots {
 
 // LIST = false, M = COV_CHK: the samples of unit u in bucket order; those
 // with uncertain tests are left to the redo (slots of unit u).  LIST = true:
-// the samples in unit u's slots, with the tests of M.
+// the samples in the slots of units u * FLAG_UNITS_PER_WG + k (k <
+// FLAG_UNITS_PER_WG; FLAG_SLOTS lanes a unit), with the tests of M.
+constexpr uint32_t FLAG_UNITS_PER_WG = BLOCK / FLAG_SLOTS;
 template <int M, bool LIST>
 DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restrict__ weight,
                            const float *__restrict__ d_render, const DevGrads &G, const uint32_t *__restrict__ offsets,
@@ -2615,21 +2617,28 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
     uint32_t n_valid = offsets[nb];
     uint32_t pos = u * BLOCK + threadIdx.x;
     uint32_t lane = threadIdx.x & 63u;
-    if (u * BLOCK >= n_valid) return;  // whole workgroup past the end
+    size_t slot_base = 0;
     if (LIST) {
-        n_valid = min(fs.count[u], FLAG_SLOTS);
-        if (n_valid == 0) return;
-        pos = threadIdx.x;
-    } else if (M == COV_CHK) {
-        if (threadIdx.x == 0) s_nflag = 0;
-        __syncthreads();
+        // this lane's unit and slot; units past the samples have no slots
+        // (no barrier follows in this mode: lanes without a sample idle)
+        uint32_t lu = u * FLAG_UNITS_PER_WG + threadIdx.x / FLAG_SLOTS;
+        uint32_t cnt = (lu * BLOCK < n_valid) ? min(fs.count[lu], FLAG_SLOTS) : 0u;
+        slot_base = (size_t)lu * FLAG_SLOTS;
+        pos = threadIdx.x % FLAG_SLOTS;
+        n_valid = cnt;
+    } else {
+        if (u * BLOCK >= n_valid) return;  // whole workgroup past the end
+        if (M == COV_CHK) {
+            if (threadIdx.x == 0) s_nflag = 0;
+            __syncthreads();
+        }
     }
     BPoint P;
     bool valid = false;
     float contrib = 0.f, nnx = 0.f, nny = 0.f;
     size_t pix = 0;
     if (pos < n_valid) {
-        size_t idx = LIST ? fs.slots[(size_t)u * FLAG_SLOTS + pos] : order[pos];
+        size_t idx = LIST ? fs.slots[slot_base + pos] : order[pos];
         valid = boundary_point(S, nullptr, idx, P);
         if (valid) {
             pix = (size_t)P.pyi * S.W + P.pxi;
@@ -3403,8 +3412,8 @@ void launch_boundary_redo(const DevScene &S, const float *weight, const float *d
                           const BoundaryWork &bw, hipStream_t stream) {
     uint64_t total = (uint64_t)S.W * S.H * S.spp;
     FlagSlots fs = {bw.flag_count, bw.flag_slots};
-    hipLaunchKernelGGL(k_bnd_redo, dim3(cdiv(total, BLOCK)), dim3(BLOCK), 0, stream, S, weight, d_render, G,
-                       bw.offsets, boundary_buckets(S), fs);
+    hipLaunchKernelGGL(k_bnd_redo, dim3(cdiv(cdiv(total, BLOCK), FLAG_UNITS_PER_WG)), dim3(BLOCK), 0, stream, S,
+                       weight, d_render, G, bw.offsets, boundary_buckets(S), fs);
 }
 
 void launch_batch_phase(int phase, const BatchScene *scenes, const WorkItem *items, uint32_t n_items,

@@ -330,8 +330,8 @@ int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *hip_stream);
  * whose f32 inside test the f64 winding of distance/winding.rs could
  * disagree with, and evaluate those samples again in f64 on a second stream
  * of the context beside the colour adjoint.  The boundary gradients then
- * follow the reference CPU sample for sample; the pass costs more (about a
- * quarter on 1024^2 with 1000 cubic paths). */
+ * follow the reference CPU sample for sample; the pass costs more (about
+ * 15 % of a step on 1024^2 with 1000 cubic paths). */
 int32_t dvg_ctx_set_exact_boundary(dvg_ctx *ctx, int enable);
 
 /* ---- scene API ------------------------------------------------------------ */
