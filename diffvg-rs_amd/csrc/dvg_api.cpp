@@ -761,7 +761,8 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
     if (S.n_cdf > 0) {
         size_t ns = npix * S.spp, nt = dvgk::boundary_buckets(S);
         size_t units = (ns + dvgk::BLOCK - 1) / dvgk::BLOCK;
-        size_t need = 3 * ns + 2 * nt + 3 + (exact ? units * (1 + dvgk::FLAG_UNIT_SLOTS) : 0);
+        size_t nsum = dvgk::scan_chunks((uint32_t)nt + 1);
+        size_t need = 3 * ns + 2 * nt + 3 + nsum + (exact ? units * (1 + dvgk::FLAG_UNIT_SLOTS) : 0);
         if (ctx->bwork_cap < need) {
             if (ctx->bwork) (void)hipFree(ctx->bwork);
             ctx->bwork = nullptr;
@@ -775,7 +776,8 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
         bw.order = bw.ranks + ns;
         bw.counts = bw.order + ns;
         bw.offsets = bw.counts + nt + 1;
-        bw.flag_count = exact ? bw.offsets + nt + 2 : nullptr;
+        bw.chunk_sums = bw.offsets + nt + 2;
+        bw.flag_count = exact ? bw.chunk_sums + nsum : nullptr;
         bw.flag_slots = exact ? bw.flag_count + units : nullptr;
         {
             KernelTimer kt(ctx, DVG_KERNEL_BOUNDARY_SORT);

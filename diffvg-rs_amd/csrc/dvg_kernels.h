@@ -132,9 +132,10 @@ __host__ __device__ inline uint32_t boundary_buckets(const DevScene &S) {
     return S.tiles_x * S.tiles_y * per_row * per_row;
 }
 // Work buffers of the boundary pass: keys/ranks/order hold one u32 per
-// sample, counts buckets+1, offsets buckets+2.
+// sample, counts buckets+1, offsets buckets+2, chunk_sums
+// scan_chunks(buckets+1).
 struct BoundaryWork {
-    uint32_t *keys, *ranks, *order, *counts, *offsets;
+    uint32_t *keys, *ranks, *order, *counts, *offsets, *chunk_sums;
     // Samples whose f32 inside tests are uncertain (a query point within
     // rounding distance of an edge), left by the pass to a redo with f64
     // windings: per unit of BLOCK bucket-ordered samples, a count
@@ -142,6 +143,10 @@ struct BoundaryWork {
     uint32_t *flag_count, *flag_slots;
 };
 constexpr uint32_t FLAG_UNIT_SLOTS = 32;
+// The bucket offsets come from a scan over many workgroups: SCAN_CHUNK
+// counts a workgroup (256 threads x 16).
+constexpr uint32_t SCAN_PER_THREAD = 16, SCAN_CHUNK = 256 * SCAN_PER_THREAD;
+__host__ __device__ inline uint32_t scan_chunks(uint32_t n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
 // Boundary samples: tile keys, counting sort; then the tile-ordered evaluation.
 void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t stream);
 void launch_boundary_eval(const DevScene &S, const float *weight, const float *d_render, const DevGrads &G,

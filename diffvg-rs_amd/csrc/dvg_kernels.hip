@@ -5,9 +5,11 @@
 //   k_forward                            per-sample scene evaluation + splat (forward.rs:1568-1735)
 //   k_resolve                            splat resolve (forward.rs:295-334)
 //   k_backward_interior                  colour adjoint per sample (backward/sampling.rs:76-224)
-//   k_bnd_key / k_bnd_scatter / k_bnd_eval
+//   k_bnd_key / k_scan_sums / k_scan_chunks / k_bnd_scatter / k_bnd_eval
 //                                        edge sampling (backward/boundary.rs:113-1148),
 //                                        counting-sorted by tile before evaluation
+//   k_bnd_eval_chk / k_bnd_redo          the same with uncertain inside tests
+//                                        evaluated again in f64 (exact boundary mode)
 //
 // One workgroup of 256 lanes (4 wave64) owns one 16x16 tile; its samples are
 // visited in 4x4-pixel blocks so that a wave covers a compact footprint and
@@ -106,6 +108,63 @@ DVG_DEV void scan_unit(const uint32_t *__restrict__ counts, uint32_t n, uint32_t
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ counts, uint32_t n,
                                                uint32_t *__restrict__ offsets) {
     scan_unit<1024>(counts, n, offsets);
+}
+
+// Exclusive scan of counts[0, n) into offsets[0, n] (offsets[n] = total)
+// over scan_chunks(n) workgroups of 256: k_scan_sums adds up each chunk of
+// SCAN_CHUNK counts, k_scan_chunks scans its chunk from the sum of the chunks
+// before it (SCAN_PER_THREAD consecutive counts a thread).
+DVG_DEV uint32_t block_sum_256(uint32_t v, uint32_t *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t *__restrict__ counts, uint32_t n,
+                                                   uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[4];
+    uint32_t b = blockIdx.x * SCAN_CHUNK, e = min(n, b + SCAN_CHUNK);
+    uint32_t s = 0;
+    for (uint32_t i = b + threadIdx.x; i < e; i += 256u) s += counts[i];
+    uint32_t t = block_sum_256(s, sh);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_scan_chunks(const uint32_t *__restrict__ counts, uint32_t n,
+                                                     const uint32_t *__restrict__ sums,
+                                                     uint32_t *__restrict__ offsets) {
+    __shared__ uint32_t sh[4], wsum[4];
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 256u) c += sums[j];
+    uint32_t carry = block_sum_256(c, sh);
+    uint32_t b = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD];
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+        v[k] = b + k < n ? counts[b + k] : 0u;
+        s += v[k];
+    }
+    // exclusive prefix of s over the workgroup: within the wave, then the
+    // totals of the waves before
+    uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t inc = s;
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63u) wsum[w] = inc;
+    __syncthreads();
+    uint32_t run = carry + inc - s;
+    for (uint32_t k = 0; k < w; ++k) run += wsum[k];
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+        if (b + k < n) offsets[b + k] = run;
+        run += v[k];
+    }
+    // the last thread of the last chunk holds the total
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255u) offsets[n] = run;
 }
 
 // ---------------------------------------------------------------------------
@@ -3391,7 +3450,9 @@ void launch_boundary_sort(const DevScene &S, const BoundaryWork &bw, hipStream_t
     uint32_t zn[1] = {nbk + 1};
     launch_zero(zp, zn, 1, stream);
     hipLaunchKernelGGL(k_bnd_key, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.ranks, bw.counts, nbk);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, bw.counts, nbk + 1, bw.offsets);
+    uint32_t nc = scan_chunks(nbk + 1);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nc), dim3(256), 0, stream, bw.counts, nbk + 1, bw.chunk_sums);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(nc), dim3(256), 0, stream, bw.counts, nbk + 1, bw.chunk_sums, bw.offsets);
     hipLaunchKernelGGL(k_bnd_scatter, dim3(grid), dim3(BLOCK), 0, stream, S, bw.keys, bw.ranks, bw.offsets, bw.order,
                        nbk);
 }
