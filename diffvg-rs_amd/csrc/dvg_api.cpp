@@ -631,6 +631,30 @@ void dvg_device_scene_destroy(dvg_device_scene *ds) {
     delete ds;
 }
 
+// Renderer::render_sdf / eval_positions (renderer.rs:586-848) on a device
+// scene; the signed distance needs its backward tables.
+int32_t dvg_device_render_sdf(dvg_ctx *ctx, dvg_device_scene *ds, float *out) {
+    if (!ctx || !ds || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
+        return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvgk::launch_sdf(ds->S, out, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
+int32_t dvg_device_eval_positions(dvg_ctx *ctx, dvg_device_scene *ds, const float *positions, uint64_t count,
+                                  float *out) {
+    if (!ctx || !ds || (count && (!positions || !out))) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+    if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
+        return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvgk::launch_sdf_points(ds->S, positions, (uint32_t)count, out, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
 int32_t dvg_device_render(dvg_ctx *ctx, dvg_device_scene *ds, float *out) {
     if (!ctx || !ds || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
     const dvgk::DevScene &S = ds->S;
@@ -954,6 +978,72 @@ int32_t dvg_render(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_o
     dvg_render_params prm;
     dvgh::pack_views(P, &pk, nullptr, &prm);
     return dvg_render_packed(ctx, &pk, &prm, out);
+}
+
+// Renderer::render_sdf (renderer.rs:586-727) and eval_positions
+// (renderer.rs:729-848): the scene is packed, uploaded with its backward
+// tables, evaluated and read back.
+namespace {
+int sdf_host(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options, const float *positions,
+             uint64_t count, float *out) {
+    dvg_prepared P;
+    const char *err = "";
+    int st = dvgh::prepare(scene, options, P, &err);
+    if (st) return set_err(ctx, st, err);
+    dvg_scene_pack pk;
+    dvg_backward_pack bp;
+    dvg_render_params prm;
+    dvgh::pack_views(P, &pk, &bp, &prm);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvg_device_scene *ds = nullptr;
+    st = dvg_device_scene_create(ctx, &pk, &bp, &prm, &ds);
+    if (st) return st;
+    size_t n = positions ? (size_t)count : (size_t)scene->width * scene->height;
+    DevBuf ob, pb;
+    hipError_t e = hipMalloc(&ob.p, n * sizeof(float));
+    if (e == hipSuccess && positions) {
+        e = hipMalloc(&pb.p, n * 2 * sizeof(float));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(pb.p, positions, n * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    }
+    if (e != hipSuccess) {
+        dvg_device_scene_destroy(ds);
+        return hip_err(ctx, e, "sdf buffers");
+    }
+    st = positions ? dvg_device_eval_positions(ctx, ds, (const float *)pb.p, count, (float *)ob.p)
+                   : dvg_device_render_sdf(ctx, ds, (float *)ob.p);
+    if (st == DVG_OK) {
+        e = hipMemcpyAsync(out, ob.p, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hip_err(ctx, e, "sdf readback");
+    }
+    dvg_device_scene_destroy(ds);
+    return st;
+}
+}  // namespace
+
+int32_t dvg_render_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options, float *out) {
+    if (!ctx || !scene || !options || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    size_t npix = (size_t)scene->width * scene->height;
+    if (npix == 0) return DVG_OK;
+    if (scene->num_groups == 0) {
+        for (size_t i = 0; i < npix; ++i) out[i] = 0.0f;
+        return DVG_OK;
+    }
+    return sdf_host(ctx, scene, options, nullptr, 0, out);
+}
+
+int32_t dvg_eval_positions(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                           const float *positions, uint64_t count, float *out) {
+    if (!ctx || !scene || !options || (count && (!positions || !out)))
+        return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (count == 0) return DVG_OK;
+    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+    if (scene->num_groups == 0) {
+        for (uint64_t i = 0; i < count; ++i) out[i] = 0.0f;
+        return DVG_OK;
+    }
+    return sdf_host(ctx, scene, options, positions, count, out);
 }
 
 int32_t dvg_render_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,

@@ -87,6 +87,11 @@ void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uin
 void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream);
 void launch_forward(const DevScene &S, const float *weight, float *accum, hipStream_t stream);
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream);
+// Signed distance (sdf.rs): W*H per-pixel means of the samples, or n points
+// (x, y pairs); both need the scene's backward tables (shape_transform,
+// prepared_to_scene).
+void launch_sdf(const DevScene &S, float *out, hipStream_t stream);
+void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream);
 // Bit-set slots of the colour adjoint for tile lists longer than the LDS bit
 // set (LDS_MASK_GROUPS entries): `slots` workgroup slots of
 // nw = (2 gcap + 31) / 32 words x BLOCK lanes at gmask, and one queue counter.

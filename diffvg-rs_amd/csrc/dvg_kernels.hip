@@ -3066,6 +3066,157 @@ __global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restric
     if ((int)wd != w64) atomicAdd(counts + 8 + fam, 1u);
 }
 
+// ---- signed distance (Renderer::render_sdf / eval_positions) ------------------
+// closest_point_shape, distance.rs:235-296: the closest boundary point of
+// prepared shape si in shape space; false when a path has no segment.
+DVG_DEV bool closest_point_shape(const DevScene &S, uint32_t si, float px, float py, float &ox, float &oy) {
+    const float *sd = S.shape_data + si * 15u;
+    uint32_t kind = (uint32_t)sd[0];
+    float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
+    if (kind == DVG_SHAPE_CIRCLE) {  // closest_point_circle, distance.rs:33-49
+        float r = absf_(q2);
+        float dx = px - q0, dy = py - q1;
+        float len = sqrtf(dx * dx + dy * dy);
+        if (len <= 1.0e-8f) {
+            ox = q0 + r;
+            oy = q1;
+        } else {
+            float inv = r / len;
+            ox = q0 + dx * inv;
+            oy = q1 + dy * inv;
+        }
+        return true;
+    }
+    if (kind == DVG_SHAPE_ELLIPSE) {  // closest_point_ellipse, distance.rs:83-139
+        ellipse_closest(q0, q1, q2, q3, px, py, ox, oy);
+        return true;
+    }
+    if (kind == DVG_SHAPE_RECT) {  // closest_point_rect, distance.rs:52-80
+        bool inside = px >= q0 && px <= q2 && py >= q1 && py <= q3;
+        if (!inside) {
+            ox = fminf(fmaxf(px, q0), q2);
+            oy = fminf(fmaxf(py, q1), q3);
+        } else {
+            float dl = px - q0, dr = q2 - px, db = py - q1, dt = q3 - py;
+            if (dl <= dr && dl <= db && dl <= dt) { ox = q0; oy = py; }
+            else if (dr <= db && dr <= dt) { ox = q2; oy = py; }
+            else if (db <= dt) { ox = px; oy = q1; }
+            else { ox = px; oy = q3; }
+        }
+        return true;
+    }
+    // closest_point_path, distance.rs:142-231: the first segment at the
+    // smallest distance, its point from the parameter
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    bool approx = sd[14] > 0.5f;
+    float best = BIG;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = S.curve_data + (off + s) * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
+        float d, t;
+        if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
+        else if (k == 1) quad_closest<true>(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
+        else cubic_closest<true>(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        if (d < best) {
+            best = d;
+            float tt = 1.0f - t;
+            if (k == 0) {
+                ox = x0 + t * (x1 - x0);
+                oy = y0 + t * (y1 - y0);
+            } else if (k == 1) {
+                ox = tt * tt * x0 + 2.0f * tt * t * x1 + t * t * x2;
+                oy = tt * tt * y0 + 2.0f * tt * t * y1 + t * t * y2;
+            } else {
+                ox = tt * tt * tt * x0 + 3.0f * tt * tt * t * x1 + 3.0f * tt * t * t * x2 + t * t * t * x3;
+                oy = tt * tt * tt * y0 + 3.0f * tt * tt * t * y1 + 3.0f * tt * t * t * y2 + t * t * t * y3;
+            }
+        }
+    }
+    return best < BIG;
+}
+
+// compute_signed_distance, sdf.rs:9-117: the nearest shape boundary over the
+// groups from last to first (compute_distance_group, distance.rs:298-513; a
+// shape whose group-local box is farther than the best distance so far is
+// skipped, as the group BVH skips nodes), negative inside the fill of the
+// nearest shape's group (fill_inside_group, backward.rs:1529-1606, which is
+// the forward's fill test).
+DVG_DEV float signed_distance(const DevScene &S, float px, float py) {
+    float best = BIG;
+    uint32_t best_g = 0;
+    bool found = false;
+    for (uint32_t gg = S.num_groups; gg > 0; --gg) {
+        uint32_t g = gg - 1u;
+        const float *gd = S.group_data + g * 16u;
+        uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+        float lx, ly;
+        group_local(S, g, px, py, lx, ly);
+        const float *gs = S.group_shape_xform + g * 6u;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint32_t si = (uint32_t)S.group_shapes[off + i];
+            const float *b = S.shape_bounds + si * 4u;
+            if (b[0] <= b[2] && b[1] <= b[3]) {  // bounds_distance, distance.rs:10-15
+                float ddx = maxf_(maxf_(b[0] - lx, 0.f), lx - b[2]);
+                float ddy = maxf_(maxf_(b[1] - ly, 0.f), ly - b[3]);
+                if (sqrtf(ddx * ddx + ddy * ddy) > best) continue;
+            }
+            const float *xf = S.shape_xform + si * 6u;
+            float spx = xf[0] * lx + xf[1] * ly + xf[2];
+            float spy = xf[3] * lx + xf[4] * ly + xf[5];
+            float cx = 0.f, cy = 0.f;
+            if (!closest_point_shape(S, si, spx, spy, cx, cy)) continue;
+            const float *T = S.shape_transform + S.prepared_to_scene[si] * 6u;
+            float gx = T[0] * cx + T[1] * cy + T[2], gy = T[3] * cx + T[4] * cy + T[5];
+            float ccx = gs[0] * gx + gs[1] * gy + gs[2], ccy = gs[3] * gx + gs[4] * gy + gs[5];
+            float dx = ccx - px, dy = ccy - py;
+            float d = sqrtf(dx * dx + dy * dy);
+            if (d < best) {
+                best = d;
+                best_g = g;
+                found = true;
+            }
+        }
+    }
+    if (!found) return 0.f;
+    const float *gd = S.group_data + best_g * 16u;
+    uint32_t fk = (uint32_t)gd[2], rule = (uint32_t)gd[7];
+    bool inside = false;
+    if (fk != DVG_PAINT_NONE) {
+        float lx, ly;
+        group_local(S, best_g, px, py, lx, ly);
+        Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+        group_cov<COV_FWD>(S, best_g, gd, lx, ly, true, false, rule, false, c);
+        inside = rule == 1u ? c.crossings > 0.f : c.winding != 0.f;
+    }
+    return inside ? -best : best;
+}
+
+// render_sdf_kernel, sdf.rs:119-200: one pixel a lane, the mean signed
+// distance of its samples (the forward's sample positions).
+__global__ __launch_bounds__(256) void k_sdf(DevScene S, float *__restrict__ out) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= S.W * S.H) return;
+    uint32_t x = i % S.W, y = i / S.W;
+    float acc = 0.f;
+    for (uint32_t sy = 0; sy < S.nsy; ++sy)
+        for (uint32_t sx = 0; sx < S.nsx; ++sx) {
+            float px, py;
+            sample_point<false>(S, x, y, sx, sy, px, py);
+            acc += signed_distance(S, px, py);
+        }
+    out[i] = acc * (1.0f / (float)(S.nsx * S.nsy));
+}
+
+// eval_positions_kernel, sdf.rs:207-262: the signed distance at n canvas
+// points (x, y pairs).
+__global__ __launch_bounds__(256) void k_sdf_points(DevScene S, const float *__restrict__ pos, uint32_t n,
+                                                    float *__restrict__ out) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    out[i] = signed_distance(S, pos[2u * i], pos[2u * i + 1u]);
+}
+
 // Winding terms of every segment (the record described with WIND_STRIDE).
 __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurves, float *__restrict__ out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3422,6 +3573,17 @@ void launch_forward(const DevScene &S, const float *weight, float *accum, hipStr
     if (S.prefilter) hipLaunchKernelGGL(k_forward<true>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
     else hipLaunchKernelGGL(k_forward<false>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
 }
+void launch_sdf(const DevScene &S, float *out, hipStream_t stream) {
+    uint64_t npix = (uint64_t)S.W * S.H;
+    if (npix == 0) return;
+    hipLaunchKernelGGL(k_sdf, dim3(cdiv(npix, 256)), dim3(256), 0, stream, S, out);
+}
+
+void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_sdf_points, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, pos, n, out);
+}
+
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream) {
     hipLaunchKernelGGL(k_resolve, dim3(cdiv((uint64_t)S.W * S.H, 256)), dim3(256), 0, stream, S, weight, accum, out);
 }

@@ -130,6 +130,11 @@ SIGNATURES = {
     "dvg_last_error": (C.c_char_p, [_vp]),
     "dvg_ctx_set_stream": (C.c_int32, [_vp, _vp]),
     "dvg_render": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions), _f32p]),
+    "dvg_render_sdf": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions), _f32p]),
+    "dvg_eval_positions": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions), _f32p, C.c_uint64,
+                                       _f32p]),
+    "dvg_device_render_sdf": (C.c_int32, [_vp, _vp, _vp]),
+    "dvg_device_eval_positions": (C.c_int32, [_vp, _vp, _vp, C.c_uint64, _vp]),
     "dvg_render_backward": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions),
                                         C.POINTER(BackwardOptions), _f32p, C.POINTER(GradsC)]),
     "dvg_render_packed": (C.c_int32, [_vp, C.POINTER(ScenePackC), C.POINTER(RenderParamsC), _f32p]),

@@ -81,6 +81,26 @@ class Renderer:
         _check(self.lib, self.ctx, st)
         return out
 
+    def render_sdf(self, scene: Scene, options: RenderOptions) -> np.ndarray:
+        """Renderer::render_sdf: W * H signed distances (negative inside a fill)."""
+        desc = SceneDesc(scene)
+        out = np.zeros(scene.width * scene.height, np.float32)
+        st = self.lib.dvg_render_sdf(self.ctx, desc.ref(), C.byref(options.to_c()),
+                                     out.ctypes.data_as(C.POINTER(C.c_float)))
+        _check(self.lib, self.ctx, st)
+        return out
+
+    def eval_positions(self, scene: Scene, positions: np.ndarray, options: RenderOptions) -> np.ndarray:
+        """Renderer::eval_positions: the signed distance at each canvas point (n x 2)."""
+        desc = SceneDesc(scene)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        out = np.zeros(pos.size // 2, np.float32)
+        st = self.lib.dvg_eval_positions(self.ctx, desc.ref(), C.byref(options.to_c()),
+                                         pos.ctypes.data_as(C.POINTER(C.c_float)), pos.size // 2,
+                                         out.ctypes.data_as(C.POINTER(C.c_float)))
+        _check(self.lib, self.ctx, st)
+        return out
+
     def render_backward(self, scene: Scene, options: RenderOptions, d_render: np.ndarray,
                         backward_options: BackwardOptions = BackwardOptions()) -> FlatGrads:
         desc = SceneDesc(scene)
@@ -160,6 +180,18 @@ class DeviceScene:
 
     def render(self, out) -> None:
         st = self.r.lib.dvg_device_render(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))
+        _check(self.r.lib, self.r.ctx, st)
+
+    def render_sdf(self, out) -> None:
+        """W * H signed distances into a device tensor (asynchronous)."""
+        st = self.r.lib.dvg_device_render_sdf(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))
+        _check(self.r.lib, self.r.ctx, st)
+
+    def eval_positions(self, positions, out) -> None:
+        """Signed distances at the points of a device tensor (n x 2) into out (n)."""
+        n = positions.numel() // 2
+        st = self.r.lib.dvg_device_eval_positions(self.r.ctx, self.handle, C.c_void_p(positions.data_ptr()), n,
+                                                  C.c_void_p(out.data_ptr()))
         _check(self.r.lib, self.r.ctx, st)
 
     def grad_buffers(self, torch, device="cuda"):

@@ -340,6 +340,21 @@ int32_t dvg_ctx_set_exact_boundary(dvg_ctx *ctx, int enable);
 int32_t dvg_render(dvg_ctx *ctx, const dvg_scene_desc *scene,
                    const dvg_render_options *options, float *out_rgba);
 
+/* Renderer::render_sdf (renderer.rs:586-727 -> render_sdf_kernel,
+ * src/gpu/kernels/sdf.rs:119-200): width*height signed distances, each the
+ * mean over the pixel's samples (the forward's sample positions) of the
+ * distance to the nearest shape boundary, negative inside the fill of that
+ * shape's group.  All zero for a scene without groups. */
+int32_t dvg_render_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene,
+                       const dvg_render_options *options, float *out_sdf);
+
+/* Renderer::eval_positions (renderer.rs:729-848 -> eval_positions_kernel,
+ * sdf.rs:207-262): the signed distance at `count` canvas points
+ * (positions: x, y pairs). */
+int32_t dvg_eval_positions(dvg_ctx *ctx, const dvg_scene_desc *scene,
+                           const dvg_render_options *options,
+                           const float *positions, uint64_t count, float *out);
+
 /* Renderer::render_backward (renderer.rs:548 -> backward_gpu.rs:21-523) with
  * d_render present and d_sdf absent. */
 int32_t dvg_render_backward(dvg_ctx *ctx, const dvg_scene_desc *scene,
@@ -386,6 +401,11 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pack,
 void dvg_device_scene_destroy(dvg_device_scene *scene);
 /* Forward into a device buffer of W*H*4 floats; asynchronous on the stream. */
 int32_t dvg_device_render(dvg_ctx *ctx, dvg_device_scene *scene, float *out_rgba_dev);
+/* dvg_render_sdf / dvg_eval_positions on a device scene (created with its
+ * backward pack) into device buffers; asynchronous on the stream. */
+int32_t dvg_device_render_sdf(dvg_ctx *ctx, dvg_device_scene *scene, float *out_sdf_dev);
+int32_t dvg_device_eval_positions(dvg_ctx *ctx, dvg_device_scene *scene, const float *positions_dev,
+                                  uint64_t count, float *out_dev);
 /* Backward from a device d_render (W*H*4) into device gradient arrays (raw
  * accumulations, as dvg_render_backward_packed); asynchronous on the stream. */
 int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,

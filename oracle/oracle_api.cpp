@@ -41,6 +41,19 @@ int oracle_render(const dvg_scene_desc *desc, const dvg_render_options *opt, flo
     return render_forward(sc, *opt, out, sample_colors, &err);
 }
 
+// Renderer::render_sdf: W*H signed distances (negative inside a fill).
+int oracle_render_sdf(const dvg_scene_desc *desc, const dvg_render_options *opt, float *out) {
+    Scene sc = read_scene(desc);
+    return render_sdf(sc, *opt, out);
+}
+
+// Renderer::eval_positions: the signed distance at n canvas points.
+int oracle_eval_positions(const dvg_scene_desc *desc, const dvg_render_options *opt, const float *pos, uint64_t n,
+                          float *out) {
+    Scene sc = read_scene(desc);
+    return eval_positions(sc, *opt, pos, (size_t)n, out);
+}
+
 // Pixel rows [y0, y1) of the forward render into out ((y1 - y0) * W * 4),
 // bit-identical to the same rows of oracle_render.
 int oracle_render_rows(const dvg_scene_desc *desc, const dvg_render_options *opt, uint32_t y0, uint32_t y1,

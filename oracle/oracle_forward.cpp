@@ -943,4 +943,204 @@ int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_im
     return DVG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Signed distance (Renderer::render_sdf / eval_positions, renderer.rs:586-848,
+// over src/gpu/kernels/sdf.rs:9-262 and distance.rs:9-513), f32 as the
+// reference GPU computes it.
+
+// closest_point_ellipse, distance.rs:83-139
+static void closest_point_ellipse(float cx, float cy, float rx_in, float ry_in, float px, float py, float &ox,
+                                  float &oy) {
+    float rx = abs_f(rx_in), ry = abs_f(ry_in);
+    const float eps = 1.0e-6f;
+    if (rx < eps && ry < eps) { ox = cx; oy = cy; return; }
+    if (rx < eps) { ox = cx; oy = cy + std::min(std::max(py - cy, -ry), ry); return; }
+    if (ry < eps) { ox = cx + std::min(std::max(px - cx, -rx), rx); oy = cy; return; }
+    float dx = px - cx, dy = py - cy;
+    if (abs_f(dx) < eps && abs_f(dy) < eps) { ox = cx + rx; oy = cy; return; }
+    float sgx = dx < 0.f ? -1.f : 1.f, sgy = dy < 0.f ? -1.f : 1.f;
+    dx = abs_f(dx);
+    dy = abs_f(dy);
+    float t = std::atan2(dy * rx, dx * ry);
+    for (int i = 0; i < 20; ++i) {
+        float s = std::sin(t), c = std::cos(t);
+        float g = rx * dx * s - ry * dy * c + (ry * ry - rx * rx) * s * c;
+        float gt = rx * dx * c + ry * dy * s + (ry * ry - rx * rx) * (c * c - s * s);
+        if (abs_f(gt) < 1.0e-12f) break;
+        float next = std::min(std::max(t - g / gt, 0.0f), 1.57079633f);
+        if (abs_f(next - t) < 1.0e-6f) { t = next; break; }
+        t = next;
+    }
+    float s = std::sin(t), c = std::cos(t);
+    ox = cx + sgx * rx * c;
+    oy = cy + sgy * ry * s;
+}
+
+// closest_point_shape, distance.rs:235-296 (shape space); false when a path
+// has no segment.
+static bool closest_point_shape(const Shape &sh, const PreparedInstance &pi, float px, float py, float &ox,
+                                float &oy) {
+    if (sh.kind == DVG_SHAPE_CIRCLE) {  // closest_point_circle, distance.rs:33-49
+        float cx = sh.center.x, cy = sh.center.y, r = std::fabs(sh.radius_c);
+        float dx = px - cx, dy = py - cy;
+        float len = sqrt_f(dx * dx + dy * dy);
+        if (len <= 1.0e-8f) { ox = cx + r; oy = cy; }
+        else { float inv = r / len; ox = cx + dx * inv; oy = cy + dy * inv; }
+        return true;
+    }
+    if (sh.kind == DVG_SHAPE_ELLIPSE) {
+        closest_point_ellipse(sh.center.x, sh.center.y, sh.radius_e.x, sh.radius_e.y, px, py, ox, oy);
+        return true;
+    }
+    if (sh.kind == DVG_SHAPE_RECT) {  // closest_point_rect, distance.rs:52-80
+        float x0 = sh.rmin.x, y0 = sh.rmin.y, x1 = sh.rmax.x, y1 = sh.rmax.y;
+        bool inside = px >= x0 && px <= x1 && py >= y0 && py <= y1;
+        if (!inside) {
+            ox = std::min(std::max(px, x0), x1);
+            oy = std::min(std::max(py, y0), y1);
+        } else {
+            float dl = px - x0, dr = x1 - px, db = py - y0, dt = y1 - py;
+            if (dl <= dr && dl <= db && dl <= dt) { ox = x0; oy = py; }
+            else if (dr <= db && dr <= dt) { ox = x1; oy = py; }
+            else if (db <= dt) { ox = px; oy = y0; }
+            else { ox = px; oy = y1; }
+        }
+        return true;
+    }
+    // closest_point_path, distance.rs:142-231: the first segment reaching the
+    // smallest distance; the point from its parameter
+    bool approx = sh.path.use_distance_approx;
+    float best = 1.0e20f;
+    for (const Seg &s : pi.segs) {
+        float d, t;
+        if (s.kind == 0) distance_to_segment_with_t(px, py, s.p0.x, s.p0.y, s.p1.x, s.p1.y, d, t);
+        else if (s.kind == 1) closest_quadratic(px, py, s.p0.x, s.p0.y, s.p1.x, s.p1.y, s.p2.x, s.p2.y, approx, d, t);
+        else closest_cubic(px, py, s.p0.x, s.p0.y, s.p1.x, s.p1.y, s.p2.x, s.p2.y, s.p3.x, s.p3.y, approx, d, t);
+        if (d < best) {
+            best = d;
+            float tt = 1.0f - t;
+            if (s.kind == 0) {
+                ox = s.p0.x + t * (s.p1.x - s.p0.x);
+                oy = s.p0.y + t * (s.p1.y - s.p0.y);
+            } else if (s.kind == 1) {
+                ox = tt * tt * s.p0.x + 2.0f * tt * t * s.p1.x + t * t * s.p2.x;
+                oy = tt * tt * s.p0.y + 2.0f * tt * t * s.p1.y + t * t * s.p2.y;
+            } else {
+                ox = tt * tt * tt * s.p0.x + 3.0f * tt * tt * t * s.p1.x + 3.0f * tt * t * t * s.p2.x +
+                     t * t * t * s.p3.x;
+                oy = tt * tt * tt * s.p0.y + 3.0f * tt * tt * t * s.p1.y + 3.0f * tt * t * t * s.p2.y +
+                     t * t * t * s.p3.y;
+            }
+        }
+    }
+    return best < 1.0e20f;
+}
+
+// compute_signed_distance, sdf.rs:9-117: the nearest shape boundary over the
+// groups from last to first (compute_distance_group, distance.rs:298-513,
+// with shapes skipped when their group-local box is farther than the best
+// distance so far, as the group BVH skips nodes), negative inside the fill of
+// the nearest shape's group (fill_inside_group, backward.rs:1529-1606: the
+// forward's fill test).
+static float signed_distance(const Scene &sc, const std::vector<PreparedGroup> &pgs, float px, float py) {
+    float best = 1.0e20f;
+    size_t best_g = 0;
+    bool found = false;
+    for (size_t gi = sc.groups.size(); gi-- > 0;) {
+        const Group &g = sc.groups[gi];
+        const PreparedGroup &pg = pgs[gi];
+        const float *x = pg.xform;
+        float lx = x[0] * px + x[1] * py + x[2];
+        float ly = x[3] * px + x[4] * py + x[5];
+        for (const PreparedInstance &pi : pg.shapes) {
+            if (pi.b[0] <= pi.b[2] && pi.b[1] <= pi.b[3]) {  // bounds_distance, distance.rs:10-15
+                float ddx = max_f(max_f(pi.b[0] - lx, 0.0f), lx - pi.b[2]);
+                float ddy = max_f(max_f(pi.b[1] - ly, 0.0f), ly - pi.b[3]);
+                if (sqrt_f(ddx * ddx + ddy * ddy) > best) continue;
+            }
+            const float *iv = pi.inv;
+            float spx = iv[0] * lx + iv[1] * ly + iv[2];
+            float spy = iv[3] * lx + iv[4] * ly + iv[5];
+            const Shape &sh = sc.shapes[pi.shape];
+            float cx = 0.f, cy = 0.f;
+            if (!closest_point_shape(sh, pi, spx, spy, cx, cy)) continue;
+            const M3 &T = sh.transform;
+            float gx = T.m[0][0] * cx + T.m[0][1] * cy + T.m[0][2];
+            float gy = T.m[1][0] * cx + T.m[1][1] * cy + T.m[1][2];
+            const M3 &Gc = g.shape_to_canvas;
+            float ccx = Gc.m[0][0] * gx + Gc.m[0][1] * gy + Gc.m[0][2];
+            float ccy = Gc.m[1][0] * gx + Gc.m[1][1] * gy + Gc.m[1][2];
+            float dx = ccx - px, dy = ccy - py;
+            float d = sqrt_f(dx * dx + dy * dy);
+            if (d < best) {
+                best = d;
+                best_g = gi;
+                found = true;
+            }
+        }
+    }
+    if (!found) return 0.0f;
+    const Group &g = sc.groups[best_g];
+    bool inside = false;
+    if (g.fill.kind != DVG_PAINT_NONE) {
+        const PreparedGroup &pg = pgs[best_g];
+        const float *x = pg.xform;
+        float lx = x[0] * px + x[1] * py + x[2];
+        float ly = x[3] * px + x[4] * py + x[5];
+        GroupAccum acc;
+        for (const PreparedInstance &pi : pg.shapes) {
+            bool inb = pi.b[0] <= pi.b[2] && pi.b[1] <= pi.b[3];
+            if (inb && (lx < pi.b[0] || lx > pi.b[2] || ly < pi.b[1] || ly > pi.b[3])) inb = false;
+            if (!inb) continue;
+            const float *iv = pi.inv;
+            float spx = iv[0] * lx + iv[1] * ly + iv[2];
+            float spy = iv[3] * lx + iv[4] * ly + iv[5];
+            shape_fill(sc.shapes[pi.shape], pi, g.fill_rule, false, spx, spy, acc);
+        }
+        inside = g.fill_rule == 1 ? acc.crossings > 0.f : acc.winding != 0.f;
+    }
+    return inside ? -best : best;
+}
+
+// render_sdf_kernel, sdf.rs:119-200: the mean signed distance of each pixel's
+// samples (the forward's jittered sample positions).
+int render_sdf(const Scene &sc, const dvg_render_options &opt, float *out) {
+    uint32_t W = sc.width, H = sc.height;
+    size_t npix = (size_t)W * H;
+    for (size_t i = 0; i < npix; ++i) out[i] = 0.0f;
+    if (npix == 0 || sc.groups.empty()) return DVG_OK;
+    std::vector<PreparedGroup> pgs;
+    prepare(sc, opt, pgs);
+    uint32_t nsx = std::max(opt.samples_x, 1u), nsy = std::max(opt.samples_y, 1u);
+    bool jitter = !opt.use_prefiltering && opt.jitter != 0;
+    float inv_sx = 1.0f / (float)nsx, inv_sy = 1.0f / (float)nsy;
+    float inv_n = 1.0f / (float)(nsx * nsy);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            float acc = 0.0f;
+            for (uint32_t sy = 0; sy < nsy; ++sy)
+                for (uint32_t sx = 0; sx < nsx; ++sx) {
+                    float rx, ry;
+                    jitter_xy(x, y, sx, sy, W, nsx, nsy, opt.seed, jitter, rx, ry);
+                    float px = (float)x + ((float)sx + rx) * inv_sx;
+                    float py = (float)y + ((float)sy + ry) * inv_sy;
+                    acc += signed_distance(sc, pgs, px, py);
+                }
+            out[(size_t)y * W + x] = acc * inv_n;
+        }
+    return DVG_OK;
+}
+
+// eval_positions_kernel, sdf.rs:207-262: the signed distance at each canvas
+// point (x, y pairs).
+int eval_positions(const Scene &sc, const dvg_render_options &opt, const float *pos, size_t n, float *out) {
+    for (size_t i = 0; i < n; ++i) out[i] = 0.0f;
+    if (n == 0 || sc.groups.empty()) return DVG_OK;
+    std::vector<PreparedGroup> pgs;
+    prepare(sc, opt, pgs);
+    for (size_t i = 0; i < n; ++i) out[i] = signed_distance(sc, pgs, pos[2 * i], pos[2 * i + 1]);
+    return DVG_OK;
+}
+
 }  // namespace oracle

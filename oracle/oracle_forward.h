@@ -36,4 +36,8 @@ void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, 
 // bit-identical to those of the whole-image render.
 int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_img, float *sample_colors,
                    const char **err, uint32_t y0 = 0, uint32_t y1 = 0xffffffffu);
+// Renderer::render_sdf (W*H mean signed distances) and eval_positions (n
+// points, x y pairs), the reference GPU's f32 arithmetic.
+int render_sdf(const Scene &sc, const dvg_render_options &opt, float *out);
+int eval_positions(const Scene &sc, const dvg_render_options &opt, const float *pos, size_t n, float *out);
 }  // namespace oracle

@@ -34,6 +34,11 @@ def lib():
         L.oracle_render_rows.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), C.c_uint32,
                                          C.c_uint32, f32p]
         L.oracle_render_rows.restype = C.c_int
+        L.oracle_render_sdf.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p]
+        L.oracle_render_sdf.restype = C.c_int
+        L.oracle_eval_positions.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p,
+                                            C.c_uint64, f32p]
+        L.oracle_eval_positions.restype = C.c_int
         L.oracle_render_backward_rows.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions),
                                                   C.POINTER(abi.BackwardOptions), f32p, C.POINTER(abi.GradsC),
                                                   C.c_uint32, C.c_uint32]
@@ -74,6 +79,28 @@ def render(desc, opts, want_samples=False):
     if st != 0:
         raise RuntimeError(f"oracle_render status {st}")
     return (out, samples) if want_samples else out
+
+
+def render_sdf(desc, opts):
+    """Renderer::render_sdf: W * H mean signed distances of each pixel's
+    samples (negative inside a fill)."""
+    sc = desc.scene
+    out = np.zeros(sc.width * sc.height, np.float32)
+    st = lib().oracle_render_sdf(desc.ref(), C.byref(opts.to_c()), _fp(out))
+    if st != 0:
+        raise RuntimeError(f"oracle_render_sdf status {st}")
+    return out
+
+
+def eval_positions(desc, opts, positions):
+    """Renderer::eval_positions: the signed distance at each canvas point
+    (positions: n x 2)."""
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    out = np.zeros(pos.size // 2, np.float32)
+    st = lib().oracle_eval_positions(desc.ref(), C.byref(opts.to_c()), _fp(pos), pos.size // 2, _fp(out))
+    if st != 0:
+        raise RuntimeError(f"oracle_eval_positions status {st}")
+    return out
 
 
 def render_rows(desc, opts, y0, y1):

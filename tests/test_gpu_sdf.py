@@ -1,0 +1,85 @@
+"""GPU: signed distance (dvg_render_sdf / dvg_eval_positions, Renderer::
+render_sdf / eval_positions, renderer.rs:586-848 over sdf.rs:9-262) against
+the oracle.
+
+Whole images on small scenes of every shape kind and sampling mode; points
+scattered over the 1024^2 canvas of 1000 cubic paths.  The closest-point
+solvers are the forward's (f32, bit-compatible with the oracle); the ellipse
+Newton iteration calls the f32 atan2 / sin / cos of each side's library, so
+values may differ in the last bits.  Tolerance: |gpu - oracle| <= 1e-5 +
+1e-5 |v| for all but 0.1 % of the values (a sample next to an edge can take
+the other sign there: both values are then within rounding of zero), and
+<= 1e-3 everywhere."""
+import numpy as np
+import pytest
+
+from pydvg import scenes
+from pydvg.scene import Circle, Ellipse, RenderOptions, Scene, Shape, ShapeGroup, Solid, SceneDesc
+
+pytestmark = pytest.mark.gpu
+
+TOL, FRACTION, MAX_ABS = 1e-5, 1e-3, 1e-3
+
+
+def _check(name, got, ref):
+    d = np.abs(got - ref)
+    bad = d > TOL + TOL * np.abs(ref)
+    print(f"{name}: {got.size} values, max |diff| {d.max():.3g}, beyond tolerance {int(bad.sum())}", flush=True)
+    assert np.all(np.isfinite(got))
+    assert bad.mean() <= FRACTION
+    assert d.max() <= MAX_ABS
+
+
+def _scaled_scene():
+    shapes = [Shape(Circle((8.0, 9.0), 4.0)), Shape(Ellipse((20.0, 12.0), (6.0, 3.0))),
+              Shape(Circle((30.0, 30.0), 5.0), stroke_width=1.5)]
+    g0 = ShapeGroup(shape_indices=[0, 1], fill=Solid((0.2, 0.3, 0.4, 1.0)))
+    g0.set_shape_to_canvas(np.array([[1.5, 0.2, 3.0], [-0.1, 1.2, -2.0], [0, 0, 1]], np.float32))
+    g1 = ShapeGroup(shape_indices=[2], stroke=Solid((0, 0, 0, 1)))
+    return Scene(48, 44, shapes=shapes, groups=[g0, g1], background=(1, 1, 1, 1))
+
+
+def _cases():
+    pf = RenderOptions(samples_x=1, samples_y=1, seed=0, jitter=False, use_prefiltering=True)
+    return {
+        "cubic48": scenes.small_cubic_scene(48, 40, 6, seed=11, spp=2),
+        "mixed64": (scenes.mixed_scene(64, 64, 20, seed=12), scenes.config2()[1]),
+        "prefiltered48": scenes.prefiltered_scene(48, 48, 6, 3, seed=13, step=8.0, radius=(4.0, 12.0),
+                                                  seg_range=(2, 5)),
+        "mixed32_pf": (scenes.mixed_scene(32, 32, 10, seed=14), pf),
+        "scaled_groups": (_scaled_scene(), RenderOptions(samples_x=2, samples_y=2, seed=3)),
+    }
+
+
+@pytest.mark.parametrize("name", list(_cases().keys()))
+def test_render_sdf_matches_oracle(renderer, oracle, name):
+    sc, opts = _cases()[name]
+    print(f"case sdf {name} {sc.width}x{sc.height}", flush=True)
+    got = renderer.render_sdf(sc, opts)
+    ref = oracle.render_sdf(SceneDesc(sc), opts)
+    _check(name, got, ref)
+    assert (ref < 0).any() and (ref > 0).any()
+
+
+@pytest.mark.parametrize("name", ["mixed64", "scaled_groups"])
+def test_eval_positions_matches_oracle(renderer, oracle, name):
+    sc, opts = _cases()[name]
+    rng = np.random.default_rng(4)
+    p = np.stack([rng.uniform(-8, sc.width + 8, 3000), rng.uniform(-8, sc.height + 8, 3000)], 1).astype(np.float32)
+    print(f"case eval_positions {name}", flush=True)
+    _check(name, renderer.eval_positions(sc, p, opts), oracle.eval_positions(SceneDesc(sc), opts, p))
+
+
+def test_eval_positions_full_canvas(renderer, oracle):
+    """2000 points over the 1024^2 canvas of 1000 cubic paths."""
+    sc, opts = scenes.headline()
+    rng = np.random.default_rng(5)
+    p = rng.uniform(0, 1024, (2000, 2)).astype(np.float32)
+    print("case eval_positions headline", flush=True)
+    _check("headline points", renderer.eval_positions(sc, p, opts), oracle.eval_positions(SceneDesc(sc), opts, p))
+
+
+def test_empty_inputs(renderer):
+    sc = Scene(16, 8)
+    assert np.array_equal(renderer.render_sdf(sc, RenderOptions()), np.zeros(128, np.float32))
+    assert renderer.eval_positions(sc, np.zeros((0, 2), np.float32), RenderOptions()).size == 0

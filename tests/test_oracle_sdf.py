@@ -1,0 +1,102 @@
+"""The oracle's signed distance (Renderer::render_sdf / eval_positions,
+renderer.rs:586-848 over sdf.rs:9-262) pinned by closed forms.
+
+A filled circle's signed distance is |p - c| - r; a filled axis-aligned box's
+is the Euclidean distance to the box outside and minus the distance to the
+nearest edge inside; a square polygon path gives the box's values; a group
+scaled by s maps a circle of radius r to one of radius s r; an unfilled
+(stroke-only) group is never "inside"; several shapes give the minimum with
+the sign of the nearest one's group; render_sdf without jitter at one sample
+a pixel is eval_positions at the pixel centres.  A wrong transform order, a
+dropped sign or a min taken over the wrong set fails one of these."""
+import numpy as np
+import pytest
+
+from pydvg import abi
+from pydvg.scene import Circle, Path, Rect, RenderOptions, Scene, Shape, ShapeGroup, Solid, SceneDesc
+
+OPTS = RenderOptions(samples_x=1, samples_y=1, jitter=False, use_prefiltering=False)
+
+
+def _scene(shapes, groups, w=40, h=32):
+    return Scene(w, h, shapes=shapes, groups=groups, background=(1.0, 1.0, 1.0, 1.0))
+
+
+def _fill(idx, **kw):
+    return ShapeGroup(shape_indices=idx, fill=Solid((0.2, 0.4, 0.6, 1.0)), **kw)
+
+
+def _pts(rng, n=400, w=40, h=32):
+    return np.stack([rng.uniform(-5, w + 5, n), rng.uniform(-5, h + 5, n)], 1).astype(np.float32)
+
+
+def _box_sdf(p, lo, hi):
+    dx = np.maximum(np.maximum(lo[0] - p[:, 0], 0), p[:, 0] - hi[0])
+    dy = np.maximum(np.maximum(lo[1] - p[:, 1], 0), p[:, 1] - hi[1])
+    out = np.hypot(dx, dy)
+    inside = (p[:, 0] >= lo[0]) & (p[:, 0] <= hi[0]) & (p[:, 1] >= lo[1]) & (p[:, 1] <= hi[1])
+    edge = np.minimum.reduce([p[:, 0] - lo[0], hi[0] - p[:, 0], p[:, 1] - lo[1], hi[1] - p[:, 1]])
+    return np.where(inside, -edge, out)
+
+
+def test_circle_is_distance_to_centre_minus_radius(oracle):
+    sc = _scene([Shape(Circle((20.0, 16.0), 7.0))], [_fill([0])])
+    p = _pts(np.random.default_rng(0))
+    got = oracle.eval_positions(SceneDesc(sc), OPTS, p)
+    want = np.hypot(p[:, 0] - 20.0, p[:, 1] - 16.0) - 7.0
+    assert np.allclose(got, want, atol=2e-5, rtol=1e-5)
+    assert (got < 0).any() and (got > 0).any()
+
+
+@pytest.mark.parametrize("as_path", [False, True])
+def test_box_and_square_polygon(oracle, as_path):
+    lo, hi = (5.0, 6.0), (25.0, 20.0)
+    if as_path:
+        geom = Path(num_control_points=[0, 0, 0, 0],
+                    points=[[lo[0], lo[1]], [hi[0], lo[1]], [hi[0], hi[1]], [lo[0], hi[1]]], is_closed=True)
+    else:
+        geom = Rect(lo, hi)
+    sc = _scene([Shape(geom)], [_fill([0])])
+    p = _pts(np.random.default_rng(1))
+    got = oracle.eval_positions(SceneDesc(sc), OPTS, p)
+    assert np.allclose(got, _box_sdf(p, lo, hi), atol=2e-5, rtol=1e-5)
+
+
+def test_group_scale_maps_the_distance(oracle):
+    s = 2.0
+    g = _fill([0])
+    g.set_shape_to_canvas(np.array([[s, 0, 3.0], [0, s, -2.0], [0, 0, 1]], np.float32))
+    sc = _scene([Shape(Circle((8.0, 9.0), 4.0))], [g])
+    p = _pts(np.random.default_rng(2))
+    got = oracle.eval_positions(SceneDesc(sc), OPTS, p)
+    c = np.array([8.0 * s + 3.0, 9.0 * s - 2.0])
+    want = np.hypot(p[:, 0] - c[0], p[:, 1] - c[1]) - 4.0 * s
+    assert np.allclose(got, want, atol=5e-5, rtol=1e-5)
+
+
+def test_nearest_shape_and_its_group_sign(oracle):
+    shapes = [Shape(Circle((10.0, 10.0), 5.0)), Shape(Circle((30.0, 20.0), 6.0))]
+    stroke_only = ShapeGroup(shape_indices=[1], stroke=Solid((0, 0, 0, 1)))
+    sc = _scene(shapes, [_fill([0]), stroke_only])
+    p = _pts(np.random.default_rng(3))
+    got = oracle.eval_positions(SceneDesc(sc), OPTS, p)
+    d0 = np.hypot(p[:, 0] - 10.0, p[:, 1] - 10.0) - 5.0
+    d1 = np.abs(np.hypot(p[:, 0] - 30.0, p[:, 1] - 20.0) - 6.0)  # unfilled: never inside
+    want = np.where(np.abs(d0) < d1, d0, d1)
+    assert np.allclose(got, want, atol=2e-5, rtol=1e-5)
+
+
+def test_render_sdf_is_eval_at_pixel_centres(oracle):
+    sc = _scene([Shape(Circle((20.0, 16.0), 7.0)), Shape(Rect((2.0, 3.0), (9.0, 30.0)))], [_fill([0]), _fill([1])])
+    desc = SceneDesc(sc)
+    img = oracle.render_sdf(desc, OPTS)
+    yy, xx = np.mgrid[0:sc.height, 0:sc.width]
+    p = np.stack([xx.reshape(-1) + 0.5, yy.reshape(-1) + 0.5], 1).astype(np.float32)
+    assert np.array_equal(img, oracle.eval_positions(desc, OPTS, p))
+
+
+def test_empty_scene_is_zero(oracle):
+    sc = _scene([], [])
+    assert np.array_equal(oracle.render_sdf(SceneDesc(sc), OPTS), np.zeros(40 * 32, np.float32))
+    assert np.array_equal(oracle.eval_positions(SceneDesc(sc), OPTS, np.zeros((3, 2), np.float32)),
+                          np.zeros(3, np.float32))
