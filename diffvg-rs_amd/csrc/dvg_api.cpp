@@ -703,6 +703,25 @@ static void grads_to_dev(const dvg_grads *g, dvgk::DevGrads &G) {
     G.d_translation = g->d_translation;
 }
 
+// Renderer::eval_positions_backward on a device scene: zeroes the device
+// gradient arrays, then accumulates the points' gradients.
+int32_t dvg_device_eval_positions_backward(dvg_ctx *ctx, dvg_device_scene *ds, const float *positions,
+                                           uint64_t count, const float *d_sdf, const dvg_grads *grads) {
+    if (!ctx || !ds || !grads || (count && (!positions || !d_sdf)))
+        return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+    if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
+        return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvgk::DevGrads G;
+    grads_to_dev(grads, G);
+    dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
+    if (ds->S.num_groups > 0) dvgk::launch_sdf_points_backward(ds->S, positions, (uint32_t)count, d_sdf, G, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
+
 int32_t dvg_device_scene_update(dvg_ctx *ctx, dvg_device_scene *ds, const float *path_points,
                                 const float *group_colors) {
     if (!ctx || !ds) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
@@ -1021,6 +1040,62 @@ int sdf_host(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options
     return st;
 }
 }  // namespace
+
+int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                                    const dvg_backward_options *bopt, const float *positions, uint64_t count,
+                                    const float *d_sdf, const dvg_grads *grads) {
+    if (!ctx || !scene || !options || !grads || (count && (!positions || !d_sdf)))
+        return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+    dvg_prepared P;
+    const char *err = "";
+    int st = dvgh::prepare(scene, options, P, &err);
+    if (st) return set_err(ctx, st, err);
+    dvg_scene_pack pk;
+    dvg_backward_pack bp;
+    dvg_render_params prm;
+    dvgh::pack_views(P, &pk, &bp, &prm);
+    prm.compute_translation = (bopt && bopt->compute_translation && grads->d_translation) ? 1u : 0u;
+    GradLayout L = grad_layout(&pk, &bp, &prm);
+    for (int k = 0; k < 14; ++k) {
+        float *dst = *grad_slot(grads, k);
+        if (dst && L.len[k]) std::memset(dst, 0, L.len[k] * sizeof(float));
+    }
+    if (count == 0 || scene->num_groups == 0) return DVG_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvg_device_scene *ds = nullptr;
+    st = dvg_device_scene_create(ctx, &pk, &bp, &prm, &ds);
+    if (st) return st;
+    DevBuf gbuf, pb, db;
+    hipError_t e = hipMalloc(&gbuf.p, std::max(L.total, (size_t)1) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&pb.p, count * 2 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&db.p, count * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpyAsync(pb.p, positions, count * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(db.p, d_sdf, count * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+        dvg_device_scene_destroy(ds);
+        return hip_err(ctx, e, "positions backward buffers");
+    }
+    dvg_grads dg;
+    float *gb = (float *)gbuf.p;
+    for (int k = 0; k < 14; ++k) *grad_slot(&dg, k) = L.len[k] ? gb + L.off[k] : nullptr;
+    if (!(*grad_slot(grads, 12))) dg.d_background_image = nullptr;
+    if (!(bopt && bopt->compute_translation && *grad_slot(grads, 13))) dg.d_translation = nullptr;
+    st = dvg_device_eval_positions_backward(ctx, ds, (const float *)pb.p, count, (const float *)db.p, &dg);
+    if (st == DVG_OK) {
+        std::vector<float> host(std::max(L.total, (size_t)1));
+        e = hipMemcpyAsync(host.data(), gbuf.p, L.total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hip_err(ctx, e, "gradient readback");
+        else
+            for (int k = 0; k < 14; ++k) {
+                float *dst = *grad_slot(grads, k);
+                if (dst && L.len[k] && *grad_slot(&dg, k)) std::memcpy(dst, host.data() + L.off[k], L.len[k] * sizeof(float));
+            }
+    }
+    dvg_device_scene_destroy(ds);
+    return st;
+}
 
 int32_t dvg_render_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options, float *out) {
     if (!ctx || !scene || !options || !out) return set_err(ctx, DVG_INVALID_SCENE, "null argument");

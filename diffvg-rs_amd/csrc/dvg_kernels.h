@@ -92,6 +92,10 @@ void launch_resolve(const DevScene &S, const float *weight, const float *accum, 
 // prepared_to_scene).
 void launch_sdf(const DevScene &S, float *out, hipStream_t stream);
 void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream);
+// Gradients of the signed distances at n points for the adjoint d_sdf (the
+// reference CPU's render_backward_positions), accumulated into G.
+void launch_sdf_points_backward(const DevScene &S, const float *pos, uint32_t n, const float *d_sdf,
+                                const DevGrads &G, hipStream_t stream);
 // Bit-set slots of the colour adjoint for tile lists longer than the LDS bit
 // set (LDS_MASK_GROUPS entries): `slots` workgroup slots of
 // nw = (2 gcap + 31) / 32 words x BLOCK lanes at gmask, and one queue counter.

@@ -3217,6 +3217,46 @@ __global__ __launch_bounds__(256) void k_sdf_points(DevScene S, const float *__r
     out[i] = signed_distance(S, pos[2u * i], pos[2u * i + 1u]);
 }
 
+// render_backward_positions (backward/render.rs:213-280) over sample_distance
+// (backward/sampling.rs:14-68), one point a lane: the nearest boundary over
+// the groups from last to first (the CPU distance query), d_compute_distance
+// of it with the sign of the group's CPU fill test; translation into the
+// pixel holding the point (coordinates truncated toward zero).
+__global__ __launch_bounds__(256) void k_sdf_points_backward(DevScene S, const float *__restrict__ pos, uint32_t n,
+                                                             const float *__restrict__ d_sdf, DevGrads G) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float d_dist = d_sdf[i];
+    if (d_dist == 0.f) return;
+    float px = pos[2u * i], py = pos[2u * i + 1u];
+    float best = __builtin_inff();
+    uint32_t best_g = 0;
+    bool found = false;
+    PHit bh = {};
+    for (uint32_t gg = S.num_groups; gg > 0; --gg) {
+        PHit h;
+        if (group_closest(S, gg - 1u, px, py, best, h) && h.dist < best) {
+            best = h.dist;
+            best_g = gg - 1u;
+            bh = h;
+            found = true;
+        }
+    }
+    if (!found) return;
+    const float *gd = S.group_data + best_g * 16u;
+    bool inside = (uint32_t)gd[2] != DVG_PAINT_NONE && group_inside(S, best_g, px, py);
+    float dtx = 0.f, dty = 0.f;
+    d_distance_query(S, G, best_g, bh, px, py, inside ? -d_dist : d_dist, dtx, dty);
+    if (G.d_translation) {
+        float tx = truncf(px), ty = truncf(py);
+        if (tx >= 0.f && ty >= 0.f && tx < (float)S.W && ty < (float)S.H) {
+            size_t pix = (size_t)ty * S.W + (size_t)tx;
+            unsafeAtomicAdd(G.d_translation + pix * 2u, dtx);
+            unsafeAtomicAdd(G.d_translation + pix * 2u + 1u, dty);
+        }
+    }
+}
+
 // Winding terms of every segment (the record described with WIND_STRIDE).
 __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurves, float *__restrict__ out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3582,6 +3622,12 @@ void launch_sdf(const DevScene &S, float *out, hipStream_t stream) {
 void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_sdf_points, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, pos, n, out);
+}
+
+void launch_sdf_points_backward(const DevScene &S, const float *pos, uint32_t n, const float *d_sdf,
+                                const DevGrads &G, hipStream_t stream) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_sdf_points_backward, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, pos, n, d_sdf, G);
 }
 
 void launch_resolve(const DevScene &S, const float *weight, const float *accum, float *out, hipStream_t stream) {

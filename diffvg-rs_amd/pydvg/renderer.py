@@ -101,6 +101,24 @@ class Renderer:
         _check(self.lib, self.ctx, st)
         return out
 
+    def eval_positions_backward(self, scene: Scene, positions: np.ndarray, options: RenderOptions,
+                                d_sdf: np.ndarray,
+                                backward_options: BackwardOptions = BackwardOptions()) -> FlatGrads:
+        """Renderer::eval_positions_backward: gradients of the signed distances
+        at the points (n x 2) for the adjoint d_sdf (n)."""
+        desc = SceneDesc(scene)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        d_sdf = np.ascontiguousarray(d_sdf, np.float32).reshape(-1)
+        if d_sdf.size != pos.size // 2:
+            raise RenderError("InvalidScene", "d_sdf_image size mismatch for eval_positions")
+        g = FlatGrads(desc, backward_options.compute_translation)
+        st = self.lib.dvg_eval_positions_backward(self.ctx, desc.ref(), C.byref(options.to_c()),
+                                                  C.byref(backward_options.to_c()),
+                                                  pos.ctypes.data_as(C.POINTER(C.c_float)), d_sdf.size,
+                                                  d_sdf.ctypes.data_as(C.POINTER(C.c_float)), g.ref())
+        _check(self.lib, self.ctx, st)
+        return g
+
     def render_backward(self, scene: Scene, options: RenderOptions, d_render: np.ndarray,
                         backward_options: BackwardOptions = BackwardOptions()) -> FlatGrads:
         desc = SceneDesc(scene)

@@ -355,6 +355,18 @@ int32_t dvg_eval_positions(dvg_ctx *ctx, const dvg_scene_desc *scene,
                            const dvg_render_options *options,
                            const float *positions, uint64_t count, float *out);
 
+/* Renderer::eval_positions_backward (renderer.rs:567-583 ->
+ * render_backward_positions, src/backward/render.rs:213-280, which the
+ * reference runs on the CPU): the gradients of the signed distances at
+ * `count` points for the adjoint d_sdf (count floats), into the flat
+ * gradient layout of dvg_render_backward (zeroed first; d_translation, when
+ * requested, at the pixel holding each point). */
+int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene,
+                                    const dvg_render_options *options,
+                                    const dvg_backward_options *backward_options,
+                                    const float *positions, uint64_t count,
+                                    const float *d_sdf, const dvg_grads *grads);
+
 /* Renderer::render_backward (renderer.rs:548 -> backward_gpu.rs:21-523) with
  * d_render present and d_sdf absent. */
 int32_t dvg_render_backward(dvg_ctx *ctx, const dvg_scene_desc *scene,
@@ -406,6 +418,11 @@ int32_t dvg_device_render(dvg_ctx *ctx, dvg_device_scene *scene, float *out_rgba
 int32_t dvg_device_render_sdf(dvg_ctx *ctx, dvg_device_scene *scene, float *out_sdf_dev);
 int32_t dvg_device_eval_positions(dvg_ctx *ctx, dvg_device_scene *scene, const float *positions_dev,
                                   uint64_t count, float *out_dev);
+/* dvg_eval_positions_backward on a device scene: device positions, d_sdf
+ * and gradient arrays (zeroed, then accumulated); asynchronous. */
+int32_t dvg_device_eval_positions_backward(dvg_ctx *ctx, dvg_device_scene *scene,
+                                           const float *positions_dev, uint64_t count,
+                                           const float *d_sdf_dev, const dvg_grads *grads_dev);
 /* Backward from a device d_render (W*H*4) into device gradient arrays (raw
  * accumulations, as dvg_render_backward_packed); asynchronous on the stream. */
 int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,

@@ -57,6 +57,11 @@ V4 gather_d_color(const Scene &sc, const float *d_render, const std::vector<floa
 void accumulate_filter_gradient(const Scene &sc, V4 color, const float *d_render, const std::vector<float> &wimg,
                                 V2 pt, float &d_radius);
 void finalize_background_gradients(const Scene &sc, Grads &gr);
+// render_backward_positions (backward/render.rs:213-280): gradients of the
+// signed distances at n points (x, y pairs) for the adjoint d_sdf.  Defined
+// in oracle_prefilter.cpp.
+void backward_positions(const Scene &sc, bool compute_translation, const float *pos, size_t n, const float *d_sdf,
+                        Grads &gr);
 
 // Prefiltered colour of one sample with its adjoint (sampling.rs:227-482);
 // gr == nullptr skips the adjoint.  Defined in oracle_prefilter.cpp.

@@ -609,5 +609,41 @@ V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 
     return V4(fr, fg, fb, fa);
 }
 
+// render_backward_positions (backward/render.rs:213-280) over
+// sample_distance (backward/sampling.rs:14-68): for each point with a
+// nonzero d_sdf, the nearest boundary over the groups from last to first,
+// and d_compute_distance of it with the sign of the group's fill test.
+void backward_positions(const Scene &sc, bool compute_translation, const float *pos, size_t n, const float *d_sdf,
+                        Grads &gr) {
+    Ctx cx;
+    make_ctx(sc, cx);
+    for (size_t i = 0; i < n; ++i) {
+        float d_dist = d_sdf[i];
+        if (d_dist == 0.0f) continue;
+        V2 pt(pos[2 * i], pos[2 * i + 1]);
+        long tidx = -1;
+        if (compute_translation) {  // translation_index_for_point: `as i32` truncates toward zero
+            long x = (long)std::trunc(pt.x), y = (long)std::trunc(pt.y);
+            if (x >= 0 && y >= 0 && x < (long)sc.width && y < (long)sc.height) tidx = y * (long)sc.width + x;
+        }
+        float best = INFINITY;
+        long best_g = -1;
+        Hit bh;
+        for (size_t gi = sc.groups.size(); gi-- > 0;) {
+            Hit h;
+            if (compute_distance(cx, (uint32_t)gi, pt, best, h) && h.distance < best) {
+                best = h.distance;
+                best_g = (long)gi;
+                bh = h;
+            }
+        }
+        if (best_g < 0) continue;
+        bool inside = sc.groups[best_g].fill.kind != DVG_PAINT_NONE && is_inside_group(cx, (uint32_t)best_g, pt);
+        float d_abs = inside ? -d_dist : d_dist;
+        V2 d_pt = d_compute_distance(cx, (uint32_t)best_g, bh.shape, pt, bh.point, bh, d_abs, gr);
+        accumulate_translation(gr, tidx, d_pt * -1.0f);
+    }
+}
+
 }  // namespace cpu
 }  // namespace oracle

@@ -36,6 +36,9 @@ def lib():
         L.oracle_render_rows.restype = C.c_int
         L.oracle_render_sdf.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p]
         L.oracle_render_sdf.restype = C.c_int
+        L.oracle_eval_positions_backward.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.BackwardOptions),
+                                                     f32p, C.c_uint64, f32p, C.POINTER(abi.GradsC)]
+        L.oracle_eval_positions_backward.restype = C.c_int
         L.oracle_eval_positions.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p,
                                             C.c_uint64, f32p]
         L.oracle_eval_positions.restype = C.c_int
@@ -101,6 +104,20 @@ def eval_positions(desc, opts, positions):
     if st != 0:
         raise RuntimeError(f"oracle_eval_positions status {st}")
     return out
+
+
+def eval_positions_backward(desc, positions, d_sdf, translation=False):
+    """Renderer::eval_positions_backward (render_backward_positions, the
+    reference CPU's): FlatGrads of the signed distances at the points."""
+    from pydvg.scene import BackwardOptions, FlatGrads
+    g = FlatGrads(desc, translation)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+    d_sdf = np.ascontiguousarray(d_sdf, np.float32).reshape(-1)
+    st = lib().oracle_eval_positions_backward(desc.ref(), C.byref(BackwardOptions(translation).to_c()), _fp(pos),
+                                              d_sdf.size, _fp(d_sdf), g.ref())
+    if st != 0:
+        raise RuntimeError(f"oracle_eval_positions_backward status {st}")
+    return g
 
 
 def render_rows(desc, opts, y0, y1):

@@ -83,3 +83,24 @@ def test_empty_inputs(renderer):
     sc = Scene(16, 8)
     assert np.array_equal(renderer.render_sdf(sc, RenderOptions()), np.zeros(128, np.float32))
     assert renderer.eval_positions(sc, np.zeros((0, 2), np.float32), RenderOptions()).size == 0
+
+
+@pytest.mark.parametrize("name", ["cubic48", "mixed64", "scaled_groups"])
+def test_eval_positions_backward_matches_oracle(renderer, oracle, name):
+    """eval_positions_backward (the reference runs it on the CPU,
+    backward/render.rs:213-280) against the oracle's restatement, with
+    translation gradients; relative L2 over the gradient arrays <= 1e-3, the
+    gradient tolerance of the render backward."""
+    from parity_util import grad_report
+    from pydvg.scene import BackwardOptions
+    sc, opts = _cases()[name]
+    rng = np.random.default_rng(7)
+    p = np.stack([rng.uniform(-4, sc.width + 4, 2000), rng.uniform(-4, sc.height + 4, 2000)], 1).astype(np.float32)
+    d = rng.normal(size=2000).astype(np.float32)
+    d[::7] = 0.0
+    print(f"case eval_positions_backward {name}", flush=True)
+    g = renderer.eval_positions_backward(sc, p, opts, d, BackwardOptions(True))
+    ref = oracle.eval_positions_backward(SceneDesc(sc), p, d, translation=True)
+    total, per = grad_report(g, ref)
+    print(f"{name}: positions backward rel L2 {total:.3g}; per field {per}", flush=True)
+    assert total <= 1e-3

@@ -100,3 +100,19 @@ def test_empty_scene_is_zero(oracle):
     assert np.array_equal(oracle.render_sdf(SceneDesc(sc), OPTS), np.zeros(40 * 32, np.float32))
     assert np.array_equal(oracle.eval_positions(SceneDesc(sc), OPTS, np.zeros((3, 2), np.float32)),
                           np.zeros(3, np.float32))
+
+
+def test_positions_backward_circle_closed_form(oracle):
+    """sdf = |p - c| - r for a filled circle: d/dr = -1 and d/dc = -(p - c)/|p - c|
+    at every point, inside or outside (render_backward_positions)."""
+    sc = _scene([Shape(Circle((20.0, 16.0), 7.0))], [_fill([0])])
+    rng = np.random.default_rng(6)
+    p = _pts(rng, 300)
+    d = rng.normal(size=300).astype(np.float32)
+    g = oracle.eval_positions_backward(SceneDesc(sc), p, d)
+    v = p - np.array([20.0, 16.0], np.float32)
+    n = v / np.linalg.norm(v, axis=1, keepdims=True)
+    want_c = -(d[:, None] * n).sum(0)
+    dp = g["d_shape_params"]
+    assert np.allclose(dp[0:2], want_c, rtol=1e-4, atol=1e-4)
+    assert np.isclose(dp[2], -d.sum(), rtol=1e-4, atol=1e-4)
