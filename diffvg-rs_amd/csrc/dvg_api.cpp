@@ -721,6 +721,22 @@ int32_t dvg_device_eval_positions_backward(dvg_ctx *ctx, dvg_device_scene *ds, c
     return DVG_OK;
 }
 
+// The d_sdf image term of render_backward on a device scene (W*H device
+// adjoints): zeroes the device gradient arrays, then accumulates.
+int32_t dvg_device_render_backward_sdf(dvg_ctx *ctx, dvg_device_scene *ds, const float *d_sdf,
+                                       const dvg_grads *grads) {
+    if (!ctx || !ds || !d_sdf || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
+        return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    dvgk::DevGrads G;
+    grads_to_dev(grads, G);
+    dvgk::launch_zero(&G.d_shape_params, ds->grad_len, 14, ctx->stream);
+    if (ds->S.num_groups > 0) dvgk::launch_sdf_image_backward(ds->S, d_sdf, G, ctx->stream);
+    HIP_TRY(ctx, hipGetLastError());
+    return DVG_OK;
+}
+
 
 int32_t dvg_device_scene_update(dvg_ctx *ctx, dvg_device_scene *ds, const float *path_points,
                                 const float *group_colors) {
@@ -1041,12 +1057,11 @@ int sdf_host(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options
 }
 }  // namespace
 
-int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
-                                    const dvg_backward_options *bopt, const float *positions, uint64_t count,
-                                    const float *d_sdf, const dvg_grads *grads) {
-    if (!ctx || !scene || !options || !grads || (count && (!positions || !d_sdf)))
-        return set_err(ctx, DVG_INVALID_SCENE, "null argument");
-    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+// Host form of the signed-distance gradients: at `count` points, or
+// (positions null) the d_sdf image term with count = W*H adjoints.
+static int sdf_grad_host(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                         const dvg_backward_options *bopt, const float *positions, uint64_t count,
+                         const float *d_sdf, const dvg_grads *grads) {
     dvg_prepared P;
     const char *err = "";
     int st = dvgh::prepare(scene, options, P, &err);
@@ -1068,9 +1083,10 @@ int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, c
     if (st) return st;
     DevBuf gbuf, pb, db;
     hipError_t e = hipMalloc(&gbuf.p, std::max(L.total, (size_t)1) * sizeof(float));
-    if (e == hipSuccess) e = hipMalloc(&pb.p, count * 2 * sizeof(float));
+    if (e == hipSuccess && positions) e = hipMalloc(&pb.p, count * 2 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&db.p, count * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpyAsync(pb.p, positions, count * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && positions)
+        e = hipMemcpyAsync(pb.p, positions, count * 2 * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(db.p, d_sdf, count * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) {
         dvg_device_scene_destroy(ds);
@@ -1081,7 +1097,8 @@ int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, c
     for (int k = 0; k < 14; ++k) *grad_slot(&dg, k) = L.len[k] ? gb + L.off[k] : nullptr;
     if (!(*grad_slot(grads, 12))) dg.d_background_image = nullptr;
     if (!(bopt && bopt->compute_translation && *grad_slot(grads, 13))) dg.d_translation = nullptr;
-    st = dvg_device_eval_positions_backward(ctx, ds, (const float *)pb.p, count, (const float *)db.p, &dg);
+    st = positions ? dvg_device_eval_positions_backward(ctx, ds, (const float *)pb.p, count, (const float *)db.p, &dg)
+                   : dvg_device_render_backward_sdf(ctx, ds, (const float *)db.p, &dg);
     if (st == DVG_OK) {
         std::vector<float> host(std::max(L.total, (size_t)1));
         e = hipMemcpyAsync(host.data(), gbuf.p, L.total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
@@ -1095,6 +1112,24 @@ int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, c
     }
     dvg_device_scene_destroy(ds);
     return st;
+}
+
+int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                                    const dvg_backward_options *bopt, const float *positions, uint64_t count,
+                                    const float *d_sdf, const dvg_grads *grads) {
+    if (!ctx || !scene || !options || !grads || (count && (!positions || !d_sdf)))
+        return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    if (count > 0xffffffffull) return set_err(ctx, DVG_INVALID_SCENE, "too many positions for 1d launch");
+    return sdf_grad_host(ctx, scene, options, bopt, positions, count, d_sdf, grads);
+}
+
+int32_t dvg_render_backward_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options,
+                                const dvg_backward_options *bopt, const float *d_sdf, const dvg_grads *grads) {
+    if (!ctx || !scene || !options || !d_sdf || !grads) return set_err(ctx, DVG_INVALID_SCENE, "null argument");
+    uint64_t npix = (uint64_t)scene->width * scene->height;
+    if (npix * std::max(options->samples_x, 1u) * std::max(options->samples_y, 1u) > 0xffffffffull)
+        return set_err(ctx, DVG_INVALID_SCENE, "too many samples for 1d launch");
+    return sdf_grad_host(ctx, scene, options, bopt, nullptr, npix, d_sdf, grads);
 }
 
 int32_t dvg_render_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene, const dvg_render_options *options, float *out) {

@@ -96,6 +96,9 @@ void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *o
 // reference CPU's render_backward_positions), accumulated into G.
 void launch_sdf_points_backward(const DevScene &S, const float *pos, uint32_t n, const float *d_sdf,
                                 const DevGrads &G, hipStream_t stream);
+// The d_sdf image term of render_backward (W*H adjoints, one per pixel, at
+// every sample), accumulated into G.
+void launch_sdf_image_backward(const DevScene &S, const float *d_sdf, const DevGrads &G, hipStream_t stream);
 // Bit-set slots of the colour adjoint for tile lists longer than the LDS bit
 // set (LDS_MASK_GROUPS entries): `slots` workgroup slots of
 // nw = (2 gcap + 31) / 32 words x BLOCK lanes at gmask, and one queue counter.

@@ -3217,18 +3217,11 @@ __global__ __launch_bounds__(256) void k_sdf_points(DevScene S, const float *__r
     out[i] = signed_distance(S, pos[2u * i], pos[2u * i + 1u]);
 }
 
-// render_backward_positions (backward/render.rs:213-280) over sample_distance
-// (backward/sampling.rs:14-68), one point a lane: the nearest boundary over
-// the groups from last to first (the CPU distance query), d_compute_distance
-// of it with the sign of the group's CPU fill test; translation into the
-// pixel holding the point (coordinates truncated toward zero).
-__global__ __launch_bounds__(256) void k_sdf_points_backward(DevScene S, const float *__restrict__ pos, uint32_t n,
-                                                             const float *__restrict__ d_sdf, DevGrads G) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    float d_dist = d_sdf[i];
-    if (d_dist == 0.f) return;
-    float px = pos[2u * i], py = pos[2u * i + 1u];
+// sample_distance (backward/sampling.rs:14-68) for the adjoint d_dist of the
+// signed distance at (px, py): the nearest boundary over the groups from last
+// to first (the CPU distance query), d_compute_distance of it with the sign of
+// the group's CPU fill test; -d(pt) into translation pixel pix (< 0: none).
+DVG_DEV void sdf_point_grad(const DevScene &S, const DevGrads &G, float px, float py, float d_dist, long pix) {
     float best = __builtin_inff();
     uint32_t best_g = 0;
     bool found = false;
@@ -3247,14 +3240,39 @@ __global__ __launch_bounds__(256) void k_sdf_points_backward(DevScene S, const f
     bool inside = (uint32_t)gd[2] != DVG_PAINT_NONE && group_inside(S, best_g, px, py);
     float dtx = 0.f, dty = 0.f;
     d_distance_query(S, G, best_g, bh, px, py, inside ? -d_dist : d_dist, dtx, dty);
-    if (G.d_translation) {
-        float tx = truncf(px), ty = truncf(py);
-        if (tx >= 0.f && ty >= 0.f && tx < (float)S.W && ty < (float)S.H) {
-            size_t pix = (size_t)ty * S.W + (size_t)tx;
-            unsafeAtomicAdd(G.d_translation + pix * 2u, dtx);
-            unsafeAtomicAdd(G.d_translation + pix * 2u + 1u, dty);
-        }
+    if (G.d_translation && pix >= 0) {
+        unsafeAtomicAdd(G.d_translation + (size_t)pix * 2u, dtx);
+        unsafeAtomicAdd(G.d_translation + (size_t)pix * 2u + 1u, dty);
     }
+}
+
+// render_backward_positions (backward/render.rs:213-280), one point a lane;
+// translation into the pixel holding the point (coordinates truncated toward
+// zero, as `as i32`).
+__global__ __launch_bounds__(256) void k_sdf_points_backward(DevScene S, const float *__restrict__ pos, uint32_t n,
+                                                             const float *__restrict__ d_sdf, DevGrads G) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float d_dist = d_sdf[i];
+    if (d_dist == 0.f) return;
+    float px = pos[2u * i], py = pos[2u * i + 1u];
+    float tx = truncf(px), ty = truncf(py);
+    long pix = (tx >= 0.f && ty >= 0.f && tx < (float)S.W && ty < (float)S.H) ? (long)ty * (long)S.W + (long)tx : -1;
+    sdf_point_grad(S, G, px, py, d_dist, pix);
+}
+
+// The d_sdf image term of render_backward (backward/render.rs:171-186), one
+// sample a lane: sample_distance at the CPU sample position with the pixel's
+// d_sdf; translation into that pixel.
+__global__ __launch_bounds__(256) void k_sdf_image_backward(DevScene S, const float *__restrict__ d_sdf, DevGrads G) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= S.W * S.H * S.spp) return;
+    uint32_t sx = i % S.nsx, sy = (i / S.nsx) % S.nsy, p = i / S.spp;
+    float d_dist = d_sdf[p];
+    if (d_dist == 0.f) return;
+    float px, py;
+    sample_point<true>(S, p % S.W, p / S.W, sx, sy, px, py);
+    sdf_point_grad(S, G, px, py, d_dist, (long)p);
 }
 
 // Winding terms of every segment (the record described with WIND_STRIDE).
@@ -3622,6 +3640,12 @@ void launch_sdf(const DevScene &S, float *out, hipStream_t stream) {
 void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_sdf_points, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, pos, n, out);
+}
+
+void launch_sdf_image_backward(const DevScene &S, const float *d_sdf, const DevGrads &G, hipStream_t stream) {
+    uint64_t ns = (uint64_t)S.W * S.H * S.spp;
+    if (ns == 0) return;
+    hipLaunchKernelGGL(k_sdf_image_backward, dim3(cdiv(ns, 256)), dim3(256), 0, stream, S, d_sdf, G);
 }
 
 void launch_sdf_points_backward(const DevScene &S, const float *pos, uint32_t n, const float *d_sdf,

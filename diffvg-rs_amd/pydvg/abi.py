@@ -138,6 +138,9 @@ SIGNATURES = {
                                                 C.POINTER(BackwardOptions), _f32p, C.c_uint64, _f32p,
                                                 C.POINTER(GradsC)]),
     "dvg_device_eval_positions_backward": (C.c_int32, [_vp, _vp, _vp, C.c_uint64, _vp, C.POINTER(GradsC)]),
+    "dvg_render_backward_sdf": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions),
+                                            C.POINTER(BackwardOptions), _f32p, C.POINTER(GradsC)]),
+    "dvg_device_render_backward_sdf": (C.c_int32, [_vp, _vp, _vp, C.POINTER(GradsC)]),
     "dvg_device_eval_positions": (C.c_int32, [_vp, _vp, _vp, C.c_uint64, _vp]),
     "dvg_render_backward": (C.c_int32, [_vp, C.POINTER(SceneDescC), C.POINTER(RenderOptions),
                                         C.POINTER(BackwardOptions), _f32p, C.POINTER(GradsC)]),

@@ -119,6 +119,20 @@ class Renderer:
         _check(self.lib, self.ctx, st)
         return g
 
+    def render_backward_sdf(self, scene: Scene, options: RenderOptions, d_sdf: np.ndarray,
+                            backward_options: BackwardOptions = BackwardOptions()) -> FlatGrads:
+        """The d_sdf term of render_backward (W * H adjoints of the SDF image)."""
+        desc = SceneDesc(scene)
+        d_sdf = np.ascontiguousarray(d_sdf, np.float32).reshape(-1)
+        if d_sdf.size != scene.width * scene.height:
+            raise RenderError("InvalidScene", "d_sdf_image size mismatch")
+        g = FlatGrads(desc, backward_options.compute_translation)
+        st = self.lib.dvg_render_backward_sdf(self.ctx, desc.ref(), C.byref(options.to_c()),
+                                              C.byref(backward_options.to_c()),
+                                              d_sdf.ctypes.data_as(C.POINTER(C.c_float)), g.ref())
+        _check(self.lib, self.ctx, st)
+        return g
+
     def render_backward(self, scene: Scene, options: RenderOptions, d_render: np.ndarray,
                         backward_options: BackwardOptions = BackwardOptions()) -> FlatGrads:
         desc = SceneDesc(scene)

@@ -367,6 +367,17 @@ int32_t dvg_eval_positions_backward(dvg_ctx *ctx, const dvg_scene_desc *scene,
                                     const float *positions, uint64_t count,
                                     const float *d_sdf, const dvg_grads *grads);
 
+/* The d_sdf term of Renderer::render_backward (renderer.rs:548 ->
+ * backward_gpu.rs:512-521, which adds the CPU's render_backward with d_sdf
+ * only: backward/render.rs:171-186): for a W*H adjoint of the signed-distance
+ * image, sample_distance at every sample of every pixel with that pixel's
+ * d_sdf, into the flat gradient layout (zeroed first).  A caller with both
+ * adjoints adds these gradients to dvg_render_backward's. */
+int32_t dvg_render_backward_sdf(dvg_ctx *ctx, const dvg_scene_desc *scene,
+                                const dvg_render_options *options,
+                                const dvg_backward_options *backward_options,
+                                const float *d_sdf, const dvg_grads *grads);
+
 /* Renderer::render_backward (renderer.rs:548 -> backward_gpu.rs:21-523) with
  * d_render present and d_sdf absent. */
 int32_t dvg_render_backward(dvg_ctx *ctx, const dvg_scene_desc *scene,
@@ -423,6 +434,9 @@ int32_t dvg_device_eval_positions(dvg_ctx *ctx, dvg_device_scene *scene, const f
 int32_t dvg_device_eval_positions_backward(dvg_ctx *ctx, dvg_device_scene *scene,
                                            const float *positions_dev, uint64_t count,
                                            const float *d_sdf_dev, const dvg_grads *grads_dev);
+/* dvg_render_backward_sdf on a device scene (W*H device adjoints). */
+int32_t dvg_device_render_backward_sdf(dvg_ctx *ctx, dvg_device_scene *scene, const float *d_sdf_dev,
+                                       const dvg_grads *grads_dev);
 /* Backward from a device d_render (W*H*4) into device gradient arrays (raw
  * accumulations, as dvg_render_backward_packed); asynchronous on the stream. */
 int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,

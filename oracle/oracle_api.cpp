@@ -118,6 +118,32 @@ int oracle_eval_positions_backward(const dvg_scene_desc *desc, const dvg_backwar
     return DVG_OK;
 }
 
+// The d_sdf image term of render_backward alone (d_render absent): what
+// render_backward with d_sdf adds to the d_render gradients.
+int oracle_render_backward_sdf(const dvg_scene_desc *desc, const dvg_render_options *opt,
+                               const dvg_backward_options *bopt, const float *d_sdf, const dvg_grads *out) {
+    Scene sc = read_scene(desc);
+    Grads gr;
+    bool tr = bopt && bopt->compute_translation;
+    grads_zero(sc, tr, gr);
+    cpu::backward_sdf_image(sc, *opt, tr, d_sdf, gr);
+    fill_out(gr.d_shape_params, out->d_shape_params);
+    fill_out(gr.d_shape_points, out->d_shape_points);
+    fill_out(gr.d_shape_thickness, out->d_shape_thickness);
+    fill_out(gr.d_shape_stroke_width, out->d_shape_stroke_width);
+    fill_out(gr.d_shape_transform, out->d_shape_transform);
+    fill_out(gr.d_group_transform, out->d_group_transform);
+    fill_out(gr.d_group_data, out->d_group_data);
+    fill_out(gr.d_gradient_data, out->d_gradient_data);
+    fill_out(gr.d_stop_offsets, out->d_stop_offsets);
+    fill_out(gr.d_stop_colors, out->d_stop_colors);
+    if (out->d_filter_radius) out->d_filter_radius[0] = gr.d_filter_radius[0];
+    if (out->d_background) std::memcpy(out->d_background, gr.d_background, 4 * sizeof(float));
+    fill_out(gr.d_background_image, out->d_background_image);
+    fill_out(gr.d_translation, out->d_translation);
+    return DVG_OK;
+}
+
 // Reference CPU render_backward into the flat gradient layout.
 int oracle_render_backward(const dvg_scene_desc *desc, const dvg_render_options *opt,
                            const dvg_backward_options *bopt, const float *d_render, const dvg_grads *out) {

@@ -62,6 +62,9 @@ void finalize_background_gradients(const Scene &sc, Grads &gr);
 // in oracle_prefilter.cpp.
 void backward_positions(const Scene &sc, bool compute_translation, const float *pos, size_t n, const float *d_sdf,
                         Grads &gr);
+// The d_sdf image term of render_backward (backward/render.rs:171-186).
+void backward_sdf_image(const Scene &sc, const dvg_render_options &opt, bool compute_translation, const float *d_sdf,
+                        Grads &gr);
 
 // Prefiltered colour of one sample with its adjoint (sampling.rs:227-482);
 // gr == nullptr skips the adjoint.  Defined in oracle_prefilter.cpp.

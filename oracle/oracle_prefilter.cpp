@@ -609,10 +609,34 @@ V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 
     return V4(fr, fg, fb, fa);
 }
 
-// render_backward_positions (backward/render.rs:213-280) over
-// sample_distance (backward/sampling.rs:14-68): for each point with a
-// nonzero d_sdf, the nearest boundary over the groups from last to first,
-// and d_compute_distance of it with the sign of the group's fill test.
+// sample_distance (backward/sampling.rs:14-68) for the adjoint d_dist of the
+// signed distance at pt: the nearest boundary over the groups from last to
+// first, d_compute_distance of it with the sign of the group's fill test,
+// -d(pt) into translation pixel tidx (-1: none).  The reference's `weight`
+// only scales a value it never reads.
+static void sample_distance_grad(const Ctx &cx, V2 pt, float d_dist, long tidx, Grads &gr) {
+    const Scene &sc = *cx.sc;
+    float best = INFINITY;
+    long best_g = -1;
+    Hit bh;
+    for (size_t gi = sc.groups.size(); gi-- > 0;) {
+        Hit h;
+        if (compute_distance(cx, (uint32_t)gi, pt, best, h) && h.distance < best) {
+            best = h.distance;
+            best_g = (long)gi;
+            bh = h;
+        }
+    }
+    if (best_g < 0) return;
+    bool inside = sc.groups[best_g].fill.kind != DVG_PAINT_NONE && is_inside_group(cx, (uint32_t)best_g, pt);
+    float d_abs = inside ? -d_dist : d_dist;
+    V2 d_pt = d_compute_distance(cx, (uint32_t)best_g, bh.shape, pt, bh.point, bh, d_abs, gr);
+    accumulate_translation(gr, tidx, d_pt * -1.0f);
+}
+
+// render_backward_positions (backward/render.rs:213-280): each point with a
+// nonzero d_sdf; translation into the pixel holding it (`as i32` truncates
+// toward zero).
 void backward_positions(const Scene &sc, bool compute_translation, const float *pos, size_t n, const float *d_sdf,
                         Grads &gr) {
     Ctx cx;
@@ -622,26 +646,38 @@ void backward_positions(const Scene &sc, bool compute_translation, const float *
         if (d_dist == 0.0f) continue;
         V2 pt(pos[2 * i], pos[2 * i + 1]);
         long tidx = -1;
-        if (compute_translation) {  // translation_index_for_point: `as i32` truncates toward zero
+        if (compute_translation) {
             long x = (long)std::trunc(pt.x), y = (long)std::trunc(pt.y);
             if (x >= 0 && y >= 0 && x < (long)sc.width && y < (long)sc.height) tidx = y * (long)sc.width + x;
         }
-        float best = INFINITY;
-        long best_g = -1;
-        Hit bh;
-        for (size_t gi = sc.groups.size(); gi-- > 0;) {
-            Hit h;
-            if (compute_distance(cx, (uint32_t)gi, pt, best, h) && h.distance < best) {
-                best = h.distance;
-                best_g = (long)gi;
-                bh = h;
-            }
+        sample_distance_grad(cx, pt, d_dist, tidx, gr);
+    }
+}
+
+// The d_sdf image term of render_backward (backward/render.rs:171-186): at
+// every sample of every pixel (the CPU sample positions), sample_distance
+// with the pixel's d_sdf; translation into that pixel.
+void backward_sdf_image(const Scene &sc, const dvg_render_options &opt, bool compute_translation, const float *d_sdf,
+                        Grads &gr) {
+    Ctx cx;
+    make_ctx(sc, cx);
+    uint32_t nsx = std::max(opt.samples_x, 1u), nsy = std::max(opt.samples_y, 1u);
+    size_t W = sc.width, H = sc.height;
+    size_t total = W * H * nsx * nsy;
+    bool jitter = opt.jitter != 0 && !opt.use_prefiltering;
+    for (size_t idx = 0; idx < total; ++idx) {
+        size_t sx = idx % nsx, sy = (idx / nsx) % nsy, x = (idx / (nsx * nsy)) % W, y = idx / ((size_t)nsx * nsy * W);
+        size_t pix = y * W + x;
+        float d_dist = d_sdf[pix];
+        if (d_dist == 0.0f) continue;  // adds exact zeros
+        float rx = 0.5f, ry = 0.5f;
+        if (jitter) {
+            Pcg32 rng(idx, opt.seed);
+            rx = rng.next_f32();
+            ry = rng.next_f32();
         }
-        if (best_g < 0) continue;
-        bool inside = sc.groups[best_g].fill.kind != DVG_PAINT_NONE && is_inside_group(cx, (uint32_t)best_g, pt);
-        float d_abs = inside ? -d_dist : d_dist;
-        V2 d_pt = d_compute_distance(cx, (uint32_t)best_g, bh.shape, pt, bh.point, bh, d_abs, gr);
-        accumulate_translation(gr, tidx, d_pt * -1.0f);
+        V2 pt((float)x + ((float)sx + rx) / (float)nsx, (float)y + ((float)sy + ry) / (float)nsy);
+        sample_distance_grad(cx, pt, d_dist, compute_translation ? (long)pix : -1, gr);
     }
 }
 

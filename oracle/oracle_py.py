@@ -39,6 +39,9 @@ def lib():
         L.oracle_eval_positions_backward.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.BackwardOptions),
                                                      f32p, C.c_uint64, f32p, C.POINTER(abi.GradsC)]
         L.oracle_eval_positions_backward.restype = C.c_int
+        L.oracle_render_backward_sdf.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions),
+                                                 C.POINTER(abi.BackwardOptions), f32p, C.POINTER(abi.GradsC)]
+        L.oracle_render_backward_sdf.restype = C.c_int
         L.oracle_eval_positions.argtypes = [C.POINTER(abi.SceneDescC), C.POINTER(abi.RenderOptions), f32p,
                                             C.c_uint64, f32p]
         L.oracle_eval_positions.restype = C.c_int
@@ -117,6 +120,19 @@ def eval_positions_backward(desc, positions, d_sdf, translation=False):
                                               d_sdf.size, _fp(d_sdf), g.ref())
     if st != 0:
         raise RuntimeError(f"oracle_eval_positions_backward status {st}")
+    return g
+
+
+def render_backward_sdf(desc, opts, d_sdf, translation=False):
+    """The d_sdf image term of render_backward (the reference CPU's; d_render
+    absent) as FlatGrads."""
+    from pydvg.scene import BackwardOptions, FlatGrads
+    g = FlatGrads(desc, translation)
+    d_sdf = np.ascontiguousarray(d_sdf, np.float32).reshape(-1)
+    st = lib().oracle_render_backward_sdf(desc.ref(), C.byref(opts.to_c()),
+                                          C.byref(BackwardOptions(translation).to_c()), _fp(d_sdf), g.ref())
+    if st != 0:
+        raise RuntimeError(f"oracle_render_backward_sdf status {st}")
     return g
 
 

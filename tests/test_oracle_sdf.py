@@ -116,3 +116,19 @@ def test_positions_backward_circle_closed_form(oracle):
     dp = g["d_shape_params"]
     assert np.allclose(dp[0:2], want_c, rtol=1e-4, atol=1e-4)
     assert np.isclose(dp[2], -d.sum(), rtol=1e-4, atol=1e-4)
+
+
+def test_sdf_image_backward_is_points_backward_at_pixel_centres(oracle):
+    """One unjittered sample a pixel: the d_sdf image term of render_backward
+    equals eval_positions_backward at the pixel centres (with translation)."""
+    sc = _scene([Shape(Circle((20.0, 16.0), 7.0)), Shape(Rect((2.0, 3.0), (9.0, 30.0)))], [_fill([0]), _fill([1])])
+    desc = SceneDesc(sc)
+    rng = np.random.default_rng(9)
+    d = rng.normal(size=sc.width * sc.height).astype(np.float32)
+    yy, xx = np.mgrid[0:sc.height, 0:sc.width]
+    p = np.stack([xx.reshape(-1) + 0.5, yy.reshape(-1) + 0.5], 1).astype(np.float32)
+    a = oracle.render_backward_sdf(desc, OPTS, d, translation=True)
+    b = oracle.eval_positions_backward(desc, p, d, translation=True)
+    for k in ("d_shape_params", "d_shape_transform", "d_group_transform", "d_translation"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.abs(a["d_shape_params"]).max() > 0
