@@ -974,6 +974,205 @@ DVG_DEV void ellipse_closest(float cx, float cy, float rx_, float ry_, float px,
     oy = cy + sy * ry * s;
 }
 
+// ---- the reference CPU's closest point on a segment ----------------------------
+// distance/curve.rs:10-264, the query of the backward's distance adjoint and of
+// the signed-distance gradients.  Points, distances and t are f32 as on the CPU;
+// the quadratic's cubic and the cubic's quintic are formed and solved in f64
+// (curve.rs:48-61, :145-242) with cubic_roots_f64, the CPU's solve_cubic.  The
+// GPU forward's f32 solves (quad_closest / cubic_closest above) place t up to
+// ~1e-3 away from these on ill-conditioned segments far from the point.
+
+// curve.rs:10-21
+DVG_DEV void seg_closest_cpu(float px, float py, float ax, float ay, float bx, float by, float &ox, float &oy,
+                             float &dist, float &t) {
+    float abx = bx - ax, aby = by - ay;
+    float denom = abx * abx + aby * aby;
+    if (denom <= 1.0e-10f) {
+        ox = ax; oy = ay; t = 0.f;
+        dist = dist_pt(px, py, ax, ay);
+        return;
+    }
+    t = clamp01(((px - ax) * abx + (py - ay) * aby) / denom);
+    ox = ax + abx * t;
+    oy = ay + aby * t;
+    dist = dist_pt(px, py, ox, oy);
+}
+
+// curve.rs:266-296 (the CPU's own approximation: t stays 0 on a degenerate
+// denominator, unlike the GPU's curves.rs:792-857)
+DVG_DEV void quad_approx_cpu(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                             float &ox, float &oy, float &ot) {
+    float b0x = x0 - px, b0y = y0 - py, b1x = x1 - px, b1y = y1 - py, b2x = x2 - px, b2y = y2 - py;
+    float a = det2(b0x, b0y, b2x, b2y);
+    float b = 2.0f * det2(b1x, b1y, b0x, b0y);
+    float d = 2.0f * det2(b2x, b2y, b1x, b1y);
+    float f = b * d - a * a;
+    float d21x = b2x - b1x, d21y = b2y - b1y, d10x = b1x - b0x, d10y = b1y - b0y, d20x = b2x - b0x,
+          d20y = b2y - b0y;
+    float gx = ((d21x * b) + (d10x * d)) + (d20x * a), gy = ((d21y * b) + (d10y * d)) + (d20y * a);
+    gx = gx * 2.0f;
+    gy = gy * 2.0f;
+    float gfx = gy, gfy = -gx;
+    float t = 0.f;
+    float denom = gfx * gfx + gfy * gfy;
+    if (denom > 1.0e-8f) {
+        float s = -f / denom;
+        float ppx = gfx * s, ppy = gfy * s;
+        float d0px = b0x - ppx, d0py = b0y - ppy;
+        float ap = det2(d0px, d0py, d20x, d20y);
+        float bp = 2.0f * det2(d10x, d10y, d0px, d0py);
+        float denom2 = 2.0f * a + b + d;
+        if (absf_(denom2) > 1.0e-8f) t = clamp01((ap + bp) / denom2);
+    }
+    float tt = 1.0f - t;
+    ox = ((x0 * (tt * tt)) + (x1 * (2.0f * tt * t))) + (x2 * (t * t));
+    oy = ((y0 * (tt * tt)) + (y1 * (2.0f * tt * t))) + (y2 * (t * t));
+    ot = t;
+}
+
+// curve.rs:26-78
+DVG_DEV void quad_closest_cpu(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                              bool approx, float &ox, float &oy, float &dist, float &tout) {
+    if (approx) {
+        quad_approx_cpu(px, py, x0, y0, x1, y1, x2, y2, ox, oy, tout);
+        dist = dist_pt(px, py, ox, oy);
+        return;
+    }
+    float bt = 0.f, bx = x0, by = y0;
+    float best = dist_pt(px, py, x0, y0);
+    float de = dist_pt(px, py, x2, y2);
+    if (de < best) { best = de; bx = x2; by = y2; bt = 1.f; }
+    double ax = (double)(x0 - 2.0f * x1 + x2), ay = (double)(y0 - 2.0f * y1 + y2);
+    double bbx = (double)(-x0 + x1), bby = (double)(-y0 + y1);
+    double cx = (double)(x0 - px), cy = (double)(y0 - py);
+    double a = ax * ax + ay * ay;
+    double b = 3.0 * (ax * bbx + ay * bby);
+    double c = 2.0 * (bbx * bbx + bby * bby) + (ax * cx + ay * cy);
+    double d = bbx * cx + bby * cy;
+    double r[3];
+    int n = cubic_roots_f64(a, b, c, d, r);
+    for (int i = 0; i < n; ++i) {
+        double t = r[i];
+        if (t >= 0.0 && t <= 1.0) {
+            float t32 = (float)t, tt = 1.0f - t32;
+            float qx = ((x0 * (tt * tt)) + (x1 * (2.0f * tt * t32))) + (x2 * (t32 * t32));
+            float qy = ((y0 * (tt * tt)) + (y1 * (2.0f * tt * t32))) + (y2 * (t32 * t32));
+            float dd = dist_pt(px, py, qx, qy);
+            if (dd < best) { best = dd; bx = qx; by = qy; bt = t32; }
+        }
+    }
+    ox = bx; oy = by; dist = best; tout = bt;
+}
+
+// curve.rs:80-119: the closest point of eight chords; the point returned is
+// the one on the chord
+DVG_DEV void cubic_approx_cpu(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                              float x3, float y3, float &ox, float &oy, float &dist, float &tout) {
+    const float inv = 1.0f / 8.0f;
+    float prev_x = x0, prev_y = y0, prev_t = 0.f;
+    float bx = x0, by = y0, bt = 0.f, best = dist_pt(px, py, x0, y0);
+    for (int i = 1; i <= 8; ++i) {
+        float t = (float)i * inv;
+        float tt = 1.0f - t, tt2 = tt * tt, t2 = t * t;
+        float a = tt2 * tt, b = 3.0f * tt2 * t, c = 3.0f * tt * t2, d = t2 * t;
+        float cx = a * x0 + b * x1 + c * x2 + d * x3;
+        float cy = a * y0 + b * y1 + c * y2 + d * y3;
+        float sx, sy, sd, st;
+        seg_closest_cpu(px, py, prev_x, prev_y, cx, cy, sx, sy, sd, st);
+        float lt = prev_t + st * (t - prev_t);
+        if (sd < best) { best = sd; bx = sx; by = sy; bt = lt; }
+        prev_x = cx; prev_y = cy; prev_t = t;
+    }
+    ox = bx; oy = by; dist = best; tout = bt;
+}
+
+// curve.rs:124-264
+DVG_DEV void cubic_closest_cpu(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                               float x3, float y3, bool approx, float &ox, float &oy, float &dist, float &tout) {
+    if (approx) {
+        cubic_approx_cpu(px, py, x0, y0, x1, y1, x2, y2, x3, y3, ox, oy, dist, tout);
+        return;
+    }
+    float bt = 0.f, bx = x0, by = y0;
+    float best = dist_pt(px, py, x0, y0);
+    float de = dist_pt(px, py, x3, y3);
+    if (de < best) { best = de; bx = x3; by = y3; bt = 1.f; }
+    ox = bx; oy = by; dist = best; tout = bt;
+    double ax = (double)(-x0 + 3.0f * x1 - 3.0f * x2 + x3);
+    double ay = (double)(-y0 + 3.0f * y1 - 3.0f * y2 + y3);
+    double bbx = (double)(3.0f * x0 - 6.0f * x1 + 3.0f * x2);
+    double bby = (double)(3.0f * y0 - 6.0f * y1 + 3.0f * y2);
+    double cx = (double)(-3.0f * x0 + 3.0f * x1);
+    double cy = (double)(-3.0f * y0 + 3.0f * y1);
+    double dx = (double)(x0 - px), dy = (double)(y0 - py);
+    double a = 3.0 * (ax * ax + ay * ay);
+    if (fabs(a) < 1.0e-10) return;
+    double b = 5.0 * (ax * bbx + ay * bby);
+    double c = 4.0 * (ax * cx + ay * cy) + 2.0 * (bbx * bbx + bby * bby);
+    double d = 3.0 * ((bbx * cx + bby * cy) + (ax * dx + ay * dy));
+    double e = (cx * cx + cy * cy) + 2.0 * (dx * bbx + dy * bby);
+    double f = dx * cx + dy * cy;
+    b = b / a; c = c / a; d = d / a; e = e / a; f = f / a;
+    double p1a = (2.0 / 5.0) * c - (4.0 / 25.0) * b * b;
+    double p1b = (3.0 / 5.0) * d - (3.0 / 25.0) * b * c;
+    double p1c = (4.0 / 5.0) * e - (2.0 / 25.0) * b * d;
+    double p1d = f - b * e / 25.0;
+    double q_root = -b / 5.0;
+    double pr[3];
+    int num_p = cubic_roots_f64(p1a, p1b, p1c, p1d, pr);
+    double iv[4];
+    int ni = 0;
+    if (q_root >= 0.0 && q_root <= 1.0) iv[ni++] = q_root;
+    for (int i = 0; i < num_p; ++i) iv[ni++] = pr[i];
+    for (int j = 1; j < ni; ++j) {
+        int k = j;
+        while (k > 0 && iv[k - 1] > iv[k]) {
+            double u = iv[k - 1];
+            iv[k - 1] = iv[k];
+            iv[k] = u;
+            --k;
+        }
+    }
+    double lower = 0.0;
+    for (int j = 0; j <= ni; ++j) {
+        if (j < ni && iv[j] < 0.0) continue;
+        double upper = (j < ni) ? fmin(iv[j], 1.0) : 1.0;
+        double lb = lower, ub = upper;
+        double lbe = ((((lb + b) * lb + c) * lb + d) * lb + e) * lb + f;
+        double ube = ((((ub + b) * ub + c) * ub + d) * ub + e) * ub + f;
+        if (lbe * ube > 0.0) {
+            lower = upper;
+            continue;
+        }
+        if (lbe > ube) {
+            double u = lb;
+            lb = ub;
+            ub = u;
+        }
+        double t = 0.5 * (lb + ub);
+        for (int it = 0; it < 20; ++it) {
+            if (!(t >= lb && t <= ub)) t = 0.5 * (lb + ub);
+            double value = ((((t + b) * t + c) * t + d) * t + e) * t + f;
+            if (fabs(value) < 1.0e-5 || it == 19) break;
+            if (value > 0.0) ub = t; else lb = t;
+            double der = (((5.0 * t + 4.0 * b) * t + 3.0 * c) * t + 2.0 * d) * t + e;
+            t -= value / der;
+        }
+        if (t >= 0.0 && t <= 1.0) {
+            float t32 = (float)t, tt = 1.0f - t32;
+            float qx = (((x0 * (tt * tt * tt)) + (x1 * (3.0f * tt * tt * t32))) + (x2 * (3.0f * tt * t32 * t32))) +
+                       (x3 * (t32 * t32 * t32));
+            float qy = (((y0 * (tt * tt * tt)) + (y1 * (3.0f * tt * tt * t32))) + (y2 * (3.0f * tt * t32 * t32))) +
+                       (y3 * (t32 * t32 * t32));
+            float dd = dist_pt(px, py, qx, qy);
+            if (dd < best) { best = dd; bx = qx; by = qy; bt = t32; }
+        }
+        if (upper >= 1.0) break;
+        lower = upper;
+    }
+    ox = bx; oy = by; dist = best; tout = bt;
+}
+
 // ---- PCG32, src/renderer/rng.rs:16-43 (native 64-bit integer arithmetic) ----
 struct Pcg32 {
     uint64_t state, inc;

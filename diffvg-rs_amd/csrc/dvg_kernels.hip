@@ -1491,7 +1491,7 @@ DVG_DEV bool closest_shape(const DevScene &S, uint32_t prep, float px, float py,
     }
     uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
     bool approx = sd[14] > 0.5f;
-    float best = BIG, bt = 0.f;
+    float best = BIG, bt = 0.f, bx = 0.f, by = 0.f;
     uint32_t bs = 0;
     bool any = false;
     for (uint32_t s = 0; s < cnt; ++s) {
@@ -1502,35 +1502,25 @@ DVG_DEV bool closest_shape(const DevScene &S, uint32_t prep, float px, float py,
         float by0 = fminf(fminf(y0, y1), fminf(y2, y3)), by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
         float lim = any ? best : cut;
         if (box_dist(bx0, by0, bx1, by1, px, py) > lim) continue;
-        float d, t;
-        if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
-        else if (k == 1) quad_closest(px, py, x0, y0, x1, y1, x2, y2, approx, d, t);
-        else cubic_closest(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, d, t);
+        float d, t, qx, qy;
+        if (k == 0) seg_closest_cpu(px, py, x0, y0, x1, y1, qx, qy, d, t);
+        else if (k == 1) quad_closest_cpu(px, py, x0, y0, x1, y1, x2, y2, approx, qx, qy, d, t);
+        else cubic_closest_cpu(px, py, x0, y0, x1, y1, x2, y2, x3, y3, approx, qx, qy, d, t);
         if (!any || d < best) {
             best = d;
             bt = t;
             bs = s;
+            bx = qx;
+            by = qy;
             any = true;
         }
     }
     if (!any) return false;
-    const float *cv = S.curve_data + (off + bs) * 13u;
-    uint32_t k = (uint32_t)cv[0];
-    float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
-    float t = bt, u = 1.0f - t;
-    if (k == 0) {
-        ox = x0 + (x1 - x0) * t;
-        oy = y0 + (y1 - y0) * t;
-    } else if (k == 1) {
-        ox = ((x0 * (u * u)) + (x1 * (2.0f * u * t))) + (x2 * (t * t));
-        oy = ((y0 * (u * u)) + (y1 * (2.0f * u * t))) + (y2 * (t * t));
-    } else {
-        ox = (((x0 * (u * u * u)) + (x1 * (3.0f * u * u * t))) + (x2 * (3.0f * u * t * t))) + (x3 * (t * t * t));
-        oy = (((y0 * (u * u * u)) + (y1 * (3.0f * u * u * t))) + (y2 * (3.0f * u * t * t))) + (y3 * (t * t * t));
-    }
+    ox = bx;
+    oy = by;
     is_path = true;
     seg = bs;
-    tt = t;
+    tt = bt;
     return true;
 }
 

@@ -1,7 +1,17 @@
-"""Comparison helpers for the GPU-vs-oracle parity tests."""
+"""Comparison helpers for the GPU-vs-oracle parity tests.
+
+Gradient parity is judged per field: every non-empty gradient array must be
+within the relative L2 bound on its own, so that a field with a small norm
+cannot hide behind a large one (d_translation holds W*H*2 entries and would
+dominate a norm over the concatenated vector).  A field whose reference norm
+is about zero is compared against a floor of FLOOR times the norm of the
+whole reference vector instead of its own norm.
+"""
 import numpy as np
 
 from pydvg.scene import GRAD_FIELDS
+
+FLOOR = 1e-6
 
 
 def image_report(img, ref, tol=1e-5):
@@ -13,15 +23,29 @@ def image_report(img, ref, tol=1e-5):
 
 
 def grad_report(g, gref, fields=GRAD_FIELDS):
-    """Relative L2 error of the whole gradient vector and per field."""
+    """Relative L2 error of the whole gradient vector and of each field.
+
+    Each field's error is |a - b| / max(|b|, FLOOR * |b_total|); empty fields
+    are skipped."""
+    fields = [k for k in fields if np.asarray(gref[k]).size > 0 or np.asarray(g[k]).size > 0]
+    a = np.concatenate([np.asarray(g[k], np.float64).ravel() for k in fields]) if fields else np.zeros(0)
+    b = np.concatenate([np.asarray(gref[k], np.float64).ravel() for k in fields]) if fields else np.zeros(0)
+    nb_total = float(np.linalg.norm(b))
     out = {}
     for k in fields:
-        a, b = g[k], gref[k]
-        if b.size == 0:
-            continue
-        nb = float(np.linalg.norm(b))
-        out[k] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
-    a = np.concatenate([g[k] for k in fields])
-    b = np.concatenate([gref[k] for k in fields])
-    total = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+        ak, bk = np.asarray(g[k], np.float64).ravel(), np.asarray(gref[k], np.float64).ravel()
+        assert ak.shape == bk.shape, (k, ak.shape, bk.shape)
+        den = max(float(np.linalg.norm(bk)), FLOOR * nb_total, 1e-30)
+        out[k] = float(np.linalg.norm(ak - bk) / den)
+    total = float(np.linalg.norm(a - b) / max(nb_total, 1e-30))
     return total, out
+
+
+def assert_grads(g, gref, tol, fields=GRAD_FIELDS, label=""):
+    """Assert the whole vector and every field within `tol` relative L2."""
+    total, per = grad_report(g, gref, fields)
+    bad = {k: v for k, v in per.items() if not v <= tol}
+    print(f"{label} gradient rel L2 {total:.3g}; per field "
+          + ", ".join(f"{k} {v:.3g}" for k, v in per.items()), flush=True)
+    assert total <= tol and not bad, f"{label}: fields beyond {tol}: {bad} (total {total:.3g})"
+    return total, per

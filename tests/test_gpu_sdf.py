@@ -89,9 +89,9 @@ def test_empty_inputs(renderer):
 def test_eval_positions_backward_matches_oracle(renderer, oracle, name):
     """eval_positions_backward (the reference runs it on the CPU,
     backward/render.rs:213-280) against the oracle's restatement, with
-    translation gradients; relative L2 over the gradient arrays <= 1e-3, the
+    translation gradients; relative L2 <= 1e-3 for every gradient array, the
     gradient tolerance of the render backward."""
-    from parity_util import grad_report
+    from parity_util import assert_grads
     from pydvg.scene import BackwardOptions
     sc, opts = _cases()[name]
     rng = np.random.default_rng(7)
@@ -101,27 +101,15 @@ def test_eval_positions_backward_matches_oracle(renderer, oracle, name):
     print(f"case eval_positions_backward {name}", flush=True)
     g = renderer.eval_positions_backward(sc, p, opts, d, BackwardOptions(True))
     ref = oracle.eval_positions_backward(SceneDesc(sc), p, d, translation=True)
-    total, per = grad_report(g, ref)
-    print(f"{name}: positions backward rel L2 {total:.3g}; per field {per}", flush=True)
-    assert total <= 1e-3
+    assert_grads(g, ref, 1e-3, label=f"{name}: positions backward")
 
 
-# The device's distance query solves the closest point on a cubic with the
-# GPU's f32 routine (curves.rs), the reference CPU with its f64 one
-# (distance/curve.rs:122-259): away from a curve the parameters differ by
-# ~1e-3, and summed over every sample of the mixed scene the path gradients
-# reach 2.8e-3 (measured), above the bound.
-_CUBIC_F64_GAP = "device cubic closest point is f32 where the reference CPU's is f64 (2.8e-3 measured)"
-
-
-@pytest.mark.parametrize("name", ["cubic48",
-                                  pytest.param("mixed64", marks=pytest.mark.xfail(reason=_CUBIC_F64_GAP, strict=False)),
-                                  "scaled_groups"])
+@pytest.mark.parametrize("name", ["cubic48", "mixed64", "scaled_groups"])
 def test_render_backward_sdf_matches_oracle(renderer, oracle, name):
     """The d_sdf term of render_backward (the reference's CPU
     backward/render.rs:171-186) against the oracle, with translation;
-    relative L2 <= 1e-3."""
-    from parity_util import grad_report
+    relative L2 <= 1e-3 for every gradient array."""
+    from parity_util import assert_grads
     from pydvg.scene import BackwardOptions
     sc, opts = _cases()[name]
     rng = np.random.default_rng(8)
@@ -130,6 +118,4 @@ def test_render_backward_sdf_matches_oracle(renderer, oracle, name):
     print(f"case render_backward_sdf {name}", flush=True)
     g = renderer.render_backward_sdf(sc, opts, d, BackwardOptions(True))
     ref = oracle.render_backward_sdf(SceneDesc(sc), opts, d, translation=True)
-    total, per = grad_report(g, ref)
-    print(f"{name}: d_sdf backward rel L2 {total:.3g}; per field {per}", flush=True)
-    assert total <= 1e-3
+    assert_grads(g, ref, 1e-3, label=f"{name}: d_sdf backward")
