@@ -98,6 +98,9 @@ dvg_ctx *dvg_ctx_create(int device_ordinal) {
         return nullptr;
     }
     ctx->stream = ctx->own_stream;
+    // the no-allocation entry point's scratch address on this device; a
+    // failure is reported by that entry point
+    (void)dvgn::noalloc_setup(device_ordinal);
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && ncu > 0)
         ctx->n_cu = (uint32_t)ncu;

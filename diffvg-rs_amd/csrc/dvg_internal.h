@@ -98,6 +98,12 @@ inline int hip_err(dvg_ctx *ctx, hipError_t e, const char *where) {
     return DVG_DEVICE_ERROR;
 }
 
+namespace dvgn {
+// Looks up the no-allocation entry point's device scratch address on
+// `device` (which must be current); dvg_noalloc.hip.
+hipError_t noalloc_setup(int device);
+}  // namespace dvgn
+
 #define HIP_TRY(ctx, expr)                                  \
     do {                                                    \
         hipError_t _e = (expr);                             \

@@ -12,6 +12,7 @@ _f32p = C.POINTER(C.c_float)
 DVG_OK = 0
 DVG_INVALID_SCENE = 1
 DVG_DEVICE_ERROR = 2
+DVG_NOALLOC_MAX_PIXELS = 1024 * 1024
 
 SHAPE_CIRCLE, SHAPE_RECT, SHAPE_PATH, SHAPE_ELLIPSE = 0, 1, 2, 3
 PAINT_NONE, PAINT_SOLID, PAINT_LINEAR, PAINT_RADIAL = 0, 1, 2, 3
@@ -159,6 +160,7 @@ SIGNATURES = {
     "dvg_device_scene_update": (C.c_int32, [_vp, _vp, _vp, _vp]),
     "dvg_device_scene_check": (C.c_int32, [_vp, _vp]),
     "dvg_device_render": (C.c_int32, [_vp, _vp, _vp]),
+    "dvg_device_render_noalloc": (C.c_int32, [_vp, _vp, _vp]),
     "dvg_device_render_backward": (C.c_int32, [_vp, _vp, _vp, C.POINTER(GradsC)]),
     "dvg_sync": (C.c_int32, [_vp]),
     "dvg_ctx_set_timing": (C.c_int32, [_vp, C.c_int]),

@@ -214,6 +214,13 @@ class DeviceScene:
         st = self.r.lib.dvg_device_render(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))
         _check(self.r.lib, self.r.ctx, st)
 
+    def render_noalloc(self, out) -> None:
+        """render() through dvg_device_render_noalloc: no device allocation;
+        the work images live in the library's static per-device scratch
+        (scenes of at most DVG_NOALLOC_MAX_PIXELS pixels)."""
+        st = self.r.lib.dvg_device_render_noalloc(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))
+        _check(self.r.lib, self.r.ctx, st)
+
     def render_sdf(self, out) -> None:
         """W * H signed distances into a device tensor (asynchronous)."""
         st = self.r.lib.dvg_device_render_sdf(self.r.ctx, self.handle, C.c_void_p(out.data_ptr()))

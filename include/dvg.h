@@ -444,6 +444,19 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *scene,
 /* Block until the context's stream is idle. */
 int32_t dvg_sync(dvg_ctx *ctx);
 
+/* dvg_device_render without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as dvg_device_render; its weight and accumulation images live in a
+ * static device array of DVG_NOALLOC_MAX_PIXELS * 5 floats (20 MiB), held by
+ * each device the library runs on until the process ends.  Scenes of more
+ * than DVG_NOALLOC_MAX_PIXELS pixels are rejected with DVG_INVALID_SCENE.  All
+ * calls on one device share that device's array: calls that overlap on one
+ * device (several streams, contexts or threads) race.  Asynchronous on the
+ * context's stream, like dvg_device_render. */
+#define DVG_NOALLOC_MAX_PIXELS (1024u * 1024u)
+int32_t dvg_device_render_noalloc(dvg_ctx *ctx, dvg_device_scene *scene, float *out_rgba_dev);
+
 /* Device-side re-packing for optimisation loops (the per-call prepare_scene
  * of renderer.rs:39 and build_boundary_sampling_data of
  * backward/boundary.rs:46, restated on the GPU for the inputs that change):
