@@ -101,6 +101,11 @@ dvg_ctx *dvg_ctx_create(int device_ordinal) {
     // the no-allocation entry point's scratch address on this device; a
     // failure is reported by that entry point
     (void)dvgn::noalloc_setup(device_ordinal);
+    // the checked boundary pass is the default (its redo stream and events)
+    if (dvg_ctx_set_exact_boundary(ctx, 1) != DVG_OK) {
+        dvg_ctx_destroy(ctx);
+        return nullptr;
+    }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device_ordinal) == hipSuccess && ncu > 0)
         ctx->n_cu = (uint32_t)ncu;
@@ -855,15 +860,27 @@ int32_t dvg_device_render_backward(dvg_ctx *ctx, dvg_device_scene *ds, const flo
             HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
             HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
             dvgk::launch_boundary_redo(S, ctx->weight, d_render, G, bw, ctx->side);
-            HIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
             redo = true;
+            // from here the side stream holds work that writes G and bwork:
+            // every return below joins it first
+            hipError_t e = hipEventRecord(ctx->ev_join, ctx->side);
+            if (e != hipSuccess) {
+                (void)hipStreamSynchronize(ctx->side);
+                return hip_err(ctx, e, "hipEventRecord(ev_join)");
+            }
         }
     }
     if (exact) {
         KernelTimer kt(ctx, DVG_KERNEL_INTERIOR);
         dvgk::launch_backward_interior(S, ctx->weight, d_render, G, iw, ctx->stream);
     }
-    if (redo) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    if (redo) {
+        hipError_t e = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(ctx->side);
+            return hip_err(ctx, e, "hipStreamWaitEvent(ev_join)");
+        }
+    }
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }

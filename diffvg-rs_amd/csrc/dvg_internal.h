@@ -45,7 +45,7 @@ struct dvg_ctx {
     // beside the colour adjoint; fork / join events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool exact_boundary = false;  // dvg_ctx_set_exact_boundary
+    bool exact_boundary = false;  // dvg_ctx_set_exact_boundary (set on by dvg_ctx_create)
 };
 
 struct dvg_device_scene {

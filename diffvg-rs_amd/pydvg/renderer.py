@@ -154,8 +154,9 @@ class Renderer:
         _check(self.lib, self.ctx, self.lib.dvg_sync(self.ctx))
 
     def set_exact_boundary(self, enable: bool):
-        """Boundary samples whose f32 inside test is uncertain are evaluated
-        again in f64 (single-scene backward passes only)."""
+        """The checked boundary pass (the default): boundary samples whose f32
+        inside test is uncertain are evaluated again in f64.  False selects
+        the unchecked f32 pass."""
         _check(self.lib, self.ctx, self.lib.dvg_ctx_set_exact_boundary(self.ctx, int(enable)))
 
     def set_timing(self, enable: bool):

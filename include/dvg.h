@@ -324,14 +324,18 @@ const char *dvg_last_error(const dvg_ctx *ctx);
  * as void*; NULL is the device's null stream).  Until called, the context
  * uses a non-blocking stream of its own. */
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *hip_stream);
-/* Exact boundary mode (no counterpart in the reference; off by default).
- * When enabled, the single-scene backward passes (dvg_render_backward, the
- * packed and device-scene forms; not the batch) flag every boundary sample
- * whose f32 inside test the f64 winding of distance/winding.rs could
- * disagree with, and evaluate those samples again in f64 on a second stream
- * of the context beside the colour adjoint.  The boundary gradients then
- * follow the reference CPU sample for sample; the pass costs more (about
- * 15 % of a step on 1024^2 with 1000 cubic paths). */
+/* Checked boundary pass (no counterpart in the reference; on by default).
+ * The single-scene backward passes (dvg_render_backward, the packed and
+ * device-scene forms, and the batch) flag every boundary sample whose f32
+ * inside test the f64 winding of distance/winding.rs could disagree with,
+ * and evaluate those samples again in f64 (on a second stream of the
+ * context, beside the colour adjoint, for the single-scene forms).  The
+ * boundary gradients then follow the reference CPU sample for sample.
+ * dvg_ctx_set_exact_boundary(ctx, 0) selects the unchecked f32 pass: about
+ * 15 % faster on 1024^2 with 1000 cubic paths, but a sample whose query point
+ * lies within f32 rounding of an edge can take the other side, and each such
+ * sample carries a 1 / pdf weight (whole-image gradients off by up to ~2e-3
+ * relative on adjoints that weight every edge). */
 int32_t dvg_ctx_set_exact_boundary(dvg_ctx *ctx, int enable);
 
 /* ---- scene API ------------------------------------------------------------ */

@@ -26,8 +26,8 @@ void grads_zero(const Scene &sc, bool with_translation, Grads &gr) {
     gr.d_stop_colors.assign((size_t)sc.num_stops * 4, 0.f);
     gr.d_background_image.assign(sc.has_bg_image ? sc.bg_image.size() : 0, 0.f);
     gr.d_translation.assign(with_translation ? (size_t)sc.width * sc.height * 2 : 0, 0.f);
-    gr.d_filter_radius[0] = 0.f;
-    for (int i = 0; i < 4; ++i) gr.d_background[i] = 0.f;
+    gr.d_filter_radius[0] = 0.0;
+    for (int i = 0; i < 4; ++i) gr.d_background[i] = 0.0;
 }
 }  // namespace oracle
 
@@ -85,8 +85,9 @@ int oracle_render_backward_rows(const dvg_scene_desc *desc, const dvg_render_opt
     fill_out(gr.d_gradient_data, out->d_gradient_data);
     fill_out(gr.d_stop_offsets, out->d_stop_offsets);
     fill_out(gr.d_stop_colors, out->d_stop_colors);
-    if (out->d_filter_radius) out->d_filter_radius[0] = gr.d_filter_radius[0];
-    if (out->d_background) std::memcpy(out->d_background, gr.d_background, 4 * sizeof(float));
+    if (out->d_filter_radius) out->d_filter_radius[0] = (float)gr.d_filter_radius[0];
+    if (out->d_background)
+        for (int i = 0; i < 4; ++i) out->d_background[i] = (float)gr.d_background[i];
     fill_out(gr.d_background_image, out->d_background_image);
     fill_out(gr.d_translation, out->d_translation);
     return DVG_OK;
@@ -111,8 +112,9 @@ int oracle_eval_positions_backward(const dvg_scene_desc *desc, const dvg_backwar
     fill_out(gr.d_gradient_data, out->d_gradient_data);
     fill_out(gr.d_stop_offsets, out->d_stop_offsets);
     fill_out(gr.d_stop_colors, out->d_stop_colors);
-    if (out->d_filter_radius) out->d_filter_radius[0] = gr.d_filter_radius[0];
-    if (out->d_background) std::memcpy(out->d_background, gr.d_background, 4 * sizeof(float));
+    if (out->d_filter_radius) out->d_filter_radius[0] = (float)gr.d_filter_radius[0];
+    if (out->d_background)
+        for (int i = 0; i < 4; ++i) out->d_background[i] = (float)gr.d_background[i];
     fill_out(gr.d_background_image, out->d_background_image);
     fill_out(gr.d_translation, out->d_translation);
     return DVG_OK;
@@ -137,8 +139,9 @@ int oracle_render_backward_sdf(const dvg_scene_desc *desc, const dvg_render_opti
     fill_out(gr.d_gradient_data, out->d_gradient_data);
     fill_out(gr.d_stop_offsets, out->d_stop_offsets);
     fill_out(gr.d_stop_colors, out->d_stop_colors);
-    if (out->d_filter_radius) out->d_filter_radius[0] = gr.d_filter_radius[0];
-    if (out->d_background) std::memcpy(out->d_background, gr.d_background, 4 * sizeof(float));
+    if (out->d_filter_radius) out->d_filter_radius[0] = (float)gr.d_filter_radius[0];
+    if (out->d_background)
+        for (int i = 0; i < 4; ++i) out->d_background[i] = (float)gr.d_background[i];
     fill_out(gr.d_background_image, out->d_background_image);
     fill_out(gr.d_translation, out->d_translation);
     return DVG_OK;

@@ -842,7 +842,8 @@ void accumulate_background_grad(const Scene &sc, Grads &gr, size_t pix, V4 d) { 
         }
         return;
     }
-    gr.d_background[0] += d.x; gr.d_background[1] += d.y; gr.d_background[2] += d.z; gr.d_background[3] += d.w;
+    gr.d_background[0] += (double)d.x; gr.d_background[1] += (double)d.y;
+    gr.d_background[2] += (double)d.z; gr.d_background[3] += (double)d.w;
 }
 
 void finalize_background_gradients(const Scene &sc, Grads &gr) {  // background.rs:29-61
@@ -855,9 +856,10 @@ void finalize_background_gradients(const Scene &sc, Grads &gr) {  // background.
         }
     } else {
         float r = sc.background.x, g = sc.background.y, b = sc.background.z, a = sc.background.w;
-        float *d = gr.d_background;
+        float d[4] = {(float)gr.d_background[0], (float)gr.d_background[1], (float)gr.d_background[2],
+                      (float)gr.d_background[3]};
         float nr = d[0] * a, ng = d[1] * a, nb = d[2] * a, na = d[0] * r + d[1] * g + d[2] * b + d[3];
-        d[0] = nr; d[1] = ng; d[2] = nb; d[3] = na;
+        gr.d_background[0] = nr; gr.d_background[1] = ng; gr.d_background[2] = nb; gr.d_background[3] = na;
     }
 }
 
@@ -1599,7 +1601,9 @@ void backward_samples(const Ctx &cx, const dvg_render_options &opt, bool compute
         long tidx = compute_translation ? (long)pix : -1;
         V4 color = opt.use_prefiltering ? sample_color_prefiltered(cx, npt, &bg, &dc, &gr, tidx, pix)
                                         : sample_color(cx, npt, &bg, &dc, nullptr, &gr, tidx, pix);
-        accumulate_filter_gradient(sc, color, d_render, wimg, pt, gr.d_filter_radius[0]);
+        float d_radius = 0.f;
+        accumulate_filter_gradient(sc, color, d_render, wimg, pt, d_radius);
+        gr.d_filter_radius[0] += (double)d_radius;
     }
 }
 

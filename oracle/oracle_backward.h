@@ -11,8 +11,14 @@ struct Grads {
     std::vector<float> d_shape_params, d_shape_points, d_shape_thickness, d_shape_stroke_width, d_shape_transform,
         d_group_transform, d_group_data, d_gradient_data, d_stop_offsets, d_stop_colors, d_background_image,
         d_translation;
-    float d_filter_radius[1] = {0.f};
-    float d_background[4] = {0.f, 0.f, 0.f, 0.f};
+    // Sums over every sample of the image.  The reference adds them up in f32
+    // in sample order (4.2M terms at 1024^2 with 4 samples a pixel, with
+    // cancellation between signs of the adjoint), so its rounding reaches
+    // ~1e-2 of the result; the oracle keeps them in f64 and rounds once, the
+    // value the reference's sum approximates (each sample's own terms are
+    // formed in f32 as in the reference).
+    double d_filter_radius[1] = {0.0};
+    double d_background[4] = {0.0, 0.0, 0.0, 0.0};
 };
 
 struct Ctx {

@@ -128,32 +128,32 @@ def test_fullsize_render_repeats(renderer, name):
     assert np.allclose(a[:, 3], 1.0, atol=1e-5)
 
 
-# Exact boundary mode (dvg_ctx_set_exact_boundary): boundary samples whose f32
-# inside tests are uncertain are evaluated again with the reference CPU's f64
-# windings, so the band gradients follow the oracle more closely than the
-# default pass's 1e-3.  Measured: 2.7e-7 (1024^2, 1000 cubic paths; the
-# default pass 2.0e-4), 9.4e-7 (256^2, 4x4 samples).
+# The default boundary pass checks its f32 inside tests: samples whose test
+# the reference CPU's f64 winding could decide otherwise are evaluated again
+# in f64, so the band gradients follow the oracle more closely than the
+# unchecked f32 pass's 1e-3.  Measured: 2.7e-7 (1024^2, 1000 cubic paths; the
+# unchecked pass 2.0e-4), 9.4e-7 (256^2, 4x4 samples).
 EXACT_GRAD_REL = 1e-5
 
 
 @pytest.mark.parametrize("name", ["paths1024", "paths256_spp16", "mixed512"])
-def test_fullsize_exact_boundary_band(renderer, oracle, name):
+def test_fullsize_checked_boundary_band(renderer, oracle, name):
     sc, opts = _scene(name)
     W, H = sc.width, sc.height
     bwd_band = dict((c[0], c[2]) for c in CASES)[name]
-    print(f"case {name} exact boundary, rows {bwd_band}", flush=True)
+    print(f"case {name} checked boundary pass, rows {bwd_band}", flush=True)
     img = renderer.render(sc, opts)
     tgt = renderer.render(scenes.jittered(sc, 2.0, seed=9), opts)
     d_render = _band_adjoint(img, tgt, W, H, bwd_band)
-    g_fast = renderer.render_backward(sc, opts, d_render)
-    renderer.set_exact_boundary(True)
+    renderer.set_exact_boundary(False)
     try:
-        g = renderer.render_backward(sc, opts, d_render)
+        g_fast = renderer.render_backward(sc, opts, d_render)
     finally:
-        renderer.set_exact_boundary(False)
+        renderer.set_exact_boundary(True)
+    g = renderer.render_backward(sc, opts, d_render)
     gref = oracle.render_backward_rows(SceneDesc(sc), opts, d_render, *bwd_band)
     fast, _ = grad_report(g_fast, gref)
-    total, per = assert_grads(g, gref, GRAD_REL, label=f"{name} exact boundary rows {bwd_band} (default pass {fast:.3g})")
+    total, per = assert_grads(g, gref, GRAD_REL, label=f"{name} checked pass rows {bwd_band} (unchecked {fast:.3g})")
     assert total <= EXACT_GRAD_REL
     # the geometry gradients come from the boundary samples alone
     for k in ("d_shape_points", "d_shape_transform", "d_shape_stroke_width", "d_shape_params"):

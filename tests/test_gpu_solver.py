@@ -26,10 +26,10 @@ def test_fast_winding_agrees_with_libm_formula(renderer, seed):
 
 
 # dvg_selftest_inside counts, per family of 2^22 pairs, (a) the pairs where
-# the checked inside test of the exact boundary mode differs from the f64
-# formula without having flagged itself uncertain (that mode evaluates flagged
+# the checked inside test of the default boundary pass differs from the f64
+# formula without having flagged itself uncertain (the pass evaluates flagged
 # samples again in f64), (b) the flagged pairs, (c) the pairs where the
-# default boundary pass's unchecked test differs.  Measured: (a) 0, 2, 0, 1-3
+# unchecked f32 pass's test differs.  Measured: (a) 0, 2, 0, 1-3
 # of 4.19M (family 1: near-quadratic cubics, family 3: points within 0.01 px
 # of the curve), bound 4e-6 of a family; (b) a small share of a family except
 # next to the curve (1.6 %) and next to a y-extremum (0.15 %).
