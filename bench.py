@@ -238,6 +238,8 @@ def bench_batch(args, r, stream, dev):
     from pydvg.renderer import DeviceScene, PreparedScene
 
     items = scenes.config4(seed=0)
+    # the batch runs the unchecked boundary pass: the plain loop runs it too
+    r.set_exact_boundary(False)
     n_samples = sum(sc.width * sc.height * o.samples_x * o.samples_y for sc, o in items)
     log(f"bench: c4 {len(items)} scenes, {n_samples} samples per step")
     targets, singles = [], []
@@ -285,7 +287,7 @@ def bench_batch(args, r, stream, dev):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": graph_s * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": WORKLOADS["c4"][1], "global_batch": len(items), "samples_per_step": n_samples,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu", "boundary_pass": "unchecked f32 (batch and plain loop)"},
         "plain_loop_ms_per_step": plain_s * 1e3, "batched_eager_ms_per_step": eager_s * 1e3,
         "speedup_vs_plain_loop": plain_s / graph_s, "autotuned": cfg, "autotune_seconds": tune_s,
         "roofline": None, "cpu_baseline": None,

@@ -395,3 +395,25 @@ class FlatGrads:
 
     def vector(self, fields: Sequence[str] = GRAD_FIELDS) -> np.ndarray:
         return np.concatenate([self.arrays[k] for k in fields])
+
+
+def finalize_background_gradients(scene: Scene, grads) -> None:
+    """finalize_background_gradients (src/backward/background.rs:29-61), in
+    place on flat gradients: the premultiplied background adjoint that the
+    packed seam, the device scenes and the batch return becomes the adjoint
+    of the straight colour (rgb times alpha; alpha from every channel).  The
+    host entry points (dvg_render_backward) apply it themselves."""
+    if scene.background_image is not None:
+        d = np.asarray(grads["d_background_image"]).reshape(-1, 4)
+        if d.size == 0:
+            return
+        im = np.asarray(scene.background_image, np.float32).reshape(-1, 4)
+        r, g, b, a = im[:, 0], im[:, 1], im[:, 2], im[:, 3]
+        da = d[:, 0] * r + d[:, 1] * g + d[:, 2] * b + d[:, 3]
+        d[:, 0], d[:, 1], d[:, 2] = d[:, 0] * a, d[:, 1] * a, d[:, 2] * a
+        d[:, 3] = da
+    else:
+        d = grads["d_background"]
+        r, g, b, a = (np.float32(v) for v in scene.background)
+        d0, d1, d2, d3 = (np.float32(v) for v in d[:4])
+        d[0], d[1], d[2], d[3] = d0 * a, d1 * a, d2 * a, d0 * r + d1 * g + d2 * b + d3

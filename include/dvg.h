@@ -326,7 +326,7 @@ const char *dvg_last_error(const dvg_ctx *ctx);
 int32_t dvg_ctx_set_stream(dvg_ctx *ctx, void *hip_stream);
 /* Checked boundary pass (no counterpart in the reference; on by default).
  * The single-scene backward passes (dvg_render_backward, the packed and
- * device-scene forms, and the batch) flag every boundary sample whose f32
+ * device-scene forms; not the batch) flag every boundary sample whose f32
  * inside test the f64 winding of distance/winding.rs could disagree with,
  * and evaluate those samples again in f64 (on a second stream of the
  * context, beside the colour adjoint, for the single-scene forms).  The
@@ -530,7 +530,9 @@ typedef struct dvg_batch_config {
 
 /* Uploads n scenes (packs as for dvg_device_scene_create) and allocates their
  * work buffers.  targets: n device pointers to W*H*4 target images (or NULL
- * when the loss is never asked for); they must outlive the batch. */
+ * when the loss is never asked for); they must outlive the batch.  The batch
+ * runs the unchecked f32 boundary pass (see dvg_ctx_set_exact_boundary),
+ * whatever the context's mode. */
 int32_t dvg_batch_create(dvg_ctx *ctx, uint32_t n, const dvg_scene_pack *packs, const dvg_backward_pack *bpacks,
                          const dvg_render_params *params, const float *const *targets, dvg_batch **out);
 void dvg_batch_destroy(dvg_batch *batch);

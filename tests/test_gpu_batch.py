@@ -4,7 +4,8 @@ A batch of scenes of different sizes and modes (non-prefiltered cubic paths,
 mixed shapes with gradients and strokes, prefiltered strokes and polygons) is
 stepped with forward + L2 loss gradient + backward.  Every scene's image and
 gradients must equal those of the one-scene device path (dvg_device_render /
-dvg_device_render_backward) on the same inputs, up to the summation order of
+dvg_device_render_backward, in the batch's unchecked boundary pass) on the
+same inputs, up to the summation order of
 float atomics: images within 1e-6 absolute, gradients within 1e-5 relative L2.
 The same holds for every replay of the captured HIP graph, interleaved with
 eager work, and after the autotuner has changed the launch shapes.
@@ -43,25 +44,30 @@ def setup(renderer):
 
     items = _items()
     targets, refs = [], []
-    for i, (sc, opts) in enumerate(items):
-        print(f"setup scene {i}: {sc.width}x{sc.height}, {len(sc.shapes)} shapes", flush=True)
-        tds = DeviceScene(renderer, PreparedScene(scenes.jittered(sc, 2.0, seed=9), opts))
-        tgt = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device="cuda")
-        tds.render(tgt)
-        renderer.sync()
-        targets.append(tgt)
-        # one-scene path
-        ds = DeviceScene(renderer, PreparedScene(sc, opts))
-        img = torch.empty_like(tgt)
-        ds.render(img)
-        renderer.sync()
-        dr = 2.0 * (img - tgt)
-        bufs, sizes = ds.grad_buffers(torch)
-        # the context's stream does not wait for torch's: finish torch's work
-        torch.cuda.synchronize()
-        ds.render_backward(dr, bufs, sizes)
-        renderer.sync()
-        refs.append((img.cpu().numpy(), {k: bufs[k].cpu().numpy()[:sizes[k]] for k in bufs}))
+    # the one-scene path in the batch's boundary pass (the unchecked one)
+    renderer.set_exact_boundary(False)
+    try:
+        for i, (sc, opts) in enumerate(items):
+            print(f"setup scene {i}: {sc.width}x{sc.height}, {len(sc.shapes)} shapes", flush=True)
+            tds = DeviceScene(renderer, PreparedScene(scenes.jittered(sc, 2.0, seed=9), opts))
+            tgt = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device="cuda")
+            tds.render(tgt)
+            renderer.sync()
+            targets.append(tgt)
+            # one-scene path
+            ds = DeviceScene(renderer, PreparedScene(sc, opts))
+            img = torch.empty_like(tgt)
+            ds.render(img)
+            renderer.sync()
+            dr = 2.0 * (img - tgt)
+            bufs, sizes = ds.grad_buffers(torch)
+            # the context's stream does not wait for torch's: finish torch's work
+            torch.cuda.synchronize()
+            ds.render_backward(dr, bufs, sizes)
+            renderer.sync()
+            refs.append((img.cpu().numpy(), {k: bufs[k].cpu().numpy()[:sizes[k]] for k in bufs}))
+    finally:
+        renderer.set_exact_boundary(True)
     print("setup: batch of the scenes", flush=True)
     batch = Batch(renderer, items, targets)
     return batch, refs
