@@ -151,19 +151,26 @@ def main():
             tgt_adam = torch.tensor(scenes.gaussian_target(W, H, 256, seed=0), device=dev)
         else:
             tgt_adam = tgt
-        loop = DeviceAdamLoop(r, sc, opts, tgt_adam, lr_points=0.5, lr_colors=1e-2)
-        losses = [loop.step(want_loss=False)]
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for i in range(args.adam_steps):
-            losses.append(loop.step(want_loss=False))
-            if (i + 1) % 50 == 0:
-                log(f"bench: Adam step {i + 1}, loss {float(losses[-1]):.1f}")
-        torch.cuda.synchronize()
-        adam_s = (time.perf_counter() - t1) / args.adam_steps
+        # the step captured in a HIP graph (its first two steps are the eager
+        # sizing step and the capture) and, beside it, the same loop eager
+        timed_s = {}
+        for mode in ("eager", "graph"):
+            loop = DeviceAdamLoop(r, sc, opts, tgt_adam, lr_points=0.5, lr_colors=1e-2, graph=mode == "graph")
+            losses = [loop.step(want_loss=False), loop.step(want_loss=False)]
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for i in range(args.adam_steps):
+                losses.append(loop.step(want_loss=False))
+                if (i + 1) % 50 == 0:
+                    log(f"bench: Adam ({mode}) step {i + 1}, loss {float(losses[-1]):.1f}")
+            torch.cuda.synchronize()
+            timed_s[mode] = (time.perf_counter() - t1) / args.adam_steps
+            del loop
+        adam_s = timed_s["graph"]
         free, total = torch.cuda.mem_get_info()
         curve = [float(x) for x in losses]
-        adam = {"steps": args.adam_steps, "steps_per_s": 1.0 / adam_s,
+        adam = {"steps": args.adam_steps, "steps_per_s": 1.0 / adam_s, "hip_graph": True,
+                "eager_steps_per_s": 1.0 / timed_s["eager"],
                 "loss_first": curve[:6], "loss_every_10": curve[::10], "loss_last": curve[-1],
                 "target": "256 anisotropic Gaussians" if args.workload == "c5" else "scene with points jittered N(0, 2 px)",
                 "device_used_bytes": total - free, "torch_peak_bytes": torch.cuda.max_memory_allocated()}

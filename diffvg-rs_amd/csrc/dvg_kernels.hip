@@ -49,32 +49,50 @@ namespace dvgk {
 
 // ---------------------------------------------------------------------------
 // Tile binning: group g belongs to tile t iff t lies in g's clamped tile range
-// (binning.rs:30-110).  One lane per tile scans the groups in ascending order,
-// so the lists come out sorted without the host sort of renderer.rs:364-367.
-__global__ void k_bin_count(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
-                            uint32_t *__restrict__ counts) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tiles_x * tiles_y) return;
+// (binning.rs:30-110).  One wave per tile tests 64 groups at a time in
+// ascending order (coalesced loads of the ranges); a hit's position in the
+// tile's list is the count of hits on lower lanes, so the lists come out
+// sorted by group id without the host sort of renderer.rs:364-367.
+constexpr uint32_t BIN_TILES_PER_WG = BLOCK / 64;
+
+__global__ __launch_bounds__(BLOCK) void k_bin_count(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x,
+                                                     uint32_t tiles_y, uint32_t *__restrict__ counts) {
+    uint32_t t = blockIdx.x * BIN_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    if (t >= tiles_x * tiles_y) return;  // (whole waves)
     int tx = (int)(t % tiles_x), ty = (int)(t / tiles_x);
     uint32_t c = 0;
-    for (uint32_t g = 0; g < G; ++g) {
-        int4 r = rng[g];
-        c += (tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w) ? 1u : 0u;
+    for (uint32_t g0 = 0; g0 < G; g0 += 64u) {
+        uint32_t g = g0 + lane;
+        bool hit = false;
+        if (g < G) {
+            int4 r = rng[g];
+            hit = tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w;
+        }
+        c += (uint32_t)__popcll(__ballot(hit));
     }
-    counts[t] = c;
+    if (lane == 0) counts[t] = c;
 }
 
-__global__ void k_bin_write(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
-                            const uint32_t *__restrict__ offsets, uint32_t *__restrict__ entries) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= tiles_x * tiles_y) return;
+__global__ __launch_bounds__(BLOCK) void k_bin_write(const int4 *__restrict__ rng, uint32_t G, uint32_t tiles_x,
+                                                     uint32_t tiles_y, const uint32_t *__restrict__ offsets,
+                                                     uint32_t *__restrict__ entries) {
+    uint32_t t = blockIdx.x * BIN_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    if (t >= tiles_x * tiles_y) return;  // (whole waves)
     int tx = (int)(t % tiles_x), ty = (int)(t / tiles_x);
     // the tile's slot [offsets[t], offsets[t+1]) bounds the writes (offsets
     // clamped to a buffer capacity cut lists short instead of overrunning)
     uint32_t o = offsets[t], e = offsets[t + 1];
-    for (uint32_t g = 0; g < G && o < e; ++g) {
-        int4 r = rng[g];
-        if (tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w) entries[o++] = g;
+    for (uint32_t g0 = 0; g0 < G && o < e; g0 += 64u) {
+        uint32_t g = g0 + lane;
+        bool hit = false;
+        if (g < G) {
+            int4 r = rng[g];
+            hit = tx >= r.x && tx <= r.z && ty >= r.y && ty <= r.w;
+        }
+        uint64_t m = __ballot(hit);
+        uint32_t pos = o + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (hit && pos < e) entries[pos] = g;
+        o += (uint32_t)__popcll(m);
     }
 }
 
@@ -3477,21 +3495,39 @@ __global__ void k_upd_lengths(DevScene S, float *__restrict__ shape_lengths, flo
     }
 }
 
-// Shape CDF over prepared instances, summed in order by one thread (the
-// host's order, so the tables match a host packing bit for bit).
-__global__ void k_upd_shape_cdf(DevScene S, float *__restrict__ cdf, float *__restrict__ pmf) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    float accum = 0.f;
-    for (uint32_t i = 0; i < S.n_cdf; ++i) {
-        float len = S.shape_lengths[S.shape_ids[i]];
-        accum += len;
-        cdf[i] = accum;
-        pmf[i] = len;
+// Shape CDF over prepared instances (build_shape_cdf, backward/boundary.rs:
+// 312-347): the lengths are gathered into LDS by every lane, a chunk at a
+// time, and summed in order by one lane (the host's order, so the tables
+// match a host packing bit for bit); then every lane normalises.
+constexpr uint32_t CDF_CHUNK = 8192, CDF_THREADS = 1024;
+
+__global__ __launch_bounds__(CDF_THREADS) void k_upd_shape_cdf(DevScene S, float *__restrict__ cdf,
+                                                               float *__restrict__ pmf) {
+    __shared__ float s_len[CDF_CHUNK];
+    __shared__ float s_total;
+    float accum = 0.f;  // (lane 0)
+    for (uint32_t base = 0; base < S.n_cdf; base += CDF_CHUNK) {
+        uint32_t n = min(CDF_CHUNK, S.n_cdf - base);
+        for (uint32_t i = threadIdx.x; i < n; i += CDF_THREADS) {
+            float len = S.shape_lengths[S.shape_ids[base + i]];
+            s_len[i] = len;
+            pmf[base + i] = len;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (uint32_t i = 0; i < n; ++i) {
+                accum += s_len[i];
+                cdf[base + i] = accum;
+            }
+        __syncthreads();  // (s_len is refilled; lane 0's cdf stores land before the last pass)
     }
-    bool ok = accum > 0.f && isfinite(accum);
-    for (uint32_t i = 0; i < S.n_cdf; ++i) {
-        cdf[i] = ok ? cdf[i] / accum : 0.f;
-        pmf[i] = ok ? pmf[i] / accum : 0.f;
+    if (threadIdx.x == 0) s_total = accum;
+    __syncthreads();
+    float total = s_total;
+    bool ok = total > 0.f && isfinite(total);
+    for (uint32_t i = threadIdx.x; i < S.n_cdf; i += CDF_THREADS) {
+        cdf[i] = ok ? cdf[i] / total : 0.f;
+        pmf[i] = ok ? pmf[i] / total : 0.f;
     }
 }
 
@@ -3599,7 +3635,7 @@ void launch_zero(float *const *ptrs, const uint32_t *lens, int count, hipStream_
 void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
                       hipStream_t stream) {
     uint32_t nt = tiles_x * tiles_y;
-    hipLaunchKernelGGL(k_bin_count, dim3(cdiv(nt, 256)), dim3(256), 0, stream,
+    hipLaunchKernelGGL(k_bin_count, dim3(cdiv(nt, BIN_TILES_PER_WG)), dim3(BLOCK), 0, stream,
                        reinterpret_cast<const int4 *>(group_aabb), G, tiles_x, tiles_y, counts);
 }
 void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream) {
@@ -3608,7 +3644,7 @@ void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStrea
 void launch_bin_write(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y,
                       const uint32_t *offsets, uint32_t *entries, hipStream_t stream) {
     uint32_t nt = tiles_x * tiles_y;
-    hipLaunchKernelGGL(k_bin_write, dim3(cdiv(nt, 256)), dim3(256), 0, stream,
+    hipLaunchKernelGGL(k_bin_write, dim3(cdiv(nt, BIN_TILES_PER_WG)), dim3(BLOCK), 0, stream,
                        reinterpret_cast<const int4 *>(group_aabb), G, tiles_x, tiles_y, offsets, entries);
 }
 void launch_weights(const DevScene &S, float *weight, bool cpu, hipStream_t stream) {
@@ -3797,7 +3833,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
                            (float *)(S.shape_lengths), (float *)(S.path_cdf),
                            (float *)(S.path_pmf));
-        hipLaunchKernelGGL(k_upd_shape_cdf, dim3(1), dim3(64), 0, stream, S, (float *)(S.shape_cdf),
+        hipLaunchKernelGGL(k_upd_shape_cdf, dim3(1), dim3(CDF_THREADS), 0, stream, S, (float *)(S.shape_cdf),
                            (float *)(S.shape_pmf));
     }
 }

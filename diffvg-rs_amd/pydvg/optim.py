@@ -92,10 +92,18 @@ class DeviceAdamLoop:
     (dvg_device_scene_update) -> forward -> loss gradient -> backward -> Adam,
     with no host round trip except the optional loss readback.
 
+    graph=True: the first step runs eagerly (it sizes every work buffer of
+    the library), the second captures the whole step once in a HIP graph
+    (torch.cuda.CUDAGraph; the library's kernels join the capture on the
+    capture stream) and every step replays it.  The Adam bias corrections
+    b^t are device scalars, so the graph carries no step count.  The loss,
+    when asked for, is reduced after the replay (a reduction stays out of
+    the graph).
+
     The renderer's stream is set to torch's current stream, so that the
     library's kernels and the torch arithmetic are ordered on one stream."""
 
-    def __init__(self, renderer, scene, options, target, lr_points=0.5, lr_colors=1e-2):
+    def __init__(self, renderer, scene, options, target, lr_points=0.5, lr_colors=1e-2, graph=False):
         import torch
         self.torch = torch
         self.r = renderer
@@ -117,6 +125,9 @@ class DeviceAdamLoop:
         self.lr_p, self.lr_c = lr_points, lr_colors
         self.b1, self.b2, self.eps = 0.9, 0.999, 1e-8
         self.t = 0
+        # b1^t and b2^t, advanced on the device every step
+        self.pw1 = torch.ones((), dtype=torch.float64, device="cuda")
+        self.pw2 = torch.ones((), dtype=torch.float64, device="cuda")
         self.m_p = torch.zeros_like(self.points)
         self.v_p = torch.zeros_like(self.points)
         self.m_c = torch.zeros_like(self.colors)
@@ -125,23 +136,27 @@ class DeviceAdamLoop:
         self.img = torch.empty(scene.width * scene.height * 4, dtype=torch.float32, device="cuda")
         self.d_render = torch.empty_like(self.img)
         self.bufs, self.sizes = self.ds.grad_buffers(torch)
+        self.use_graph = graph
+        self.graph = None
 
     def _adam(self, x, g, m, v, lr):
         m.mul_(self.b1).add_(g, alpha=1 - self.b1)
         v.mul_(self.b2).addcmul_(g, g, value=1 - self.b2)
-        mh = m / (1 - self.b1 ** self.t)
-        vh = v / (1 - self.b2 ** self.t)
+        mh = m / (1.0 - self.pw1).float()
+        vh = v / (1.0 - self.pw2).float()
         x.sub_(lr * mh / (vh.sqrt() + self.eps))
 
-    def step(self, want_loss=True):
+    def _body(self):
+        """One step without the loss: update, forward, 2 (img - target),
+        backward, Adam."""
         torch = self.torch
         self.ds.update(self.points, self.colors)
         self.ds.render(self.img)
         torch.sub(self.img, self.target, out=self.d_render)
-        loss = (self.d_render * self.d_render).sum()
         self.d_render.mul_(2.0)
         self.ds.render_backward(self.d_render, self.bufs, self.sizes)
-        self.t += 1
+        self.pw1.mul_(self.b1)
+        self.pw2.mul_(self.b2)
         n = self.points.numel()
         if n:
             self._adam(self.points, self.bufs["d_shape_points"][:n], self.m_p, self.v_p, self.lr_p)
@@ -150,4 +165,35 @@ class DeviceAdamLoop:
         self._adam(self.colors, g_c, self.m_c, self.v_c, self.lr_c)
         # only solid fill colours move, kept in [0, 1]
         self.colors.copy_(torch.where(self.mask > 0, self.colors.clamp(0.0, 1.0), old))
+
+    def _capture(self):
+        torch = self.torch
+        cur = torch.cuda.current_stream()
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(cur)
+        try:
+            with torch.cuda.graph(g, stream=side):
+                self.r.set_stream(side.cuda_stream)
+                self._body()
+        finally:
+            self.r.set_stream(cur.cuda_stream)
+        cur.wait_stream(side)
+        self.graph = g
+
+    def step(self, want_loss=True):
+        """One step; the loss L = sum((img - target)^2) of the image the step
+        rendered (before its update), as a float or (want_loss=False) a
+        device scalar."""
+        torch = self.torch
+        self.t += 1
+        if self.use_graph and self.t >= 2:
+            if self.graph is None:
+                # the capture records without running: this step's work is
+                # its first replay
+                self._capture()
+            self.graph.replay()
+        else:
+            self._body()
+        loss = (self.d_render * self.d_render).sum() * 0.25
         return float(loss) if want_loss else loss

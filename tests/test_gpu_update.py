@@ -115,3 +115,45 @@ def test_device_adam_follows_oracle_loss_curve(renderer):
     print("device", losses, "oracle", ref, "rel", rel)
     assert rel[0] <= 1e-5
     assert max(rel) <= 1e-3
+
+
+def test_device_adam_graph_replays_match_eager(renderer):
+    """The device Adam loop with its step captured in a HIP graph against an
+    eager step taken from the same state, step after step, with the
+    program's own eager work and synchronisations between the replays.  Two
+    runs differ only in the order of float atomic additions, which Adam's
+    normalised update would amplify over steps, so before every step the
+    eager loop takes the graph loop's state (parameters, moments, b^t).
+    Bounds: image 1e-6, gradients 1e-5 relative (every field 1e-3),
+    parameters 1e-3 px and 1e-5, loss 1e-6 relative."""
+    import torch
+    from pydvg.optim import DeviceAdamLoop
+    sc, opts = scenes.config5_reduced(seed=0)
+    tgt = torch.tensor(scenes.gaussian_target(sc.width, sc.height, 256, seed=0), device="cuda")
+    print("case device adam graph replay", flush=True)
+    graph = DeviceAdamLoop(renderer, copy.deepcopy(sc), opts, tgt, lr_points=0.5, lr_colors=1e-2, graph=True)
+    eager = DeviceAdamLoop(renderer, copy.deepcopy(sc), opts, tgt, lr_points=0.5, lr_colors=1e-2)
+    state = ("points", "colors", "m_p", "v_p", "m_c", "v_c", "pw1", "pw2")
+    for k in range(6):
+        for name in state:
+            getattr(eager, name).copy_(getattr(graph, name))
+        eager.t = graph.t
+        torch.cuda.synchronize()
+        lg = graph.step()
+        le = eager.step()
+        # eager work of the program between the replays
+        x = torch.ones(1 << 16, device="cuda")
+        (x * 3.0).sum().item()
+        torch.cuda.synchronize()
+        di = float((eager.img - graph.img).abs().max())
+        dp = float((eager.points - graph.points).abs().max())
+        dc = float((eager.colors - graph.colors).abs().max())
+        print(f"step {k} ({'replay' if graph.graph is not None else 'eager'}): loss {lg:.7g} / {le:.7g}; "
+              f"max |image diff| {di:.3g}, |points diff| {dp:.3g}, |colours diff| {dc:.3g}", flush=True)
+        ge = {n: eager.bufs[n].cpu().numpy()[:eager.sizes[n]] for n in eager.bufs}
+        gg = {n: graph.bufs[n].cpu().numpy()[:graph.sizes[n]] for n in graph.bufs}
+        total, _ = assert_grads(gg, ge, 1e-3, label=f"step {k} gradients, graph against eager")
+        assert total <= 1e-5
+        assert di <= 1e-6 and dp <= 1e-3 and dc <= 1e-5
+        assert abs(le - lg) <= 1e-6 * abs(le)
+    assert graph.graph is not None
