@@ -7,12 +7,11 @@
 //   prepare_scene_backward      src/renderer/prepare_backward.rs:45-115
 //   build_boundary_sampling_data src/backward/boundary.rs:46-110, 200-451
 //
-// Two deliberate differences from the reference buffers, neither visible to
-// the kernels: no BVHs are emitted (their meta is all zero, which the
-// reference kernels also accept and which means "scan linearly"; libdvg culls
-// with its own tile lists), and no flattened `segment_data` is stored (the
-// reference reads it only for a path with no curve segments, whose flattening
-// is then empty too), so shape_data's segment offset/count are 0.
+//   append_group_bvh / append_path_bvh  src/renderer/bvh.rs:62-226
+//   flatten_with_thickness / annotate_segments  src/geometry.rs:218-476
+//
+// Every buffer of PreparedScene is filled as the reference fills it,
+// including the flattened stroke segments and the group / path BVHs.
 #include "dvg_pack.h"
 
 #include <algorithm>
@@ -119,29 +118,141 @@ static float dist_point_line(V2 p, V2 a, V2 b) {
 }
 static V2 lerp(V2 a, V2 b, float t) { return {a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t}; }
 
-// Max |radius| of the flattened stroke segments (geometry.rs:218-375).
-static void flat_quad(V2 p0, V2 p1, V2 p2, float r0, float r1, float r2, float tol, int depth, float &m) {
+// Flattened stroke segments (geometry.rs:218-375, StrokeSegment of
+// geometry.rs:30-56): recursive halving until flat within the tolerance,
+// radii interpolated, then prev / next directions and cap flags.
+struct StrokeSeg {
+    V2 a, b;
+    float r0, r1;
+    V2 prev_dir, next_dir;
+    bool start_cap, end_cap;
+};
+static void flat_quad(V2 p0, V2 p1, V2 p2, float r0, float r1, float r2, float tol, int depth,
+                      std::vector<StrokeSeg> &out) {
     if (depth >= 16 || dist_point_line(p1, p0, p2) <= tol) {
-        m = std::fmax(std::fmax(m, std::fabs(r0)), std::fabs(r2));
+        out.push_back({p0, p2, r0, r2, {0.f, 0.f}, {0.f, 0.f}, false, false});
         return;
     }
     V2 p01 = lerp(p0, p1, 0.5f), p12 = lerp(p1, p2, 0.5f), p012 = lerp(p01, p12, 0.5f);
     float r01 = 0.5f * (r0 + r1), r12 = 0.5f * (r1 + r2), r012 = 0.5f * (r01 + r12);
-    flat_quad(p0, p01, p012, r0, r01, r012, tol, depth + 1, m);
-    flat_quad(p012, p12, p2, r012, r12, r2, tol, depth + 1, m);
+    flat_quad(p0, p01, p012, r0, r01, r012, tol, depth + 1, out);
+    flat_quad(p012, p12, p2, r012, r12, r2, tol, depth + 1, out);
 }
 static void flat_cubic(V2 p0, V2 p1, V2 p2, V2 p3, float r0, float r1, float r2, float r3, float tol, int depth,
-                       float &m) {
+                       std::vector<StrokeSeg> &out) {
     if (depth >= 16 || std::fmax(dist_point_line(p1, p0, p3), dist_point_line(p2, p0, p3)) <= tol) {
-        m = std::fmax(std::fmax(m, std::fabs(r0)), std::fabs(r3));
+        out.push_back({p0, p3, r0, r3, {0.f, 0.f}, {0.f, 0.f}, false, false});
         return;
     }
     V2 p01 = lerp(p0, p1, 0.5f), p12 = lerp(p1, p2, 0.5f), p23 = lerp(p2, p3, 0.5f);
     V2 p012 = lerp(p01, p12, 0.5f), p123 = lerp(p12, p23, 0.5f), p0123 = lerp(p012, p123, 0.5f);
     float r01 = 0.5f * (r0 + r1), r12 = 0.5f * (r1 + r2), r23 = 0.5f * (r2 + r3);
     float r012 = 0.5f * (r01 + r12), r123 = 0.5f * (r12 + r23), r0123 = 0.5f * (r012 + r123);
-    flat_cubic(p0, p01, p012, p0123, r0, r01, r012, r0123, tol, depth + 1, m);
-    flat_cubic(p0123, p123, p23, p3, r0123, r123, r23, r3, tol, depth + 1, m);
+    flat_cubic(p0, p01, p012, p0123, r0, r01, r012, r0123, tol, depth + 1, out);
+    flat_cubic(p0123, p123, p23, p3, r0123, r123, r23, r3, tol, depth + 1, out);
+}
+// annotate_segments (geometry.rs:432-476)
+static void annotate(std::vector<StrokeSeg> &segs, size_t first, bool closed) {
+    size_t count = segs.size() - first;
+    if (count == 0) return;
+    std::vector<V2> dirs(count);
+    for (size_t i = 0; i < count; ++i) {
+        const StrokeSeg &sg = segs[first + i];
+        V2 v = {sg.b.x - sg.a.x, sg.b.y - sg.a.y};
+        float len = std::sqrt(v.x * v.x + v.y * v.y);
+        dirs[i] = len > 0.f ? V2{v.x / len, v.y / len} : V2{0.f, 0.f};
+    }
+    for (size_t i = 0; i < count; ++i) {
+        StrokeSeg &sg = segs[first + i];
+        sg.start_cap = !closed && i == 0;
+        sg.end_cap = !closed && i + 1 == count;
+        V2 pd = {0.f, 0.f};
+        if (i == 0) {
+            if (closed) pd = {-dirs[count - 1].x, -dirs[count - 1].y};
+        } else {
+            pd = {-dirs[i - 1].x, -dirs[i - 1].y};
+        }
+        V2 nd = {0.f, 0.f};
+        if (i + 1 < count) nd = dirs[i + 1];
+        else if (closed) nd = dirs[0];
+        sg.prev_dir = pd;
+        sg.next_dir = nd;
+    }
+}
+
+// ---- BVHs (renderer/bvh.rs:62-226) ------------------------------------------
+// Median split by centre on the longer axis (a stable sort, as the
+// reference's sort_by), leaves of at most 8 items (constants.rs:23), nodes in
+// preorder with skip pointers for stackless traversal.
+static const uint32_t BVH_LEAF = 8, BVH_NONE = 0xffffffffu;
+struct Box {
+    V2 mn, mx;
+};
+struct BvhNode {
+    Box b;
+    uint32_t left, right, skip, start, count;
+};
+static uint32_t bvh_node(std::vector<BvhNode> &nodes, const std::vector<Box> &bx, std::vector<size_t> &idx,
+                         size_t start, size_t end) {
+    Box nb = {{INFINITY, INFINITY}, {-INFINITY, -INFINITY}};
+    for (size_t i = start; i < end; ++i) {
+        const Box &o = bx[idx[i]];
+        nb.mn = {std::fmin(nb.mn.x, o.mn.x), std::fmin(nb.mn.y, o.mn.y)};
+        nb.mx = {std::fmax(nb.mx.x, o.mx.x), std::fmax(nb.mx.y, o.mx.y)};
+    }
+    size_t count = end - start;
+    uint32_t ni = (uint32_t)nodes.size();
+    nodes.push_back({nb, BVH_NONE, BVH_NONE, BVH_NONE, (uint32_t)start, (uint32_t)count});
+    if (count <= BVH_LEAF) return ni;
+    int axis = (nb.mx.x - nb.mn.x) >= (nb.mx.y - nb.mn.y) ? 0 : 1;
+    auto ctr = [&](size_t k) {
+        const Box &o = bx[k];
+        return axis == 0 ? (o.mn.x + o.mx.x) * 0.5f : (o.mn.y + o.mx.y) * 0.5f;
+    };
+    std::stable_sort(idx.begin() + start, idx.begin() + end, [&](size_t a, size_t b) { return ctr(a) < ctr(b); });
+    size_t mid = start + count / 2;
+    uint32_t l = bvh_node(nodes, bx, idx, start, mid);
+    uint32_t r = bvh_node(nodes, bx, idx, mid, end);
+    nodes[ni].left = l;
+    nodes[ni].right = r;
+    nodes[ni].count = 0;
+    return ni;
+}
+static void bvh_skip(std::vector<BvhNode> &nodes, uint32_t root) {  // assign_bvh_skip
+    if (nodes.empty()) return;
+    std::vector<std::pair<uint32_t, uint32_t>> st = {{root, BVH_NONE}};
+    while (!st.empty()) {
+        auto [id, sk] = st.back();
+        st.pop_back();
+        if (id == BVH_NONE) continue;
+        BvhNode &n = nodes[id];
+        n.skip = sk;
+        if (n.count == 0) {
+            if (n.right != BVH_NONE) st.push_back({n.right, sk});
+            if (n.left != BVH_NONE) st.push_back({n.left, n.right});
+        }
+    }
+}
+// Appends a BVH over `bx` (item ids `ids`) and its meta [node_off, node_cnt,
+// idx_off, idx_cnt] (append_group_bvh / append_path_bvh).
+static void append_bvh(const std::vector<Box> &bx, const std::vector<uint32_t> &ids, std::vector<float> &ob,
+                       std::vector<uint32_t> &on, std::vector<uint32_t> &oi, std::vector<uint32_t> &meta) {
+    if (bx.empty()) {
+        meta.insert(meta.end(), {0u, 0u, 0u, 0u});
+        return;
+    }
+    uint32_t node_off = (uint32_t)(on.size() / 4), idx_off = (uint32_t)oi.size();
+    std::vector<size_t> idx(bx.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::vector<BvhNode> nodes;
+    uint32_t root = bvh_node(nodes, bx, idx, 0, idx.size());
+    bvh_skip(nodes, root);
+    for (size_t i : idx) oi.push_back(ids[i]);
+    for (const BvhNode &n : nodes) {
+        ob.insert(ob.end(), {n.b.mn.x, n.b.mn.y, n.b.mx.x, n.b.mx.y});
+        on.insert(on.end(), {n.left, n.skip, n.start, n.count});
+    }
+    meta.insert(meta.end(), {node_off, (uint32_t)nodes.size(), idx_off, (uint32_t)idx.size()});
 }
 
 static float polyline_length(const Curve &c) {  // backward/boundary.rs:230-309 per segment
@@ -206,6 +317,9 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
         return v;
     };
     std::vector<Curve> curves;
+    std::vector<StrokeSeg> segs;
+    std::vector<Box> path_boxes, group_boxes;
+    std::vector<uint32_t> path_ids, group_ids;
     for (uint32_t g = 0; g < G; ++g) {
         const float *s2c = d->shape_to_canvas + 9 * (size_t)g;
         const float *c2s = d->canvas_to_shape + 9 * (size_t)g;
@@ -217,6 +331,8 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
         uint32_t shape_count = 0;
         bool gb_valid = false;
         float gb[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
+        group_boxes.clear();
+        group_ids.clear();
         P.group_xform.insert(P.group_xform.end(), {c2s[0], c2s[1], c2s[2], c2s[3], c2s[4], c2s[5]});
         P.group_shape_xform.insert(P.group_shape_xform.end(), {s2c[0], s2c[1], s2c[2], s2c[3], s2c[4], s2c[5]});
         P.group_inv_scale.push_back(inv_scale);
@@ -235,6 +351,9 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
             float max_r = std::fabs(sw);
             float params[4] = {0.f, 0.f, 0.f, 0.f};
             bool has_bounds = true, has_thickness = false, approx = false;
+            uint32_t seg_off = 0, seg_cnt = 0;
+            path_boxes.clear();
+            path_ids.clear();
             V2 bmin{0.f, 0.f}, bmax{0.f, 0.f};
             uint32_t curve_off = (uint32_t)(P.curve_data.size() / 13), curve_cnt = 0;
             if (kind == DVG_SHAPE_CIRCLE) {
@@ -256,10 +375,36 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
                 has_thickness = pv.nt > 0;
                 approx = d->path_use_distance_approx[si] != 0;
                 curve_segments(pv, sw, curves);
+                // the flattened stroke segments (prepare.rs:520-530), in
+                // the order flatten_with_thickness visits the path
+                size_t first = segs.size();
                 for (const Curve &c : curves) {
-                    if (c.kind == 0) max_r = std::fmax(std::fmax(max_r, std::fabs(c.r[0])), std::fabs(c.r[1]));
-                    else if (c.kind == 1) flat_quad(c.p[0], c.p[1], c.p[2], c.r[0], c.r[1], c.r[2], tol, 0, max_r);
-                    else flat_cubic(c.p[0], c.p[1], c.p[2], c.p[3], c.r[0], c.r[1], c.r[2], c.r[3], tol, 0, max_r);
+                    if (c.kind == 0) segs.push_back({c.p[0], c.p[1], c.r[0], c.r[1], {0.f, 0.f}, {0.f, 0.f}, false, false});
+                    else if (c.kind == 1) flat_quad(c.p[0], c.p[1], c.p[2], c.r[0], c.r[1], c.r[2], tol, 0, segs);
+                    else flat_cubic(c.p[0], c.p[1], c.p[2], c.p[3], c.r[0], c.r[1], c.r[2], c.r[3], tol, 0, segs);
+                }
+                annotate(segs, first, pv.closed);
+                for (size_t k = first; k < segs.size(); ++k)
+                    max_r = std::fmax(std::fmax(max_r, std::fabs(segs[k].r0)), std::fabs(segs[k].r1));
+                seg_off = (uint32_t)first;
+                seg_cnt = (uint32_t)(segs.size() - first);
+                // path BVH over the curve segments (prepare.rs:241-250)
+                path_boxes.clear();
+                path_ids.clear();
+                for (size_t k = 0; k < curves.size(); ++k) {
+                    const Curve &c = curves[k];
+                    Box b = {{INFINITY, INFINITY}, {-INFINITY, -INFINITY}};
+                    for (int j = 0; j < 4; ++j) {
+                        b.mn = {std::fmin(b.mn.x, c.p[j].x), std::fmin(b.mn.y, c.p[j].y)};
+                        b.mx = {std::fmax(b.mx.x, c.p[j].x), std::fmax(b.mx.y, c.p[j].y)};
+                    }
+                    float pad = std::fmax(std::fmax(std::fmax(std::fabs(c.r[0]), std::fabs(c.r[1])), std::fabs(c.r[2])),
+                                          std::fabs(c.r[3]));
+                    pad = std::fmax(pad, 0.0f);
+                    path_boxes.push_back({{b.mn.x - pad, b.mn.y - pad}, {b.mx.x + pad, b.mx.y + pad}});
+                    path_ids.push_back(curve_off + (uint32_t)k);
+                }
+                for (const Curve &c : curves) {
                     P.curve_data.push_back((float)c.kind);
                     for (int j = 0; j < 4; ++j) {
                         P.curve_data.push_back(c.p[j].x);
@@ -286,7 +431,7 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
             uint32_t prep = (uint32_t)(P.shape_data.size() / 15);
             P.group_shapes.push_back((float)prep);
             P.prepared_to_scene.push_back(si);
-            float sd[15] = {(float)kind, 0.f, 0.f, sw, params[0], params[1], params[2], params[3],
+            float sd[15] = {(float)kind, (float)seg_off, (float)seg_cnt, sw, params[0], params[1], params[2], params[3],
                             has_thickness ? 1.f : 0.f, (float)d->stroke_join[si], (float)d->stroke_cap[si],
                             d->stroke_miter_limit[si], (float)curve_off, (float)curve_cnt, approx ? 1.f : 0.f};
             P.shape_data.insert(P.shape_data.end(), sd, sd + 15);
@@ -303,6 +448,8 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
                 }
                 float b[4] = {mn.x - pad, mn.y - pad, mx.x + pad, mx.y + pad};
                 P.shape_bounds.insert(P.shape_bounds.end(), b, b + 4);
+                group_boxes.push_back({{b[0], b[1]}, {b[2], b[3]}});
+                group_ids.push_back(prep);
                 if (!gb_valid) {
                     std::memcpy(gb, b, sizeof(b));
                     gb_valid = true;
@@ -313,7 +460,8 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
             } else {
                 P.shape_bounds.insert(P.shape_bounds.end(), {1.f, 1.f, 0.f, 0.f});
             }
-            P.path_bvh_meta.insert(P.path_bvh_meta.end(), {0u, 0u, 0u, 0u});
+            append_bvh(path_boxes, path_ids, P.path_bvh_bounds, P.path_bvh_nodes, P.path_bvh_indices,
+                       P.path_bvh_meta);
             ++shape_count;
         }
         if (P.group_shapes.size() > MAX_F32_INT) {
@@ -348,8 +496,20 @@ int prepare(const dvg_scene_desc *d, const dvg_render_options *opt, dvg_prepared
         }
         if (gb_valid) P.group_bounds.insert(P.group_bounds.end(), gb, gb + 4);
         else P.group_bounds.insert(P.group_bounds.end(), {1.f, 1.f, 0.f, 0.f});
-        P.group_bvh_meta.insert(P.group_bvh_meta.end(), {0u, 0u, 0u, 0u});
+        // group BVH over the shapes with bounds (prepare.rs:290-297)
+        append_bvh(group_boxes, group_ids, P.group_bvh_bounds, P.group_bvh_nodes, P.group_bvh_indices,
+                   P.group_bvh_meta);
         P.group_data.insert(P.group_data.end(), gd, gd + 16);
+    }
+    // segments_to_f32 (path.rs:176-193)
+    P.segment_data.reserve(segs.size() * 12);
+    for (const StrokeSeg &sg : segs)
+        P.segment_data.insert(P.segment_data.end(), {sg.a.x, sg.a.y, sg.b.x, sg.b.y, sg.r0, sg.r1, sg.prev_dir.x,
+                                                     sg.prev_dir.y, sg.next_dir.x, sg.next_dir.y,
+                                                     sg.start_cap ? 1.f : 0.f, sg.end_cap ? 1.f : 0.f});
+    if (segs.size() > MAX_F32_INT) {
+        *err = "too many segments for f32 indexing";
+        return DVG_INVALID_SCENE;
     }
     if (P.curve_data.size() / 13 > MAX_F32_INT) {
         *err = "too many curve segments for f32 indexing";
