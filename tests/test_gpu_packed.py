@@ -7,10 +7,11 @@ entry points dvg_render / dvg_render_backward on the same scenes.
 Three packs per scene: the packer's; the packer's with segment_data and the
 BVH buffers replaced by oracle/oracle_pack.py's restatement (the buffers a
 Rust caller computes itself); and one with no BVH at all (all-zero meta, no
-segments), which the reference's kernels also accept.  The forward must be
-identical to dvg_render; the backward's raw accumulations, finalized as
-finalize_background_gradients does, must equal dvg_render_backward up to the
-order of float atomics (1e-5 relative per field).
+segments), which the reference's kernels also accept.  The forward must
+equal dvg_render's up to the order of the splat's float atomics (1e-6); the
+backward's raw accumulations, finalized as finalize_background_gradients
+does, must equal dvg_render_backward up to the order of float atomics (1e-5
+relative per field).
 """
 import ctypes as C
 
@@ -139,7 +140,9 @@ def test_packed_seam_equals_host_entry_points(renderer, case):
         img = np.zeros(sc.width * sc.height * 4, np.float32)
         st = lib.dvg_render_packed(ctx, C.byref(pk), C.byref(p.params), _as(img, C.c_float))
         assert st == abi.DVG_OK, (name, lib.dvg_last_error(ctx))
-        assert np.array_equal(img, np.asarray(img_ref).reshape(-1)), name
+        # the splat's float atomics may add in another order from run to run
+        err = float(np.abs(img - np.asarray(img_ref).reshape(-1)).max())
+        assert err <= 1e-6, (name, err)
         g = FlatGrads(p.desc)
         st = lib.dvg_render_backward_packed(ctx, C.byref(pk), C.byref(p.bpack), C.byref(p.params),
                                             _as(d_render, C.c_float), C.byref(g.c))
