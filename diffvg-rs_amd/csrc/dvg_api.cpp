@@ -122,6 +122,7 @@ void dvg_ctx_destroy(dvg_ctx *ctx) {
     if (ctx->accum) (void)hipFree(ctx->accum);
     if (ctx->bwork) (void)hipFree(ctx->bwork);
     if (ctx->imask) (void)hipFree(ctx->imask);
+    if (ctx->sdfw) (void)hipFree(ctx->sdfw);
     for (const TimedLaunch &tl : ctx->pending) ctx->event_pool.push_back({tl.start, tl.stop});
     for (auto &e : ctx->event_pool) {
         (void)hipEventDestroy(e.first);
@@ -413,6 +414,37 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     if (bp && bp->prepared_to_scene)
         for (uint32_t i = SP; i-- > 0;) s2p[bp->prepared_to_scene[i]] = i;
     std::vector<float> cpu_bounds = group_cpu_bounds(pk, bp);
+    // group BVHs (renderer/bvh.rs), checked so that a skip-pointer walk
+    // always moves forward and stays in its arrays; a null meta is no BVH
+    std::vector<uint32_t> gbvh_meta((size_t)std::max(G, 1u) * 4, 0);
+    if (pk->group_bvh_meta) std::copy(pk->group_bvh_meta, pk->group_bvh_meta + 4 * (size_t)G, gbvh_meta.begin());
+    for (uint32_t g = 0; g < G; ++g) {
+        const uint32_t *m = &gbvh_meta[4 * (size_t)g];
+        if (m[1] == 0 || m[3] == 0) continue;
+        uint64_t nodes_end = (uint64_t)m[0] + m[1], idx_end = (uint64_t)m[2] + m[3];
+        if (!pk->group_bvh_nodes || !pk->group_bvh_bounds || !pk->group_bvh_indices ||
+            nodes_end * 4 > pk->group_bvh_nodes_len || nodes_end * 4 > pk->group_bvh_bounds_len ||
+            idx_end > pk->group_bvh_indices_len)
+            return set_err(ctx, DVG_INVALID_SCENE, "group BVH outside its arrays");
+        for (uint32_t k = 0; k < m[1]; ++k) {
+            const uint32_t *nd = pk->group_bvh_nodes + 4 * ((size_t)m[0] + k);
+            bool skip_ok = nd[1] == 0xffffffffu || (nd[1] > k && nd[1] < m[1]);
+            bool node_ok = nd[3] == 0 ? (nd[0] > k && nd[0] < m[1]) : ((uint64_t)nd[2] + nd[3] <= m[3]);
+            if (!skip_ok || !node_ok) return set_err(ctx, DVG_INVALID_SCENE, "invalid group BVH node");
+        }
+        for (uint32_t k = 0; k < m[3]; ++k)
+            if (pk->group_bvh_indices[(size_t)m[2] + k] >= SP)
+                return set_err(ctx, DVG_INVALID_SCENE, "group BVH index out of range");
+    }
+    // tiled signed distance: no group's shape-to-canvas map shrinks
+    uint32_t sdf_tiled = 1;
+    for (uint32_t g = 0; g < G; ++g) {
+        const float *m = pk->group_shape_xform + 6 * (size_t)g;
+        double a = m[0], b = m[1], c = m[3], d = m[4];
+        double q = a * a + b * b + c * c + d * d, det = a * d - b * c;
+        double smin = std::sqrt(std::max(0.5 * (q - std::sqrt(std::max(q * q - 4.0 * det * det, 0.0))), 0.0));
+        if (!(smin >= 1.0 - 1.0e-6)) sdf_tiled = 0;
+    }
 
     Arena A;
     auto addv = [&](const void *p, size_t bytes) { return A.add(p, p ? bytes : 0); };
@@ -476,6 +508,10 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         i_s2p = addv(s2p.data(), (size_t)std::max(S, 1u) * 4);
     }
     size_t i_gcb = addv(cpu_bounds.data(), cpu_bounds.size() * 4);
+    size_t i_gbm = addv(gbvh_meta.data(), gbvh_meta.size() * 4);
+    size_t i_gbb = addv(pk->group_bvh_bounds, pk->group_bvh_bounds ? pk->group_bvh_bounds_len * 4 : 0);
+    size_t i_gbn = addv(pk->group_bvh_nodes, pk->group_bvh_nodes ? pk->group_bvh_nodes_len * 4 : 0);
+    size_t i_gbi = addv(pk->group_bvh_indices, pk->group_bvh_indices ? pk->group_bvh_indices_len * 4 : 0);
     // plan of device-side updates: control-point indices of every curve and
     // the padding of every instance's bounds
     std::vector<uint32_t> curve_pt;
@@ -581,6 +617,11 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         D.scene_to_prepared = (dvgk::kuint *)at(i_s2p);
     }
     D.group_cpu_bounds = (dvgk::kfloat *)at(i_gcb);
+    D.gbvh_meta = (dvgk::kuint *)at(i_gbm);
+    D.gbvh_bounds = (dvgk::kfloat *)at(i_gbb);
+    D.gbvh_nodes = (dvgk::kuint *)at(i_gbn);
+    D.gbvh_indices = (dvgk::kuint *)at(i_gbi);
+    D.sdf_tiled = sdf_tiled;
     if (plan_ok) {
         dvgk::UpdatePlan &P = ds->plan;
         P.ncurves = (uint32_t)(curve_pt.size() / 4);
@@ -596,6 +637,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     ds->num_groups = G;
     ds->num_shapes = S;
     ds->list_max = list_max;
+    ds->ncurves = (uint32_t)ncurves;
     ds->gcap = prm->use_prefiltering ? 0u : dvgk::interior_gcap(list_max);
     {
         size_t npix = (size_t)W * H;
@@ -646,7 +688,11 @@ int32_t dvg_device_render_sdf(dvg_ctx *ctx, dvg_device_scene *ds, float *out) {
     if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
         return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    dvgk::launch_sdf(ds->S, out, ctx->stream);
+    const dvgk::DevScene &S = ds->S;
+    size_t bytes = dvgk::sdf_work_bytes(S.num_prepared, ds->ncurves, S.tiles_x * S.tiles_y);
+    int st = ensure(ctx, ctx->sdfw, ctx->sdfw_cap, (bytes + 3) / 4);
+    if (st) return st;
+    dvgk::launch_sdf(S, ds->ncurves, ctx->sdfw, out, ctx->stream);
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }
@@ -658,7 +704,11 @@ int32_t dvg_device_eval_positions(dvg_ctx *ctx, dvg_device_scene *ds, const floa
     if (!ds->S.prepared_to_scene || !ds->S.shape_transform)
         return set_err(ctx, DVG_INVALID_SCENE, "device scene created without a backward pack");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    dvgk::launch_sdf_points(ds->S, positions, (uint32_t)count, out, ctx->stream);
+    const dvgk::DevScene &S = ds->S;
+    size_t bytes = dvgk::sdf_work_bytes(S.num_prepared, ds->ncurves, S.tiles_x * S.tiles_y);
+    int st = ensure(ctx, ctx->sdfw, ctx->sdfw_cap, (bytes + 3) / 4);
+    if (st) return st;
+    dvgk::launch_sdf_points(S, ds->ncurves, ctx->sdfw, positions, (uint32_t)count, out, ctx->stream);
     HIP_TRY(ctx, hipGetLastError());
     return DVG_OK;
 }

@@ -40,6 +40,9 @@ struct dvg_ctx {
     // the slots), grown on demand
     uint32_t *imask = nullptr;
     size_t imask_cap = 0;
+    // signed-distance work arrays (dvgk::sdf_work_bytes), grown on demand
+    float *sdfw = nullptr;
+    size_t sdfw_cap = 0;
     uint32_t n_cu = 0;
     // second stream (default priority) for the boundary redo, which runs
     // beside the colour adjoint; fork / join events
@@ -56,6 +59,7 @@ struct dvg_device_scene {
     uint32_t num_groups = 0, num_shapes = 0;
     uint32_t grad_len[14] = {};  // lengths of the 14 gradient arrays
     uint32_t list_max = 0;       // longest tile list at creation
+    uint32_t ncurves = 0;        // curve segments of the pack
     uint32_t gcap = 0;           // list capacity of the adjoint's global bit-set slots (0: none needed)
 };
 

@@ -69,7 +69,40 @@ struct DevScene {
     // 4 per group, group-local: the root box of the reference's CPU group BVH
     // (union of distance/shape.rs shape_bounds); min > max when it has none
     kfloat *group_cpu_bounds;
+    // the pack's group BVHs (renderer/bvh.rs): node boxes (4 floats), nodes
+    // [left, skip, start, count], leaf shape indices, meta [node offset, node
+    // count, index offset, index count] per group (all zero: no BVH)
+    kfloat *gbvh_bounds;
+    kuint *gbvh_nodes, *gbvh_indices, *gbvh_meta;
+    // no group's shape-to-canvas map shrinks (smallest singular value >= 1):
+    // the reference's signed-distance walk then returns the smallest distance
+    // over all shapes, which the tiled signed-distance kernel computes
+    uint32_t sdf_tiled;
 };
+
+// Signed-distance work arrays (derived on the device at every call):
+// per prepared shape a canvas box of its boundary (4 floats, padded for
+// rounding; min > max when it has none), its similarity scale (0 when its
+// shape-to-canvas map is no similarity or it is a distance-approximation
+// path) and its group; per curve segment the box of its control points and
+// SDF_PIECES stadiums bounding sub-curves of equal parameter length (start
+// point and extent, 4 floats; radius and 1 / |extent|^2, 2 floats); per tile
+// the candidate radius, the shape nearest the centre and up to SDF_LCAP
+// candidate shapes with their count.
+constexpr uint32_t SDF_PIECES = 8, SDF_LCAP = 512, SDF_NONE = 0xffffffffu;
+struct SdfWork {
+    float *cbox, *sim, *cvbox, *piece, *piece_h, *tile_u;
+    uint32_t *pgroup, *tile_seed, *tile_count, *tile_list;
+};
+// The same arrays read through the constant address space.
+struct SdfView {
+    kfloat *cbox, *sim, *cvbox, *piece, *piece_h, *tile_u;
+    kuint *pgroup, *tile_seed, *tile_count, *tile_list;
+};
+// Bytes of the work arrays for np prepared shapes, nc curves and nt tiles, and
+// their carving from one buffer of that size (256-byte aligned pieces).
+size_t sdf_work_bytes(uint32_t np, uint32_t nc, uint32_t nt);
+SdfWork sdf_work_carve(void *buf, uint32_t np, uint32_t nc, uint32_t nt);
 
 struct DevGrads {
     float *d_shape_params, *d_shape_points, *d_shape_thickness, *d_shape_stroke_width, *d_shape_transform,
@@ -90,8 +123,10 @@ void launch_resolve(const DevScene &S, const float *weight, const float *accum, 
 // Signed distance (sdf.rs): W*H per-pixel means of the samples, or n points
 // (x, y pairs); both need the scene's backward tables (shape_transform,
 // prepared_to_scene).
-void launch_sdf(const DevScene &S, float *out, hipStream_t stream);
-void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream);
+// `work` holds sdf_work_bytes(num_prepared, ncurves, tiles) bytes.
+void launch_sdf(const DevScene &S, uint32_t ncurves, void *work, float *out, hipStream_t stream);
+void launch_sdf_points(const DevScene &S, uint32_t ncurves, void *work, const float *pos, uint32_t n, float *out,
+                       hipStream_t stream);
 // Gradients of the signed distances at n points for the adjoint d_sdf (the
 // reference CPU's render_backward_positions), accumulated into G.
 void launch_sdf_points_backward(const DevScene &S, const float *pos, uint32_t n, const float *d_sdf,

@@ -3075,10 +3075,62 @@ __global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restric
 }
 
 // ---- signed distance (Renderer::render_sdf / eval_positions) ------------------
+//
+// The reference's query (compute_signed_distance, sdf.rs:9-117, over
+// compute_distance_group, distance.rs:298-513) walks every group from the last
+// to the first; in a group with a BVH it skips a node whose group-local box
+// lies farther than the best canvas distance so far and measures every shape
+// of a visited leaf.  Two walks give its result here:
+//
+//   sdf_walk       that walk itself, over the scene's group BVHs, for any
+//                  scene (a node test in group-local units against a canvas
+//                  distance is exact only when the group does not shrink;
+//                  the walk keeps the reference's test, so it keeps the
+//                  reference's result when a group does shrink);
+//   k_sdf_tiles    for scenes whose groups do not shrink, where the walk's
+//                  result is the smallest distance over all shapes (the
+//                  highest group on a tie): per 16x16 tile, the shapes within
+//                  a radius U of the tile (U = the distance at the tile
+//                  centre + the half diagonal); a sample's minimum over them
+//                  is the scene minimum whenever it is below U (every other
+//                  shape is farther than U), else the sample takes sdf_walk.
+//
+// Both measure a shape as the reference does (same transforms, same solver)
+// and skip only work that cannot change the result: shapes whose canvas box
+// lies beyond the best distance, and path segments ruled out by lower bounds
+// (control-point boxes, then SDF_PIECES stadiums around sub-curves).
+
+DVG_DEV float box4_dist(kfloat *b, float px, float py) {  // INF for an empty box
+    float x0 = b[0], y0 = b[1], x1 = b[2], y1 = b[3];
+    if (!(x0 <= x1 && y0 <= y1)) return __builtin_inff();
+    return box_dist(x0, y0, x1, y1, px, py);
+}
+
+// True when every stadium of curve cv is farther than `cut` from (x, y): the
+// curve is then farther than cut.
+DVG_DEV bool pieces_beyond(const SdfView &V, uint32_t cv, float x, float y, float cut) {
+    kfloat *pc = V.piece + (size_t)cv * (SDF_PIECES * 4u);
+    kfloat *ph = V.piece_h + (size_t)cv * (SDF_PIECES * 2u);
+    for (uint32_t i = 0; i < SDF_PIECES; ++i) {
+        float wx = x - pc[4u * i], wy = y - pc[4u * i + 1u], vx = pc[4u * i + 2u], vy = pc[4u * i + 3u];
+        float t = clamp01((wx * vx + wy * vy) * ph[2u * i + 1u]);
+        float dx = wx - t * vx, dy = wy - t * vy;
+        float r = cut + ph[2u * i];
+        if (dx * dx + dy * dy <= r * r) return false;
+    }
+    return true;
+}
+
 // closest_point_shape, distance.rs:235-296: the closest boundary point of
-// prepared shape si in shape space; false when a path has no segment.
-DVG_DEV bool closest_point_shape(const DevScene &S, uint32_t si, float px, float py, float &ox, float &oy) {
-    const float *sd = S.shape_data + si * 15u;
+// prepared shape si to the shape-space point (px, py); false when a path has
+// no segment.  A path segment is skipped when a lower bound of its distance
+// exceeds the best segment distance so far (it cannot be the first nearest
+// segment) or `gcut` (the shape is then farther than gcut whichever segment is
+// nearest; gcut = INF when no such bound is known); the result is the full
+// scan's whenever the nearest point lies within gcut.
+DVG_DEV bool closest_point_shape_cut(const DevScene &S, const SdfView &V, uint32_t si, float px, float py,
+                                     float gcut, float &ox, float &oy) {
+    kfloat *sd = S.shape_data + si * 15u;
     uint32_t kind = (uint32_t)sd[0];
     float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
     if (kind == DVG_SHAPE_CIRCLE) {  // closest_point_circle, distance.rs:33-49
@@ -3114,13 +3166,23 @@ DVG_DEV bool closest_point_shape(const DevScene &S, uint32_t si, float px, float
         return true;
     }
     // closest_point_path, distance.rs:142-231: the first segment at the
-    // smallest distance, its point from the parameter
+    // smallest distance, its point from the parameter.  The approximate
+    // cubic distance is measured to a chord polygon, not to the curve, so no
+    // lower bound of the curve's distance applies to it: such paths scan
+    // every segment.
     uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
     bool approx = sd[14] > 0.5f;
+    float eps = 1.0e-6f * (absf_(px) + absf_(py)) + 1.0e-6f;  // rounding of the bounds
     float best = BIG;
     for (uint32_t s = 0; s < cnt; ++s) {
-        const float *cv = S.curve_data + (off + s) * 13u;
+        uint32_t cvi = off + s;
+        kfloat *cv = S.curve_data + cvi * 13u;
         uint32_t k = (uint32_t)cv[0];
+        if (!approx) {
+            float cut = fminf(best, gcut) * 1.00001f + eps;
+            if (box4_dist(V.cvbox + cvi * 4u, px, py) > cut) continue;
+            if (k != 0u && pieces_beyond(V, cvi, px, py, cut)) continue;
+        }
         float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
         float d, t;
         if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
@@ -3144,65 +3206,342 @@ DVG_DEV bool closest_point_shape(const DevScene &S, uint32_t si, float px, float
     return best < BIG;
 }
 
-// compute_signed_distance, sdf.rs:9-117: the nearest shape boundary over the
-// groups from last to first (compute_distance_group, distance.rs:298-513; a
-// shape whose group-local box is farther than the best distance so far is
-// skipped, as the group BVH skips nodes), negative inside the fill of the
-// nearest shape's group (fill_inside_group, backward.rs:1529-1606, which is
-// the forward's fill test).
-DVG_DEV float signed_distance(const DevScene &S, float px, float py) {
-    float best = BIG;
-    uint32_t best_g = 0;
-    bool found = false;
+// The canvas distance from (px, py) to prepared shape si of group g as
+// compute_distance_group measures it (distance.rs:346-413): group-local point
+// (lx, ly), shape-space closest point, back through the shape and group
+// transforms.  INF when the shape has no boundary, or when it provably lies
+// farther than `best` (a similarity map of scale s turns best into the
+// shape-space bound best / s).
+DVG_DEV float sdf_shape_dist(const DevScene &S, const SdfView &V, uint32_t si, uint32_t g, float px, float py,
+                             float lx, float ly, float best) {
+    kfloat *xf = S.shape_xform + si * 6u;
+    float spx = xf[0] * lx + xf[1] * ly + xf[2];
+    float spy = xf[3] * lx + xf[4] * ly + xf[5];
+    float s = V.sim[si];
+    float gcut = s > 0.f ? (best * 1.00001f + 1.0e-3f) / s : __builtin_inff();
+    float cx = 0.f, cy = 0.f;
+    if (!closest_point_shape_cut(S, V, si, spx, spy, gcut, cx, cy)) return __builtin_inff();
+    kfloat *T = S.shape_transform + S.prepared_to_scene[si] * 6u;
+    float gx = T[0] * cx + T[1] * cy + T[2], gy = T[3] * cx + T[4] * cy + T[5];
+    kfloat *gs = S.group_shape_xform + g * 6u;
+    float ccx = gs[0] * gx + gs[1] * gy + gs[2], ccy = gs[3] * gx + gs[4] * gy + gs[5];
+    float dx = ccx - px, dy = ccy - py;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+struct SdfBest {
+    float dist;
+    uint32_t group;
+    bool found;
+};
+
+// Measures shape si of group g unless its canvas box lies beyond the best
+// distance; the nearer shape wins, the higher group on a tie (the reference
+// visits groups from the last, keeping the first strict improvement).
+DVG_DEV void sdf_visit(const DevScene &S, const SdfView &V, uint32_t si, uint32_t g, float px, float py, float lx,
+                       float ly, SdfBest &b) {
+    if (box4_dist(V.cbox + si * 4u, px, py) > b.dist) return;
+    float d = sdf_shape_dist(S, V, si, g, px, py, lx, ly, b.dist);
+    if (d < b.dist || (d == b.dist && g > b.group)) {
+        b.dist = d;
+        b.group = g;
+        b.found = true;
+    }
+}
+
+// The reference's walk (compute_signed_distance's group loop with
+// compute_distance_group's BVH traversal).  The BVHs were checked on the
+// host: left and skip > node, leaves within the index range.  BVH = false
+// scans every group's shapes instead: the same result when no group shrinks
+// (no node the reference skips could hold a nearer shape).
+template <bool BVH>
+DVG_DEV SdfBest sdf_walk(const DevScene &S, const SdfView &V, float px, float py) {
+    SdfBest b = {BIG, 0u, false};
     for (uint32_t gg = S.num_groups; gg > 0; --gg) {
         uint32_t g = gg - 1u;
-        const float *gd = S.group_data + g * 16u;
-        uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
         float lx, ly;
         group_local(S, g, px, py, lx, ly);
-        const float *gs = S.group_shape_xform + g * 6u;
-        for (uint32_t i = 0; i < cnt; ++i) {
-            uint32_t si = (uint32_t)S.group_shapes[off + i];
-            const float *b = S.shape_bounds + si * 4u;
-            if (b[0] <= b[2] && b[1] <= b[3]) {  // bounds_distance, distance.rs:10-15
-                float ddx = maxf_(maxf_(b[0] - lx, 0.f), lx - b[2]);
-                float ddy = maxf_(maxf_(b[1] - ly, 0.f), ly - b[3]);
-                if (sqrtf(ddx * ddx + ddy * ddy) > best) continue;
+        kuint *meta = S.gbvh_meta + g * 4u;
+        if (BVH && meta[1] > 0u && meta[3] > 0u) {
+            uint32_t noff = meta[0], ioff = meta[2], node = 0;
+            while (node != SDF_NONE) {
+                kfloat *nb = S.gbvh_bounds + (size_t)(noff + node) * 4u;
+                kuint *nd = S.gbvh_nodes + (size_t)(noff + node) * 4u;
+                if (box_dist(nb[0], nb[1], nb[2], nb[3], lx, ly) > b.dist) {  // bounds_distance, distance.rs:9-15
+                    node = nd[1];
+                    continue;
+                }
+                uint32_t cnt = nd[3];
+                if (cnt == 0u) {
+                    node = nd[0];
+                    continue;
+                }
+                for (uint32_t i = 0; i < cnt; ++i)
+                    sdf_visit(S, V, S.gbvh_indices[ioff + nd[2] + i], g, px, py, lx, ly, b);
+                node = nd[1];
             }
-            const float *xf = S.shape_xform + si * 6u;
-            float spx = xf[0] * lx + xf[1] * ly + xf[2];
-            float spy = xf[3] * lx + xf[4] * ly + xf[5];
-            float cx = 0.f, cy = 0.f;
-            if (!closest_point_shape(S, si, spx, spy, cx, cy)) continue;
-            const float *T = S.shape_transform + S.prepared_to_scene[si] * 6u;
-            float gx = T[0] * cx + T[1] * cy + T[2], gy = T[3] * cx + T[4] * cy + T[5];
-            float ccx = gs[0] * gx + gs[1] * gy + gs[2], ccy = gs[3] * gx + gs[4] * gy + gs[5];
-            float dx = ccx - px, dy = ccy - py;
-            float d = sqrtf(dx * dx + dy * dy);
-            if (d < best) {
-                best = d;
-                best_g = g;
-                found = true;
-            }
+        } else {
+            kfloat *gd = S.group_data + g * 16u;
+            uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+            for (uint32_t i = 0; i < cnt; ++i) sdf_visit(S, V, (uint32_t)S.group_shapes[off + i], g, px, py, lx, ly, b);
         }
     }
-    if (!found) return 0.f;
-    const float *gd = S.group_data + best_g * 16u;
+    return b;
+}
+
+// Signed value of a walk's result: negative inside the fill of the nearest
+// shape's group (fill_inside_group, backward.rs:1529-1606, which is the
+// forward's fill test); 0 when no shape has a boundary.
+DVG_DEV float sdf_signed(const DevScene &S, const SdfBest &b, float px, float py) {
+    if (!b.found) return 0.f;
+    kfloat *gd = S.group_data + b.group * 16u;
     uint32_t fk = (uint32_t)gd[2], rule = (uint32_t)gd[7];
     bool inside = false;
     if (fk != DVG_PAINT_NONE) {
         float lx, ly;
-        group_local(S, best_g, px, py, lx, ly);
+        group_local(S, b.group, px, py, lx, ly);
         Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
-        group_cov<COV_FWD>(S, best_g, gd, lx, ly, true, false, rule, false, c);
+        group_cov<COV_FWD>(S, b.group, gd, lx, ly, true, false, rule, false, c);
         inside = rule == 1u ? c.crossings > 0.f : c.winding != 0.f;
     }
-    return inside ? -best : best;
+    return inside ? -b.dist : b.dist;
 }
 
-// render_sdf_kernel, sdf.rs:119-200: one pixel a lane, the mean signed
-// distance of its samples (the forward's sample positions).
-__global__ __launch_bounds__(256) void k_sdf(DevScene S, float *__restrict__ out) {
+// ---- work arrays, derived from the scene at every call -----------------------
+// The group of every prepared shape (group_data's shape lists).
+__global__ void k_sdf_prep_groups(DevScene S, uint32_t *__restrict__ pgroup) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.num_groups) return;
+    kfloat *gd = S.group_data + g * 16u;
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    for (uint32_t i = 0; i < cnt; ++i) pgroup[(uint32_t)S.group_shapes[off + i]] = g;
+}
+
+DVG_DEV float2 lerp2(float2 a, float2 b, float t) { return make_float2(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t); }
+
+// Canvas boxes and similarity scales of the prepared shapes (after
+// k_sdf_prep_groups), control-point boxes and stadiums of the curves.
+__global__ void k_sdf_prep(DevScene S, SdfWork Wk, uint32_t ncurves) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < S.num_prepared) {
+        kfloat *sd = S.shape_data + i * 15u;
+        uint32_t kind = (uint32_t)sd[0];
+        float x0 = __builtin_inff(), y0 = __builtin_inff(), x1 = -__builtin_inff(), y1 = -__builtin_inff();
+        if (kind == DVG_SHAPE_CIRCLE || kind == DVG_SHAPE_ELLIPSE) {
+            float rx = absf_(sd[6]), ry = kind == DVG_SHAPE_CIRCLE ? rx : absf_(sd[7]);
+            x0 = sd[4] - rx; x1 = sd[4] + rx; y0 = sd[5] - ry; y1 = sd[5] + ry;
+        } else if (kind == DVG_SHAPE_RECT) {
+            x0 = fminf(sd[4], sd[6]); x1 = fmaxf(sd[4], sd[6]); y0 = fminf(sd[5], sd[7]); y1 = fmaxf(sd[5], sd[7]);
+        } else {
+            uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+            for (uint32_t s = 0; s < cnt; ++s) {
+                kfloat *cv = S.curve_data + (off + s) * 13u;
+                for (int j = 0; j < 4; ++j) {
+                    x0 = fminf(x0, cv[1 + 2 * j]); x1 = fmaxf(x1, cv[1 + 2 * j]);
+                    y0 = fminf(y0, cv[2 + 2 * j]); y1 = fmaxf(y1, cv[2 + 2 * j]);
+                }
+            }
+        }
+        float4 box = make_float4(1.f, 1.f, 0.f, 0.f);  // empty
+        float sim = 0.f;
+        uint32_t g = Wk.pgroup[i];
+        if (g >= S.num_groups) g = 0;  // every prepared shape is in a group; stay in bounds regardless
+        kfloat *T = S.shape_transform + S.prepared_to_scene[i] * 6u;
+        kfloat *G = S.group_shape_xform + g * 6u;
+        if (x0 <= x1 && y0 <= y1) {
+            float cx0 = __builtin_inff(), cy0 = __builtin_inff(), cx1 = -__builtin_inff(), cy1 = -__builtin_inff();
+            for (int c = 0; c < 4; ++c) {
+                float ux = (c & 1) ? x1 : x0, uy = (c & 2) ? y1 : y0;
+                float gx = T[0] * ux + T[1] * uy + T[2], gy = T[3] * ux + T[4] * uy + T[5];
+                float vx = G[0] * gx + G[1] * gy + G[2], vy = G[3] * gx + G[4] * gy + G[5];
+                cx0 = fminf(cx0, vx); cx1 = fmaxf(cx1, vx); cy0 = fminf(cy0, vy); cy1 = fmaxf(cy1, vy);
+            }
+            float mag = fmaxf(fmaxf(absf_(cx0), absf_(cx1)), fmaxf(absf_(cy0), absf_(cy1)));
+            float pad = 1.0e-5f * mag + 1.0e-4f;
+            box = make_float4(cx0 - pad, cy0 - pad, cx1 + pad, cy1 + pad);
+        }
+        // shape-to-canvas linear map A = G T: a similarity when its columns
+        // are orthogonal and of one length
+        float a = G[0] * T[0] + G[1] * T[3], b = G[0] * T[1] + G[1] * T[4];
+        float c = G[3] * T[0] + G[4] * T[3], d = G[3] * T[1] + G[4] * T[4];
+        float n = absf_(a) + absf_(b) + absf_(c) + absf_(d), tol = 1.0e-6f * n;
+        bool rot = absf_(a - d) <= tol && absf_(b + c) <= tol, refl = absf_(a + d) <= tol && absf_(b - c) <= tol;
+        bool approx = kind == DVG_SHAPE_PATH && sd[14] > 0.5f;
+        if ((rot || refl) && !approx) sim = sqrtf(a * a + c * c);
+        reinterpret_cast<float4 *>(Wk.cbox)[i] = box;
+        Wk.sim[i] = sim;
+    }
+    if (i < ncurves) {
+        kfloat *cv = S.curve_data + (size_t)i * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        float2 p[4];
+        for (int j = 0; j < 4; ++j) p[j] = make_float2(cv[1 + 2 * j], cv[2 + 2 * j]);
+        float4 bx = make_float4(p[0].x, p[0].y, p[0].x, p[0].y);
+        for (int j = 1; j < 4; ++j)
+            bx = make_float4(fminf(bx.x, p[j].x), fminf(bx.y, p[j].y), fmaxf(bx.z, p[j].x), fmaxf(bx.w, p[j].y));
+        reinterpret_cast<float4 *>(Wk.cvbox)[i] = bx;
+        float4 *pc = reinterpret_cast<float4 *>(Wk.piece) + (size_t)i * SDF_PIECES;
+        float2 *ph = reinterpret_cast<float2 *>(Wk.piece_h) + (size_t)i * SDF_PIECES;
+        for (uint32_t q = 0; q < SDF_PIECES; ++q) {
+            if (k == 0u) {  // lines are measured directly
+                pc[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                ph[q] = make_float2(0.f, 0.f);
+                continue;
+            }
+            // control points of the sub-curve on [ta, tb]: blossoms
+            // (ta, .., ta, tb, .., tb), by de Casteljau steps
+            float ta = (float)q / (float)SDF_PIECES, tb = (float)(q + 1u) / (float)SDF_PIECES;
+            int deg = k == 1u ? 2 : 3;
+            float2 Q[4];
+            for (int m = 0; m <= deg; ++m) {
+                float2 w[4] = {p[0], p[1], p[2], p[3]};
+                for (int lev = 0; lev < deg; ++lev) {
+                    float tt = lev < deg - m ? ta : tb;
+                    for (int j = 0; j < deg - lev; ++j) w[j] = lerp2(w[j], w[j + 1], tt);
+                }
+                Q[m] = w[0];
+            }
+            // stadium: the chord line, the control points' extent along it
+            // and their largest distance from it
+            float2 u = make_float2(Q[deg].x - Q[0].x, Q[deg].y - Q[0].y);
+            float len = sqrtf(u.x * u.x + u.y * u.y);
+            float mag = 0.f;
+            for (int m = 0; m <= deg; ++m) mag = fmaxf(mag, fmaxf(absf_(Q[m].x), absf_(Q[m].y)));
+            float pad = 1.0e-5f * mag + 1.0e-5f;
+            if (len > 1.0e-6f * (mag + 1.0f)) {
+                u.x /= len; u.y /= len;
+                float smin = 0.f, smax = 0.f, h = 0.f;
+                for (int m = 1; m <= deg; ++m) {
+                    float wx = Q[m].x - Q[0].x, wy = Q[m].y - Q[0].y;
+                    float sp = wx * u.x + wy * u.y;
+                    smin = fminf(smin, sp); smax = fmaxf(smax, sp);
+                    h = fmaxf(h, absf_(wx * u.y - wy * u.x));
+                }
+                float ex = Q[0].x + smin * u.x, ey = Q[0].y + smin * u.y;
+                float vx = (smax - smin) * u.x, vy = (smax - smin) * u.y;
+                float vv = vx * vx + vy * vy;
+                pc[q] = make_float4(ex, ey, vx, vy);
+                ph[q] = make_float2(h + pad, vv > 0.f ? 1.0f / vv : 0.f);
+            } else {  // a point-like piece: a disc around Q0
+                float h = 0.f;
+                for (int m = 1; m <= deg; ++m)
+                    h = fmaxf(h, sqrtf((Q[m].x - Q[0].x) * (Q[m].x - Q[0].x) + (Q[m].y - Q[0].y) * (Q[m].y - Q[0].y)));
+                pc[q] = make_float4(Q[0].x, Q[0].y, 0.f, 0.f);
+                ph[q] = make_float2(h + pad, 0.f);
+            }
+        }
+    }
+}
+
+// One wave per tile: the nearest shape to the tile centre (lanes over the
+// shapes, then a wave minimum), the candidate radius U, and the shapes whose
+// canvas box lies within U of the tile, in prepared order (a hit's position is
+// the count of hits on lower lanes).  Lists longer than SDF_LCAP are cut; the
+// count is kept, and such tiles take the walk.
+constexpr uint32_t SDF_LIST_TILES_PER_WG = BLOCK / 64;
+__global__ __launch_bounds__(BLOCK) void k_sdf_lists(DevScene S, SdfView V, SdfWork Wk) {
+    uint32_t t = blockIdx.x * SDF_LIST_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    if (t >= ntiles) return;  // the whole wave
+    float x0 = (float)((t % S.tiles_x) * (uint32_t)TILE), y0 = (float)((t / S.tiles_x) * (uint32_t)TILE);
+    float cx = x0 + 0.5f * (float)TILE, cy = y0 + 0.5f * (float)TILE;
+    float lb = BIG;
+    uint32_t la = SDF_NONE;
+    for (uint32_t si = lane; si < S.num_prepared; si += 64u) {
+        if (box4_dist(V.cbox + si * 4u, cx, cy) > lb) continue;
+        uint32_t g = V.pgroup[si];
+        float lx, ly;
+        group_local(S, g, cx, cy, lx, ly);
+        float d = sdf_shape_dist(S, V, si, g, cx, cy, lx, ly, lb);
+        if (d < lb) {
+            lb = d;
+            la = si;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        float ob = __shfl_xor(lb, o);
+        uint32_t oa = __shfl_xor(la, o);
+        if (ob < lb || (ob == lb && oa < la)) {
+            lb = ob;
+            la = oa;
+        }
+    }
+    // a sample of the tile lies within the half diagonal of the centre
+    float U = lb < BIG ? lb * 1.0001f + 0.5f * 1.41422f * (float)TILE + 0.1f : __builtin_inff();
+    float x1 = x0 + (float)TILE, y1 = y0 + (float)TILE;
+    uint32_t count = 0;
+    uint32_t *list = Wk.tile_list + (size_t)t * SDF_LCAP;
+    for (uint32_t base = 0; base < S.num_prepared; base += 64u) {
+        uint32_t si = base + lane;
+        bool hit = false;
+        if (si < S.num_prepared) {
+            kfloat *b = V.cbox + si * 4u;
+            if (b[0] <= b[2] && b[1] <= b[3]) {
+                float dx = fmaxf(fmaxf(b[0] - x1, 0.f), x0 - b[2]);
+                float dy = fmaxf(fmaxf(b[1] - y1, 0.f), y0 - b[3]);
+                hit = sqrtf(dx * dx + dy * dy) <= U;
+            }
+        }
+        uint64_t m = __ballot(hit);
+        if (hit) {
+            uint32_t pos = count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (pos < SDF_LCAP) list[pos] = si;
+        }
+        count += (uint32_t)__popcll(m);
+    }
+    if (lane == 0u) {
+        Wk.tile_u[t] = U;
+        Wk.tile_seed[t] = la;
+        Wk.tile_count[t] = count;
+    }
+}
+
+// render_sdf_kernel, sdf.rs:119-200, for scenes whose groups do not shrink:
+// one workgroup a tile, one pixel a lane, the mean signed distance of its
+// samples (the forward's sample positions).  A sample measures the tile's
+// candidates, the one nearest the centre first; a minimum below the tile's U
+// is the scene minimum (every shape off the list lies farther than U), else
+// (or for a cut list) the sample takes the walk.
+__global__ __launch_bounds__(256) void k_sdf_tiles(DevScene S, SdfView V, float *__restrict__ out) {
+    uint32_t t = blockIdx.x;
+    uint32_t x = (t % S.tiles_x) * (uint32_t)TILE + threadIdx.x % (uint32_t)TILE;
+    uint32_t y = (t / S.tiles_x) * (uint32_t)TILE + threadIdx.x / (uint32_t)TILE;
+    if (x >= S.W || y >= S.H) return;
+    uint32_t n = V.tile_count[t], seed = V.tile_seed[t];
+    float U = V.tile_u[t];
+    bool listed = n <= SDF_LCAP;
+    kuint *list = V.tile_list + (size_t)t * SDF_LCAP;
+    float acc = 0.f;
+    for (uint32_t sy = 0; sy < S.nsy; ++sy)
+        for (uint32_t sx = 0; sx < S.nsx; ++sx) {
+            float px, py;
+            sample_point<false>(S, x, y, sx, sy, px, py);
+            SdfBest b = {BIG, 0u, false};
+            if (listed) {
+                if (seed != SDF_NONE) {
+                    uint32_t g = V.pgroup[seed];
+                    float lx, ly;
+                    group_local(S, g, px, py, lx, ly);
+                    sdf_visit(S, V, seed, g, px, py, lx, ly, b);
+                }
+                for (uint32_t k = 0; k < n; ++k) {
+                    uint32_t si = list[k];
+                    if (si == seed || box4_dist(V.cbox + si * 4u, px, py) > b.dist) continue;
+                    uint32_t g = V.pgroup[si];
+                    float lx, ly;
+                    group_local(S, g, px, py, lx, ly);
+                    sdf_visit(S, V, si, g, px, py, lx, ly, b);
+                }
+            }
+            if (!listed || !(b.found && b.dist < U)) b = sdf_walk<false>(S, V, px, py);
+            acc += sdf_signed(S, b, px, py);
+        }
+    out[(size_t)y * S.W + x] = acc * (1.0f / (float)(S.nsx * S.nsy));
+}
+
+// render_sdf_kernel by the walk (scenes with a shrinking group): one pixel a
+// lane.
+__global__ __launch_bounds__(256) void k_sdf_walk(DevScene S, SdfView V, float *__restrict__ out) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= S.W * S.H) return;
     uint32_t x = i % S.W, y = i / S.W;
@@ -3211,18 +3550,20 @@ __global__ __launch_bounds__(256) void k_sdf(DevScene S, float *__restrict__ out
         for (uint32_t sx = 0; sx < S.nsx; ++sx) {
             float px, py;
             sample_point<false>(S, x, y, sx, sy, px, py);
-            acc += signed_distance(S, px, py);
+            acc += sdf_signed(S, sdf_walk<true>(S, V, px, py), px, py);
         }
     out[i] = acc * (1.0f / (float)(S.nsx * S.nsy));
 }
 
 // eval_positions_kernel, sdf.rs:207-262: the signed distance at n canvas
-// points (x, y pairs).
-__global__ __launch_bounds__(256) void k_sdf_points(DevScene S, const float *__restrict__ pos, uint32_t n,
+// points (x, y pairs), by the walk.
+__global__ __launch_bounds__(256) void k_sdf_points(DevScene S, SdfView V, const float *__restrict__ pos, uint32_t n,
                                                     float *__restrict__ out) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    out[i] = signed_distance(S, pos[2u * i], pos[2u * i + 1u]);
+    float px = pos[2u * i], py = pos[2u * i + 1u];
+    out[i] = S.sdf_tiled ? sdf_signed(S, sdf_walk<false>(S, V, px, py), px, py)
+                         : sdf_signed(S, sdf_walk<true>(S, V, px, py), px, py);
 }
 
 // sample_distance (backward/sampling.rs:14-68) for the adjoint d_dist of the
@@ -3657,15 +3998,75 @@ void launch_forward(const DevScene &S, const float *weight, float *accum, hipStr
     if (S.prefilter) hipLaunchKernelGGL(k_forward<true>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
     else hipLaunchKernelGGL(k_forward<false>, grid, dim3(BLOCK), 0, stream, S, weight, accum);
 }
-void launch_sdf(const DevScene &S, float *out, hipStream_t stream) {
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+size_t sdf_work_bytes(uint32_t np, uint32_t nc, uint32_t nt) {
+    size_t n = std::max(np, 1u), c = std::max(nc, 1u), t = std::max(nt, 1u);
+    return align256(n * 16) + align256(n * 4) + align256(c * 16) + align256(c * SDF_PIECES * 16) +
+           align256(c * SDF_PIECES * 8) + align256(t * 4) + align256(n * 4) + align256(t * 4) + align256(t * 4) +
+           align256(t * SDF_LCAP * 4);
+}
+SdfWork sdf_work_carve(void *buf, uint32_t np, uint32_t nc, uint32_t nt) {
+    size_t n = std::max(np, 1u), c = std::max(nc, 1u), t = std::max(nt, 1u);
+    uint8_t *p = (uint8_t *)buf;
+    auto take = [&](size_t bytes) {
+        void *r = p;
+        p += align256(bytes);
+        return r;
+    };
+    SdfWork w;
+    w.cbox = (float *)take(n * 16);
+    w.sim = (float *)take(n * 4);
+    w.cvbox = (float *)take(c * 16);
+    w.piece = (float *)take(c * SDF_PIECES * 16);
+    w.piece_h = (float *)take(c * SDF_PIECES * 8);
+    w.tile_u = (float *)take(t * 4);
+    w.pgroup = (uint32_t *)take(n * 4);
+    w.tile_seed = (uint32_t *)take(t * 4);
+    w.tile_count = (uint32_t *)take(t * 4);
+    w.tile_list = (uint32_t *)take(t * SDF_LCAP * 4);
+    return w;
+}
+static SdfView sdf_view(const SdfWork &w) {
+    SdfView v;
+    v.cbox = (kfloat *)w.cbox;
+    v.sim = (kfloat *)w.sim;
+    v.cvbox = (kfloat *)w.cvbox;
+    v.piece = (kfloat *)w.piece;
+    v.piece_h = (kfloat *)w.piece_h;
+    v.tile_u = (kfloat *)w.tile_u;
+    v.pgroup = (kuint *)w.pgroup;
+    v.tile_seed = (kuint *)w.tile_seed;
+    v.tile_count = (kuint *)w.tile_count;
+    v.tile_list = (kuint *)w.tile_list;
+    return v;
+}
+static void launch_sdf_prep(const DevScene &S, uint32_t ncurves, const SdfWork &w, hipStream_t stream) {
+    if (S.num_groups)
+        hipLaunchKernelGGL(k_sdf_prep_groups, dim3(cdiv(S.num_groups, 256)), dim3(256), 0, stream, S, w.pgroup);
+    uint32_t n = std::max(S.num_prepared, ncurves);
+    if (n) hipLaunchKernelGGL(k_sdf_prep, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, w, ncurves);
+}
+void launch_sdf(const DevScene &S, uint32_t ncurves, void *work, float *out, hipStream_t stream) {
     uint64_t npix = (uint64_t)S.W * S.H;
     if (npix == 0) return;
-    hipLaunchKernelGGL(k_sdf, dim3(cdiv(npix, 256)), dim3(256), 0, stream, S, out);
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    SdfWork w = sdf_work_carve(work, S.num_prepared, ncurves, ntiles);
+    SdfView v = sdf_view(w);
+    launch_sdf_prep(S, ncurves, w, stream);
+    if (S.sdf_tiled) {
+        hipLaunchKernelGGL(k_sdf_lists, dim3(cdiv(ntiles, SDF_LIST_TILES_PER_WG)), dim3(BLOCK), 0, stream, S, v, w);
+        hipLaunchKernelGGL(k_sdf_tiles, dim3(ntiles), dim3(256), 0, stream, S, v, out);
+    } else {
+        hipLaunchKernelGGL(k_sdf_walk, dim3(cdiv(npix, 256)), dim3(256), 0, stream, S, v, out);
+    }
 }
 
-void launch_sdf_points(const DevScene &S, const float *pos, uint32_t n, float *out, hipStream_t stream) {
+void launch_sdf_points(const DevScene &S, uint32_t ncurves, void *work, const float *pos, uint32_t n, float *out,
+                       hipStream_t stream) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_sdf_points, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, pos, n, out);
+    SdfWork w = sdf_work_carve(work, S.num_prepared, ncurves, S.tiles_x * S.tiles_y);
+    launch_sdf_prep(S, ncurves, w, stream);
+    hipLaunchKernelGGL(k_sdf_points, dim3(cdiv(n, 256)), dim3(256), 0, stream, S, sdf_view(w), pos, n, out);
 }
 
 void launch_sdf_image_backward(const DevScene &S, const float *d_sdf, const DevGrads &G, hipStream_t stream) {
@@ -3795,6 +4196,33 @@ void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wi
     hipLaunchKernelGGL(k_wind_coef, dim3(cdiv(ncurves, 256)), dim3(256), 0, stream, curve_data, ncurves, curve_wind);
 }
 
+// Group BVH boxes refitted to the updated shape bounds, one lane a group: a
+// node's box is the union of the shapes of the leaves in its subtree (nodes
+// [k, skip) in preorder).  The tree keeps the split of the scene it was built
+// for.
+__global__ void k_upd_group_bvh(DevScene S, float *__restrict__ bounds) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.num_groups) return;
+    kuint *m = S.gbvh_meta + g * 4u;
+    uint32_t noff = m[0], ncnt = m[1], ioff = m[2];
+    if (ncnt == 0u || m[3] == 0u) return;
+    for (uint32_t k = 0; k < ncnt; ++k) {
+        uint32_t skip = S.gbvh_nodes[(size_t)(noff + k) * 4u + 1u];
+        uint32_t end = skip == SDF_NONE ? ncnt : skip;
+        float x0 = __builtin_inff(), y0 = __builtin_inff(), x1 = -__builtin_inff(), y1 = -__builtin_inff();
+        for (uint32_t j = k; j < end; ++j) {
+            kuint *nd = S.gbvh_nodes + (size_t)(noff + j) * 4u;
+            for (uint32_t i = 0; i < nd[3]; ++i) {
+                kfloat *b = S.shape_bounds + S.gbvh_indices[ioff + nd[2] + i] * 4u;
+                if (!(b[0] <= b[2] && b[1] <= b[3])) continue;
+                x0 = fminf(x0, b[0]); y0 = fminf(y0, b[1]); x1 = fmaxf(x1, b[2]); y1 = fmaxf(y1, b[3]);
+            }
+        }
+        float *o = bounds + (size_t)(noff + k) * 4u;
+        o[0] = x0; o[1] = y0; o[2] = x1; o[3] = y1;
+    }
+}
+
 void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
                          hipStream_t stream) {
     // the scene's own arena buffers, written in place
@@ -3820,6 +4248,9 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
     if (S.num_groups)
         hipLaunchKernelGGL(k_upd_groups, dim3(cdiv(S.num_groups, 256)), dim3(256), 0, stream, S, shape_bounds,
                            P.cpu_inst_bounds, group_data, group_colors, reinterpret_cast<int4 *>(P.ranges), cpu_root);
+    if (S.num_groups)
+        hipLaunchKernelGGL(k_upd_group_bvh, dim3(cdiv(S.num_groups, 256)), dim3(256), 0, stream, S,
+                           (float *)(S.gbvh_bounds));
     if (ntiles > 0) {
         launch_bin_count(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, P.tile_counts,
                          stream);

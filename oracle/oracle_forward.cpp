@@ -488,12 +488,82 @@ struct PreparedInstance {
     float inv[6];          // inverse shape transform, rows 0-1
     std::vector<Seg> segs; // curve segments in shape space
 };
+// A node of a group BVH (renderer/bvh.rs:44-59); items are positions in
+// PreparedGroup::shapes.
+struct BvhNode {
+    float b[4];
+    uint32_t left, right, skip, start, count;
+};
 struct PreparedGroup {
     int tile[4] = {1, 1, 0, 0};  // 16-px tile range of the canvas AABB (count mode)
     float xform[6];        // canvas_to_shape rows 0-1
     float inv_scale;
     std::vector<PreparedInstance> shapes;
+    std::vector<BvhNode> bvh;       // empty: no shape with bounds
+    std::vector<uint32_t> bvh_items;
 };
+
+static const uint32_t BVH_NONE = 0xffffffffu;
+
+// build_bvh_node, renderer/bvh.rs:149-190: median split by box centre on the
+// longer axis (a stable sort, as sort_by), leaves of at most 8 items
+// (renderer/constants.rs:23).
+static uint32_t bvh_build(std::vector<BvhNode> &nodes, const std::vector<const float *> &boxes,
+                          std::vector<uint32_t> &idx, size_t start, size_t end) {
+    BvhNode n;
+    n.b[0] = INFINITY; n.b[1] = INFINITY; n.b[2] = -INFINITY; n.b[3] = -INFINITY;
+    for (size_t i = start; i < end; ++i) {
+        const float *o = boxes[idx[i]];
+        n.b[0] = rmin(n.b[0], o[0]); n.b[1] = rmin(n.b[1], o[1]);
+        n.b[2] = rmax(n.b[2], o[2]); n.b[3] = rmax(n.b[3], o[3]);
+    }
+    n.left = n.right = n.skip = BVH_NONE;
+    n.start = (uint32_t)start;
+    n.count = (uint32_t)(end - start);
+    uint32_t id = (uint32_t)nodes.size();
+    nodes.push_back(n);
+    if (end - start <= 8) return id;
+    int axis = (n.b[2] - n.b[0]) >= (n.b[3] - n.b[1]) ? 0 : 1;
+    std::stable_sort(idx.begin() + start, idx.begin() + end, [&](uint32_t a, uint32_t b) {
+        return (boxes[a][axis] + boxes[a][axis + 2]) * 0.5f < (boxes[b][axis] + boxes[b][axis + 2]) * 0.5f;
+    });
+    size_t mid = start + (end - start) / 2;
+    uint32_t l = bvh_build(nodes, boxes, idx, start, mid);
+    uint32_t r = bvh_build(nodes, boxes, idx, mid, end);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    nodes[id].count = 0;
+    return id;
+}
+
+// append_group_bvh, renderer/bvh.rs:62-109, over the shapes with bounds, with
+// the skip pointers of assign_bvh_skip (:192-215).
+static void group_bvh(PreparedGroup &pg) {
+    std::vector<const float *> boxes;
+    std::vector<uint32_t> ids;
+    for (size_t i = 0; i < pg.shapes.size(); ++i)
+        if (pg.shapes[i].bounds_valid) {
+            boxes.push_back(pg.shapes[i].b);
+            ids.push_back((uint32_t)i);
+        }
+    if (boxes.empty()) return;
+    std::vector<uint32_t> idx(boxes.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (uint32_t)i;
+    uint32_t root = bvh_build(pg.bvh, boxes, idx, 0, idx.size());
+    std::vector<std::pair<uint32_t, uint32_t>> st = {{root, BVH_NONE}};
+    while (!st.empty()) {
+        auto [id, sk] = st.back();
+        st.pop_back();
+        if (id == BVH_NONE) continue;
+        BvhNode &n = pg.bvh[id];
+        n.skip = sk;
+        if (n.count == 0) {
+            if (n.right != BVH_NONE) st.push_back({n.right, sk});
+            if (n.left != BVH_NONE) st.push_back({n.left, n.right});
+        }
+    }
+    for (uint32_t i : idx) pg.bvh_items.push_back(ids[i]);
+}
 
 static void prepare(const Scene &sc, const dvg_render_options &opt, std::vector<PreparedGroup> &out) {
     out.clear();
@@ -563,6 +633,7 @@ static void prepare(const Scene &sc, const dvg_render_options &opt, std::vector<
             }
             pg.shapes.push_back(pi);
         }
+        group_bvh(pg);
         // canvas AABB -> tile range, as bin_tiles_count (binning.rs:30-110)
         {
             float u[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
@@ -1038,11 +1109,12 @@ static bool closest_point_shape(const Shape &sh, const PreparedInstance &pi, flo
 }
 
 // compute_signed_distance, sdf.rs:9-117: the nearest shape boundary over the
-// groups from last to first (compute_distance_group, distance.rs:298-513,
-// with shapes skipped when their group-local box is farther than the best
-// distance so far, as the group BVH skips nodes), negative inside the fill of
-// the nearest shape's group (fill_inside_group, backward.rs:1529-1606: the
-// forward's fill test).
+// groups from last to first (compute_distance_group, distance.rs:298-513): a
+// group's BVH walked by its skip pointers, a node skipped when its group-local
+// box lies farther than the best canvas distance so far, every shape of a
+// visited leaf measured (a group without shapes with bounds scans its shapes);
+// negative inside the fill of the nearest shape's group (fill_inside_group,
+// backward.rs:1529-1606: the forward's fill test).
 static float signed_distance(const Scene &sc, const std::vector<PreparedGroup> &pgs, float px, float py) {
     float best = 1.0e20f;
     size_t best_g = 0;
@@ -1053,18 +1125,13 @@ static float signed_distance(const Scene &sc, const std::vector<PreparedGroup> &
         const float *x = pg.xform;
         float lx = x[0] * px + x[1] * py + x[2];
         float ly = x[3] * px + x[4] * py + x[5];
-        for (const PreparedInstance &pi : pg.shapes) {
-            if (pi.b[0] <= pi.b[2] && pi.b[1] <= pi.b[3]) {  // bounds_distance, distance.rs:10-15
-                float ddx = max_f(max_f(pi.b[0] - lx, 0.0f), lx - pi.b[2]);
-                float ddy = max_f(max_f(pi.b[1] - ly, 0.0f), ly - pi.b[3]);
-                if (sqrt_f(ddx * ddx + ddy * ddy) > best) continue;
-            }
+        auto measure = [&](const PreparedInstance &pi) {
             const float *iv = pi.inv;
             float spx = iv[0] * lx + iv[1] * ly + iv[2];
             float spy = iv[3] * lx + iv[4] * ly + iv[5];
             const Shape &sh = sc.shapes[pi.shape];
             float cx = 0.f, cy = 0.f;
-            if (!closest_point_shape(sh, pi, spx, spy, cx, cy)) continue;
+            if (!closest_point_shape(sh, pi, spx, spy, cx, cy)) return;
             const M3 &T = sh.transform;
             float gx = T.m[0][0] * cx + T.m[0][1] * cy + T.m[0][2];
             float gy = T.m[1][0] * cx + T.m[1][1] * cy + T.m[1][2];
@@ -1078,6 +1145,25 @@ static float signed_distance(const Scene &sc, const std::vector<PreparedGroup> &
                 best_g = gi;
                 found = true;
             }
+        };
+        if (!pg.bvh.empty()) {
+            uint32_t node = 0;
+            while (node != BVH_NONE) {
+                const BvhNode &n = pg.bvh[node];
+                // bounds_distance, distance.rs:9-15
+                float ddx = max_f(max_f(n.b[0] - lx, 0.0f), lx - n.b[2]);
+                float ddy = max_f(max_f(n.b[1] - ly, 0.0f), ly - n.b[3]);
+                if (sqrt_f(ddx * ddx + ddy * ddy) > best) {
+                    node = n.skip;
+                } else if (n.count > 0) {
+                    for (uint32_t i = 0; i < n.count; ++i) measure(pg.shapes[pg.bvh_items[n.start + i]]);
+                    node = n.skip;
+                } else {
+                    node = n.left;
+                }
+            }
+        } else {
+            for (const PreparedInstance &pi : pg.shapes) measure(pi);
         }
     }
     if (!found) return 0.0f;

@@ -39,6 +39,22 @@ def _scaled_scene():
     return Scene(48, 44, shapes=shapes, groups=[g0, g1], background=(1, 1, 1, 1))
 
 
+def _shrunk_groups():
+    """Four groups of 12 mixed shapes (BVHs of two levels), shape-to-canvas
+    scales 0.6, 1.0, 1.5 and 0.8 with a slight rotation: the reference's
+    group-local node test then skips nodes by local distances."""
+    sc = scenes.mixed_scene(96, 96, 48, 3)
+    base = sc.groups
+    sc.groups = []
+    for k, cut in enumerate(range(0, 48, 12)):
+        g = ShapeGroup(list(range(cut, cut + 12)), fill=base[cut].fill or Solid((0.3, 0.3, 0.3, 1.0)),
+                       stroke=base[cut].stroke)
+        s = (0.6, 1.0, 1.5, 0.8)[k]
+        g.set_shape_to_canvas(np.array([[s, 0.1, 4.0 * k], [-0.1, s, 3.0], [0, 0, 1]], np.float32))
+        sc.groups.append(g)
+    return sc
+
+
 def _cases():
     pf = RenderOptions(samples_x=1, samples_y=1, seed=0, jitter=False, use_prefiltering=True)
     return {
@@ -48,6 +64,7 @@ def _cases():
                                                   seg_range=(2, 5)),
         "mixed32_pf": (scenes.mixed_scene(32, 32, 10, seed=14), pf),
         "scaled_groups": (_scaled_scene(), RenderOptions(samples_x=2, samples_y=2, seed=3)),
+        "shrunk_groups": (_shrunk_groups(), RenderOptions(samples_x=2, samples_y=2, seed=5)),
     }
 
 
@@ -61,13 +78,52 @@ def test_render_sdf_matches_oracle(renderer, oracle, name):
     assert (ref < 0).any() and (ref > 0).any()
 
 
-@pytest.mark.parametrize("name", ["mixed64", "scaled_groups"])
+@pytest.mark.parametrize("name", ["mixed64", "scaled_groups", "shrunk_groups"])
 def test_eval_positions_matches_oracle(renderer, oracle, name):
     sc, opts = _cases()[name]
     rng = np.random.default_rng(4)
     p = np.stack([rng.uniform(-8, sc.width + 8, 3000), rng.uniform(-8, sc.height + 8, 3000)], 1).astype(np.float32)
     print(f"case eval_positions {name}", flush=True)
     _check(name, renderer.eval_positions(sc, p, opts), oracle.eval_positions(SceneDesc(sc), opts, p))
+
+
+def test_shrinking_group_keeps_the_reference_node_test(renderer):
+    """The reference skips a group BVH node by its group-local distance
+    (distance.rs:346-360): at p, group 0's circle (12 px away, 24 local units)
+    is skipped while group 1's circle is 15 px away, and wins when that one is
+    40 px away (the closed forms of the oracle's pin)."""
+    import test_oracle_sdf as t
+    p = np.array([[50.0, 40.0]], np.float32)
+    for far, want in ((False, 15.0), (True, 12.0)):
+        sc = t._shrunk_pair(far)
+        print(f"case shrunk pair far={far}", flush=True)
+        assert abs(float(renderer.eval_positions(sc, p, t.OPTS)[0]) - want) < 1e-4
+        img = renderer.render_sdf(sc, t.OPTS)
+        assert abs(float(img[40 * sc.width + 50]) - float(
+            renderer.eval_positions(sc, np.array([[50.5, 40.5]], np.float32), t.OPTS)[0])) < 1e-5
+
+
+def test_tiled_sdf_on_the_full_canvas(renderer, oracle):
+    """The 1024^2 image of 1000 cubic paths (2x2 samples, unjittered), computed
+    by the tiled kernel (candidate lists per tile): against the walk at every
+    sample (eval_positions, pixel means), and against the oracle at the
+    samples of three 32x32 crops."""
+    sc, _ = scenes.headline()
+    opts = RenderOptions(samples_x=2, samples_y=2, seed=0, jitter=False, use_prefiltering=False)
+    print("case tiled sdf headline", flush=True)
+    img = renderer.render_sdf(sc, opts).reshape(sc.height, sc.width)
+    offs = np.array([[0.25, 0.25], [0.75, 0.25], [0.25, 0.75], [0.75, 0.75]], np.float32)
+    yy, xx = np.mgrid[0:sc.height, 0:sc.width]
+    pix = np.stack([xx.reshape(-1), yy.reshape(-1)], 1).astype(np.float32)
+    pts = (pix[:, None, :] + offs[None]).reshape(-1, 2)
+    walk = renderer.eval_positions(sc, pts, opts).reshape(sc.height, sc.width, 4).mean(2)
+    _check("tiled against walk", img.reshape(-1), walk.reshape(-1).astype(np.float32))
+    for y0, x0 in ((0, 0), (500, 320), (992, 992)):
+        crop = (pix.reshape(sc.height, sc.width, 2)[y0:y0 + 32, x0:x0 + 32].reshape(-1, 2)[:, None, :]
+                + offs[None]).reshape(-1, 2)
+        ref = oracle.eval_positions(SceneDesc(sc), opts, crop).reshape(32, 32, 4).mean(2)
+        _check(f"tiled against oracle at ({x0}, {y0})", img[y0:y0 + 32, x0:x0 + 32].reshape(-1),
+               ref.reshape(-1).astype(np.float32))
 
 
 def test_eval_positions_full_canvas(renderer, oracle):

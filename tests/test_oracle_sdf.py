@@ -132,3 +132,46 @@ def test_sdf_image_backward_is_points_backward_at_pixel_centres(oracle):
     for k in ("d_shape_params", "d_shape_transform", "d_group_transform", "d_translation"):
         assert np.array_equal(a[k], b[k]), k
     assert np.abs(a["d_shape_params"]).max() > 0
+
+
+def _shrunk_pair(far):
+    """Group 0 maps shape space to canvas at scale 0.5: its circle (centre
+    (100, 110), radius 6) lies at canvas (50, 55), radius 3, 12 px from
+    p = (50, 40) but 24 group-local units away.  Group 1's circle (radius 5)
+    is 15 px from p, or 40 px when `far`."""
+    g0 = _fill([0])
+    g0.set_shape_to_canvas(np.array([[0.5, 0, 0], [0, 0.5, 0], [0, 0, 1]], np.float32))
+    c1 = (50.0, -5.0) if far else (50.0, 20.0)
+    return _scene([Shape(Circle((100.0, 110.0), 6.0)), Shape(Circle(c1, 5.0))], [g0, _fill([1])], w=100, h=80)
+
+
+def test_shrinking_group_keeps_the_reference_node_test(oracle):
+    """compute_distance_group (distance.rs:346-360) skips a BVH node whose
+    group-local box distance exceeds the best canvas distance so far.  Group 1
+    is walked first (groups from the last): with its circle 15 px away, group
+    0's node (24 local units) is skipped although its circle is 12 px away;
+    with group 1's circle 40 px away the node is visited and 12 wins."""
+    p = np.array([[50.0, 40.0]], np.float32)
+    near = oracle.eval_positions(SceneDesc(_shrunk_pair(False)), OPTS, p)
+    far = oracle.eval_positions(SceneDesc(_shrunk_pair(True)), OPTS, p)
+    assert abs(float(near[0]) - 15.0) < 1e-5
+    assert abs(float(far[0]) - 12.0) < 1e-5
+
+
+def test_group_bvh_walk_finds_the_nearest_of_many(oracle):
+    """A group of 40 circles (a BVH of several levels) scaled by 1.25: the
+    walk's value is the closed-form nearest boundary distance, negative inside
+    any circle (non-zero fill of circles of one orientation)."""
+    rng = np.random.default_rng(8)
+    c = rng.uniform(5, 75, size=(40, 2))
+    r = rng.uniform(1.0, 4.0, size=40)
+    g = _fill(list(range(40)))
+    s = 1.25
+    g.set_shape_to_canvas(np.array([[s, 0, -3.0], [0, s, 2.0], [0, 0, 1]], np.float32))
+    sc = _scene([Shape(Circle(tuple(ci), float(ri))) for ci, ri in zip(c, r)], [g], w=100, h=100)
+    p = _pts(rng, 600, 100, 100)
+    got = oracle.eval_positions(SceneDesc(sc), OPTS, p)
+    cc = c * s + np.array([-3.0, 2.0])
+    d = np.hypot(p[:, None, 0] - cc[None, :, 0], p[:, None, 1] - cc[None, :, 1]) - s * r[None, :]
+    want = np.where((d < 0).any(1), -np.abs(d).min(1), np.abs(d).min(1))
+    assert np.allclose(got, want, atol=5e-5, rtol=1e-5)
