@@ -230,9 +230,12 @@ typedef struct dvg_grads {
 } dvg_grads;
 
 /* PreparedScene, src/renderer/prepare.rs:20-67.  Indices into shape_data are
- * prepared indices (order of first appearance in the groups' shape lists,
- * prepare.rs:194-196).  segment_data and the group BVH are accepted for ABI
- * completeness; the kernels cull with their own tile lists instead. */
+ * prepared indices: one per entry of the groups' shape lists, in group order
+ * (prepare.rs:186-205).  segment_data and the path BVHs are accepted as the
+ * reference fills them; the render and adjoint kernels cull with their own
+ * tile lists.  The group BVHs (renderer/bvh.rs layout) are checked at upload
+ * (DVG_INVALID_SCENE "invalid group BVH node" etc.) and walked by the signed
+ * distance; a null group_bvh_meta, or a group's zero meta, is no BVH. */
 typedef struct dvg_scene_pack {
     uint32_t num_groups;
     uint32_t num_prepared_shapes;
