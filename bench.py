@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="headline", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch-scenes", type=int, default=4,
+                    help="headline-size scenes in the batched measurement of the default line (0: none)")
     ap.add_argument("--unchecked-boundary", action="store_true",
                     help="the unchecked f32 boundary pass instead of the default checked one (single-scene workloads)")
     args = ap.parse_args()
@@ -205,13 +207,19 @@ def main():
         if os.path.exists(tr_path):
             with open(tr_path) as f:
                 traffic = json.load(f).get("traffic_bytes", {}).get(dom)
-        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                     "kernel": "k_bnd_eval" if dom == "boundary_eval" else dom, "kernel_ms": ms,
                     "per_kernel_tflops": {k: round(alg[k] / (per_kernel_ms[k] * 1e-3) / 1e12, 3)
                                           for k in alg if per_kernel_ms.get(k)},
-                    "note": "peak is MI355X fp32 (vector) 157.3 TFLOP/s: the path is scalar fp32 VALU work, "
-                            f"no MFMA; algorithmic flops per sample from profiles/flops_{args.workload}.json"}
+                    "note": "VALU-bound fp32 scalar geometry (no MFMA, not HBM); peak is MI355X's published "
+                            "fp32 vector 157.3 TFLOP/s, which counts packed fp32 (this build compiles without "
+                            "packed fp32 ops: its own ceiling is half that); algorithmic flops per sample counted "
+                            f"by the oracle's op-counting scalar type, profiles/flops_{args.workload}.json"}
+
+    batch = None
+    if headline and args.batch_scenes > 0:
+        batch = bench_headline_batch(args, r, dev, args.batch_scenes)
 
     cpu = None
     if headline and not args.no_cpu_baseline:
@@ -231,9 +239,72 @@ def main():
                    "boundary_pass": "unchecked f32" if args.unchecked_boundary else "checked, uncertain samples in f64"},
         "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adam_steps_per_s": (1.0 / adam_s) if adam_s else None, "adam": adam,
         "kernel_ms": {k: round(v, 4) for k, v in per_kernel_ms.items()},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))
+
+
+def bench_headline_batch(args, r, dev, n):
+    """n headline-size scenes (seeds 0..n-1) stepped together by the batch
+    (one persistent launch per phase, replayed from a HIP graph) against the
+    plain per-scene loop over the same device scenes.  Both run the unchecked
+    boundary pass, the one the batch has."""
+    import torch
+    from pydvg import scenes
+    from pydvg.batch import Batch
+    from pydvg.renderer import DeviceScene, PreparedScene
+
+    exact = True
+    r.set_exact_boundary(False)
+    try:
+        items = [scenes.headline(seed=k) for k in range(n)]
+        n_samples = sum(sc.width * sc.height * o.samples_x * o.samples_y for sc, o in items)
+        targets, singles = [], []
+        for sc, opts in items:
+            tds = DeviceScene(r, PreparedScene(scenes.jittered(sc, 2.0, seed=1), opts))
+            tgt = torch.empty(sc.width * sc.height * 4, dtype=torch.float32, device=dev)
+            tds.render(tgt)
+            targets.append(tgt)
+            ds = DeviceScene(r, PreparedScene(sc, opts))
+            bufs, sizes = ds.grad_buffers(torch)
+            singles.append((ds, tgt, torch.empty_like(tgt), torch.empty_like(tgt), bufs, sizes))
+        torch.cuda.synchronize()
+
+        def plain_step():
+            for ds, tgt, img, dr, bufs, sizes in singles:
+                ds.render(img)
+                torch.sub(img, tgt, out=dr)
+                dr.mul_(2.0)
+                ds.render_backward(dr, bufs, sizes)
+
+        def timed(fn, k):
+            for _ in range(max(args.warmup, 1)):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / k
+
+        k = max(3, min(args.steps, 10))
+        plain_s = timed(plain_step, k)
+        b = Batch(r, items, targets)
+        cfg = b.autotune(reps=1)
+        b.set_graph(False)
+        eager_s = timed(b.step, k)
+        b.set_graph(True)
+        graph_s = timed(b.step, k)
+        log(f"bench: batch of {n} headline scenes: plain loop {plain_s * 1e3:.2f} ms, batched "
+            f"{eager_s * 1e3:.2f} ms eager, {graph_s * 1e3:.2f} ms graph")
+        del b, singles
+        return {"scenes": n, "workload": "headline generator, seeds 0..%d" % (n - 1),
+                "samples_per_step": n_samples, "boundary_pass": "unchecked f32 (batch and plain loop)",
+                "plain_loop_ms_per_step": plain_s * 1e3, "batched_eager_ms_per_step": eager_s * 1e3,
+                "batched_graph_ms_per_step": graph_s * 1e3, "batched_samples_per_s": n_samples / graph_s,
+                "speedup_vs_plain_loop": plain_s / graph_s, "autotuned": cfg}
+    finally:
+        r.set_exact_boundary(exact and not args.unchecked_boundary)
 
 
 def bench_batch(args, r, stream, dev):

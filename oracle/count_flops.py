@@ -1,66 +1,75 @@
-"""Algorithmic FLOPs per pixel-sample of the headline workload.
+"""Algorithmic FLOPs per pixel-sample of the benchmark workloads -- TEST
+INFRASTRUCTURE (it runs the oracle; nothing in the product path loads it).
 
-Runs the oracle in count mode over evenly spaced slices of the sample index
-range and writes profiles/flops_headline.json.  Counting rules: +, -, *, /,
-compare, min/max and sqrt are 1 flop each, acos/cos/sin/atan2/pow are 1 SFU op
-each (reported apart and added to the flop total).  Scene evaluations count
-the reference's work with its own culling: groups of the sample's 16x16 tile
-list, every segment of an in-bounds path (the path BVH has one leaf for <= 8
-segments), no distance queries in non-prefiltered mode (they do not reach the
-output).  Fixed per-sample costs (gather of d_render over the 3x3 filter
-support, filter-radius adjoint, reverse compositing, boundary point sampling
-and gradient scatter) are counted from their formulas:
-  gather_d_color   9 taps x 12      = 108
-  filter adjoint   9 taps x 25      = 225
-  reverse adjoint  25 per fragment
-  boundary point   100; boundary gradient scatter 150; contribution 12
-Usage: python oracle/count_flops.py  (CPU; about a minute)
+Runs liboracle_count.so -- the oracle's sources compiled with the op-counting
+scalars of counted.h (Counted<float>, Counted<double>) -- over evenly spaced
+slices of each workload's pixel-sample index range, and writes
+profiles/flops_<workload>.json.  Rules (counted.h): +, -, *, /, comparisons,
+min / max and sqrt are 1 flop each; acos / cos / sin / atan2 / pow / cbrt /
+exp / log are 1 SFU op each, reported apart and added to the total.  The
+work counted is the reference algorithm's with the reference GPU's culling:
+a scene query counts only the groups of its 16x16 tile's list (the oracle
+itself loops over every group).  Phases: the forward (scene evaluation and
+splat per sample), the interior colour adjoint (scene evaluation with
+fragments, reverse compositing, paint and filter adjoints, d_render gather)
+and boundary sampling (edge point sampling, the two colour queries,
+contribution, gradient scatter), each per pixel-sample of the workload.
+
+Usage: python oracle/count_flops.py [workload ...]   (CPU; minutes)
 """
 import json
 import os
 import sys
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "diffvg-rs_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
+WORKLOADS = {"headline": "headline", "c1": "config1", "c2": "config2", "c3": "config3", "c5": "config5"}
 
-def main():
+
+def count(name, n_slices=16, per=2048):
     from pydvg import scenes
     from pydvg.scene import SceneDesc
     import oracle_py
-    sc, opts = scenes.headline(seed=0)
+    sc, opts = getattr(scenes, WORKLOADS[name])(seed=0)
     desc = SceneDesc(sc)
     total = sc.width * sc.height * opts.samples_x * opts.samples_y
-    n_slices, per = 16, 4096
-    acc = {}
+    d_render = np.random.default_rng(0).normal(size=sc.width * sc.height * 4).astype(np.float32)
+    acc = {"fwd": [0.0, 0.0], "interior": [0.0, 0.0], "boundary": [0.0, 0.0]}
+    n = 0
     for k in range(n_slices):
-        i0 = (total // n_slices) * k + 12345
-        c = oracle_py.count_ops(desc, opts, i0, i0 + per)
-        for key, v in c.items():
-            acc[key] = acc.get(key, 0) + v
-    n = acc["samples"]
-    fwd = (acc["fwd_flops"] + acc["fwd_sfu"]) / n + 4 * 3  # + splat (4 channels)
-    frags = acc["fragments"] / n
-    interior = (acc["fwd_flops"] + acc["fwd_sfu"]) / n + 25 * frags + 108 + 225
-    bnd_frac = acc["bnd_samples"] / n
-    boundary = (acc["bnd_flops"] + acc["bnd_sfu"]) / n + bnd_frac * (100 + 150 + 12 + 108)
-    out = {
-        "workload": "headline 1024x1024, 1000 closed cubic paths, 2x2 spp",
-        "samples_counted": n,
-        "fwd_flops_per_sample": fwd,
-        "fragments_per_sample": frags,
-        "bwd_interior_flops_per_sample": interior,
-        "bwd_boundary_flops_per_sample": boundary,
-        "bwd_flops_per_sample": interior + boundary,
-        "sfu_share_fwd": acc["fwd_sfu"] / max(1.0, acc["fwd_flops"] + acc["fwd_sfu"]),
-        "boundary_samples_contributing": bnd_frac,
-        "bwd_hbm_bytes": None,
+        i0 = (total // n_slices) * k + 12345 % max(total // n_slices, 1)
+        i1 = min(i0 + per, total)
+        c = oracle_py.count_slice(desc, opts, d_render, i0, i1)
+        for key in acc:
+            acc[key][0] += c[key][0]
+            acc[key][1] += c[key][1]
+        n += i1 - i0
+    per_sample = {k: (v[0] + v[1]) / n for k, v in acc.items()}
+    return {
+        "workload": f"{name}: {sc.width}x{sc.height}, {len(sc.shapes)} shapes, "
+                    f"{opts.samples_x}x{opts.samples_y} spp, prefiltered={bool(opts.use_prefiltering)}",
+        "method": "oracle compiled with op-counting scalars (oracle/counted.h), reference GPU tile culling",
+        "samples_counted": n, "slices": n_slices,
+        "fwd_flops_per_sample": per_sample["fwd"],
+        "bwd_interior_flops_per_sample": per_sample["interior"],
+        "bwd_boundary_flops_per_sample": per_sample["boundary"],
+        "bwd_flops_per_sample": per_sample["interior"] + per_sample["boundary"],
+        "sfu_per_sample": {k: v[1] / n for k, v in acc.items()},
     }
-    path = os.path.join(ROOT, "profiles", "flops_headline.json")
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
-    print(json.dumps(out, indent=1))
+
+
+def main():
+    names = sys.argv[1:] or list(WORKLOADS)
+    for name in names:
+        out = count(name)
+        path = os.path.join(ROOT, "profiles", f"flops_{name}.json")
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -233,36 +233,8 @@ double oracle_time_slice(const dvg_scene_desc *desc, const dvg_render_options *o
     backward_samples(cx, *opt, false, d_render, wimg, gr, i0, i1);
     cpu::boundary_sampling(cx, nsx, nsy, opt->seed, d_render, wimg, gr, false, i0, i1);
     auto t2 = std::chrono::steady_clock::now();
-    if (weight_seconds) *weight_seconds = std::chrono::duration<double>(t1 - t0).count();
-    return std::chrono::duration<double>(t2 - t1).count();
-}
-
-// Algorithmic operation counts over samples [i0, i1) (out[0..6]): forward
-// scene-evaluation flops and sfu ops, fragments, boundary-query flops and sfu
-// ops, contributing boundary samples.  The scene evaluations count the
-// reference's work with its tile culling (see oracle_forward.cpp).
-void oracle_count_ops(const dvg_scene_desc *desc, const dvg_render_options *opt, uint64_t i0, uint64_t i1,
-                      double *out) {
-    Scene sc = read_scene(desc);
-    uint32_t nsx = std::max(opt->samples_x, 1u), nsy = std::max(opt->samples_y, 1u);
-    size_t W = sc.width;
-    std::vector<float> pts;
-    bool jitter = opt->jitter && !opt->use_prefiltering;
-    for (uint64_t idx = i0; idx < i1; ++idx) {
-        uint32_t sx = idx % nsx, sy = (idx / nsx) % nsy, x = (idx / (nsx * nsy)) % W, y = idx / ((uint64_t)nsx * nsy * W);
-        float rx, ry;
-        gpu::jitter_xy(x, y, sx, sy, (uint32_t)W, nsx, nsy, opt->seed, jitter, rx, ry);
-        pts.push_back((float)x + ((float)sx + rx) * (1.0f / (float)nsx));
-        pts.push_back((float)y + ((float)sy + ry) * (1.0f / (float)nsy));
-    }
-    OpCount f = count_eval_points(sc, *opt, pts.data(), pts.size() / 2);
-    Ctx cx;
-    make_ctx(sc, cx);
-    std::vector<float> bp;
-    cpu::boundary_query_points(cx, nsx, nsy, opt->seed, i0, i1, bp);
-    OpCount b = count_eval_points(sc, *opt, bp.data(), bp.size() / 2);
-    out[0] = (double)f.flops; out[1] = (double)f.sfu; out[2] = (double)f.frags;
-    out[3] = (double)b.flops; out[4] = (double)b.sfu; out[5] = (double)(bp.size() / 4);
+    if (weight_seconds) *weight_seconds = std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count() * 1e-9;
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count() * 1e-9;
 }
 
 // The scalar whose gradient render_backward returns for a prefiltered scene:

@@ -914,8 +914,10 @@ V4 sample_color(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 *d_color, Ed
     if (eq) eq->hit = false;
     V2 pt(screen_pt.x * (float)sc.width, screen_pt.y * (float)sc.height);
     std::vector<Fragment> frags;
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
     for (uint32_t gi = 0; gi < sc.groups.size(); ++gi) {
         const Group &g = sc.groups[gi];
+        count_gate(cx, counting, pt, gi);
         if (g.stroke.present() && within_edge(cx, gi, pt, eq)) {
             V4 c = paint_color(g.stroke, pt);
             frags.push_back({c, gi, true});
@@ -925,6 +927,7 @@ V4 sample_color(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 *d_color, Ed
             frags.push_back({c, gi, false});
         }
     }
+    COUNT_ON(counting);
     if (frags.empty()) {
         if (bg) {
             if (d_color && gr) accumulate_background_grad(sc, *gr, pix, *d_color);
@@ -1460,10 +1463,15 @@ void boundary_sampling(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t seed,
                        size_t i1, long row_lo, long row_hi) {  // boundary.rs:113-197, over sample indices [i0, i1)
     const Scene &sc = *cx.sc;
     if (sc.groups.empty()) return;
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
+    COUNT_ON(false);  // the sampling tables are per step, not per sample
     std::vector<float> L = compute_shape_lengths(cx);
     ShapeCdf cdf;
-    if (!build_shape_cdf(sc, L, cdf)) return;
-    std::vector<PathCdf> pcs = build_path_cdfs(cx, L);
+    bool have_cdf = build_shape_cdf(sc, L, cdf);
+    std::vector<PathCdf> pcs;
+    if (have_cdf) pcs = build_path_cdfs(cx, L);
+    COUNT_ON(counting);
+    if (!have_cdf) return;
     size_t total = (size_t)sc.width * sc.height * nsx * nsy;
     if (i1 > total) i1 = total;
     int W = (int)sc.width, H = (int)sc.height;
@@ -1536,10 +1544,15 @@ void boundary_query_points(const Ctx &cx, uint32_t nsx, uint32_t nsy, uint32_t s
     const Scene &sc = *cx.sc;
     pts.clear();
     if (sc.groups.empty()) return;
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
+    COUNT_ON(false);  // the sampling tables are per step, not per sample
     std::vector<float> L = compute_shape_lengths(cx);
     ShapeCdf cdf;
-    if (!build_shape_cdf(sc, L, cdf)) return;
-    std::vector<PathCdf> pcs = build_path_cdfs(cx, L);
+    bool have_cdf = build_shape_cdf(sc, L, cdf);
+    std::vector<PathCdf> pcs;
+    if (have_cdf) pcs = build_path_cdfs(cx, L);
+    COUNT_ON(counting);
+    if (!have_cdf) return;
     size_t total = (size_t)sc.width * sc.height * nsx * nsy;
     if (i1 > total) i1 = total;
     for (size_t idx = i0; idx < i1; ++idx) {

@@ -26,7 +26,19 @@ struct Ctx {
     std::vector<std::vector<Seg>> segs;  // per scene shape, default radius = stroke_width
     std::vector<float> group_bounds;     // 4 per group: union of CPU shape bounds
     std::vector<uint8_t> group_has_bounds;
+    // operation counting only: 16x16 tile range of every group (4 ints a
+    // group, the reference's tile lists); empty = count every group
+    std::vector<int> count_tiles;
 };
+
+// Operation counting: counts only while group gi is in the tile list of the
+// canvas point's 16x16 tile (no-op in the checking build).
+inline void count_gate(const Ctx &cx, bool counting, V2 pt, uint32_t gi) {
+    if (!counting || cx.count_tiles.empty()) return;
+    [[maybe_unused]] int tx = (int)std::floor(pt.x / 16.f), ty = (int)std::floor(pt.y / 16.f);
+    [[maybe_unused]] const int *t = &cx.count_tiles[4 * (size_t)gi];
+    COUNT_ON(tx >= t[0] && tx <= t[2] && ty >= t[1] && ty <= t[3]);
+}
 
 namespace cpu {
 bool solve_quadratic(double a, double b, double c, double t[2]);

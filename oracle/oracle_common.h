@@ -29,6 +29,19 @@
 
 #include "../include/dvg.h"
 
+// Operation counting (the liboracle_count.so build, counted.h): COUNT_ON
+// switches the tally on or off, COUNT_PHASE picks its bucket.  No-ops in the
+// checking build.
+#ifdef ORACLE_COUNT
+#define COUNT_ON(b) (cnt::on = (b))
+#define COUNT_IS_ON() (cnt::on)
+#define COUNT_PHASE(p) (cnt::phase = (p))
+#else
+#define COUNT_ON(b) ((void)0)
+#define COUNT_IS_ON() (false)
+#define COUNT_PHASE(p) ((void)0)
+#endif
+
 namespace oracle {
 
 static const float PI_F = 3.14159265358979323846f;  // core::f32::consts::PI
@@ -247,10 +260,6 @@ static inline bool path_point_radius(const Path &p, size_t index, float default_
     else r = default_radius;
     return true;
 }
-
-struct OpCount {
-    uint64_t flops = 0, sfu = 0, frags = 0;
-};
 
 // One path segment with control points and per-point radii
 // (src/renderer/path.rs:100-173, src/distance/bvh.rs:262-380).

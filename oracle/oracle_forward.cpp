@@ -12,17 +12,7 @@
 
 namespace oracle {
 
-// Algorithmic operation tally (count mode only): +, -, *, /, compare,
-// min/max and sqrt count 1 flop; acos/cos/sin/atan2/pow count 1 sfu.
-OpCount g_ops;
-bool g_counting = false;
-#define OPS(f, s_)                    \
-    do {                              \
-        if (g_counting) {             \
-            g_ops.flops += (f);       \
-            g_ops.sfu += (s_);        \
-        }                             \
-    } while (0)
+float g_forward_sink = 0.f;
 
 namespace gpu {
 
@@ -51,7 +41,6 @@ static inline float dist_pt(float px, float py, float ax, float ay) {  // curves
 
 void distance_to_segment_with_t(float px, float py, float ax, float ay, float bx, float by, float &dist,
                                 float &t) {  // curves.rs:39-66
-    OPS(18, 0);
     float vx = bx - ax, vy = by - ay, wx = px - ax, wy = py - ay;
     float c1 = vx * wx + vy * wy;
     float c2 = vx * vx + vy * vy;
@@ -84,9 +73,7 @@ float distance_to_segment(float px, float py, float ax, float ay, float bx, floa
 
 int solve_quadratic(float a, float b, float c, float out[2]) {  // curves.rs:865-893
     float disc = b * b - 4.0f * a * c;
-    OPS(5, 0);
     if (disc < 0.f) { out[0] = out[1] = 0.f; return 0; }
-    OPS(8, 0);
     float root = sqrt_f(disc);
     float q = (b < 0.f) ? -0.5f * (b - root) : -0.5f * (b + root);
     float t0 = q / a, t1 = c / q;
@@ -105,7 +92,6 @@ static float cbrt_gpu(float x) {  // curves.rs:949-959
 
 int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.rs:897-946
     const float eps = 1.0e-6f;
-    OPS(2, 0);
     if (abs_f(a) < eps) {
         float r[2];
         int n = solve_quadratic(b, c, d, r);
@@ -117,9 +103,7 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
     float q = (bb * bb - 3.0f * cc) / 9.0f;
     float r = (2.0f * bb * bb * bb - 9.0f * bb * cc + 27.0f * dd) / 54.0f;
     float r2 = r * r, q3 = q * q * q;
-    OPS(21, 0);
     if (r2 < q3) {
-        OPS(22, 4);
         float theta = acos_f(r / sqrt_f(q3));
         float sqrt_q = sqrt_f(q);
         const float two_pi = 2.0f * 3.14159265f;
@@ -128,7 +112,6 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
         out[2] = -2.0f * sqrt_q * cos_f((theta - two_pi) * third) - bb * third;
         return 3;
     }
-    OPS(11, 1);
     float a_root = (r > 0.f) ? -cbrt_gpu(r + sqrt_f(r2 - q3)) : cbrt_gpu(-r + sqrt_f(r2 - q3));
     float b_root = (abs_f(a_root) > eps) ? q / a_root : 0.f;
     out[0] = (a_root + b_root) - bb * third;
@@ -139,9 +122,7 @@ int solve_cubic(float a, float b, float c, float d, float out[3]) {  // curves.r
 void winding_line(float px, float py, float x0, float y0, float x1, float y1, uint32_t rule, float &w,
                   float &cr) {  // curves.rs:97-123
     bool a = y0 <= py, b = y1 <= py;
-    OPS(3, 0);
     if (a != b) {
-        OPS(7, 0);
         float t = (py - y0) / (y1 - y0);
         float xi = x0 + t * (x1 - x0);
         if (xi > px) {
@@ -156,14 +137,11 @@ void winding_quadratic(float px, float py, float x0, float y0, float x1, float y
     float ay = y0 - 2.0f * y1 + y2;
     float by = -2.0f * y0 + 2.0f * y1;
     float cy = y0 - py;
-    OPS(8, 0);
     float roots[2];
     int n = solve_quadratic(ay, by, cy, roots);
     for (int i = 0; i < n; ++i) {
         float t = roots[i];
-        OPS(2, 0);
         if (t >= 0.f && t <= 1.f) {
-            OPS(15, 0);
             float tt = 1.f - t;
             float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
             if (x > px) {
@@ -183,14 +161,11 @@ void winding_cubic(float px, float py, float x0, float y0, float x1, float y1, f
     float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
     float c = -3.0f * y0 + 3.0f * y1;
     float d = y0 - py;
-    OPS(15, 0);
     float roots[3];
     int n = solve_cubic(a, b, c, d, roots);
     for (int i = 0; i < n; ++i) {
         float t = roots[i];
-        OPS(2, 0);
         if (t >= 0.f && t <= 1.f) {
-            OPS(25, 0);
             float tt = 1.f - t;
             float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
             if (x > px) {
@@ -241,7 +216,6 @@ void closest_quadratic(float px, float py, float x0, float y0, float x1, float y
                        bool approx, float &dist, float &tout) {  // curves.rs:218-279
     float best_t = 0.f;
     float best = dist_pt(px, py, x0, y0);
-    OPS(10, 0);
     if (approx) {
         float ax, ay, at;
         quadratic_closest_pt_approx(px, py, x0, y0, x1, y1, x2, y2, ax, ay, at);
@@ -257,14 +231,11 @@ void closest_quadratic(float px, float py, float x0, float y0, float x1, float y
         float b = 3.0f * (ax * bx + ay * by);
         float c = 2.0f * (bx * bx + by * by) + (ax * cx + ay * cy);
         float d = bx * cx + by * cy;
-        OPS(28, 0);
         float roots[3];
         int n = solve_cubic(a, b, c, d, roots);
         for (int i = 0; i < n; ++i) {
             float t = roots[i];
-            OPS(2, 0);
             if (t >= 0.f && t <= 1.f) {
-                OPS(22, 0);
                 float tt = 1.f - t;
                 float x = (tt * tt) * x0 + (2.0f * tt * t) * x1 + (t * t) * x2;
                 float y = (tt * tt) * y0 + (2.0f * tt * t) * y1 + (t * t) * y2;
@@ -317,7 +288,6 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
     float best_t = 0.f;
     float best = dist_pt(px, py, x0, y0);
     float de = dist_pt(px, py, x3, y3);
-    OPS(30, 0);
     if (de < best) { best = de; best_t = 1.f; }
     float ax = -x0 + 3.0f * x1 - 3.0f * x2 + x3;
     float ay = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
@@ -339,7 +309,6 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
         float p1c = (4.0f / 5.0f) * e - (2.0f / 25.0f) * b * d;
         float p1d = f - b * e / 25.0f;
         float q_root = -b / 5.0f;
-        OPS(58, 0);
         float p_roots[3];
         int num_p = solve_cubic(p1a, p1b, p1c, p1d, p_roots);
         float iv[4];
@@ -361,7 +330,6 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
             if (idx < ni && iv[idx] < 0.f) { ++idx; continue; }
             float upper = (idx < ni) ? min_f(iv[idx], 1.f) : 1.f;
             float lb = lower, ub = upper;
-            OPS(24, 0);
             float lbe = eval_quintic(lb, b, c, d, e, f);
             float ube = eval_quintic(ub, b, c, d, e, f);
             if (lbe * ube > 0.f) {
@@ -375,7 +343,6 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
             }
             float t = (lb + ub) * 0.5f;
             for (uint32_t it = 0; it < 20; ++it) {
-                OPS(30, 0);
                 if (t < lb || t > ub) t = (lb + ub) * 0.5f;
                 float value = eval_quintic(t, b, c, d, e, f);
                 if (abs_f(value) < 1.0e-5f || it == 19) break;
@@ -383,9 +350,7 @@ void closest_cubic(float px, float py, float x0, float y0, float x1, float y1, f
                 float der = eval_quintic_deriv(t, b, c, d, e);
                 t = t - value / der;
             }
-            OPS(2, 0);
             if (t >= 0.f && t <= 1.f) {
-                OPS(34, 0);
                 float tt = 1.f - t;
                 float x = (tt * tt * tt) * x0 + (3.0f * tt * tt * t) * x1 + (3.0f * tt * t * t) * x2 + (t * t * t) * x3;
                 float y = (tt * tt * tt) * y0 + (3.0f * tt * tt * t) * y1 + (3.0f * tt * t * t) * y2 + (t * t * t) * y3;
@@ -787,8 +752,6 @@ static void blend_group(float out[4], const Paint &fill, const Paint &stroke, ui
             cov = 1.f;
         }
         if (cov > 0.f) {
-            OPS(12, 0);
-            if (g_counting) ++g_ops.frags;
             float a = cov * scol.w, inv = 1.f - a;
             out[0] = scol.x * a + out[0] * inv;
             out[1] = scol.y * a + out[1] * inv;
@@ -808,8 +771,6 @@ static void blend_group(float out[4], const Paint &fill, const Paint &stroke, ui
             cov = 1.f;
         }
         if (cov > 0.f) {
-            OPS(12, 0);
-            if (g_counting) ++g_ops.frags;
             float a = cov * fc.w, inv = 1.f - a;
             out[0] = fc.x * a + out[0] * inv;
             out[1] = fc.y * a + out[1] * inv;
@@ -827,14 +788,14 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
     out[1] = bg[1] * bg[3];
     out[2] = bg[2] * bg[3];
     out[3] = bg[3];
-    bool counting = g_counting;
-    int tx = (int)std::floor(px / 16.f), ty = (int)std::floor(py / 16.f);
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
+    [[maybe_unused]] int tx = (int)std::floor(px / 16.f), ty = (int)std::floor(py / 16.f);
     for (size_t gi = 0; gi < sc.groups.size(); ++gi) {
         const Group &g = sc.groups[gi];
         const PreparedGroup &pg = pgs[gi];
-        // count only the groups the reference's tile list holds for this tile
-        g_counting = counting && tx >= pg.tile[0] && tx <= pg.tile[2] && ty >= pg.tile[1] && ty <= pg.tile[3];
-        OPS(14, 0);
+        // counting: only the groups of this tile's list (the reference's
+        // culling; the loop over every group is the restatement's)
+        COUNT_ON(counting && tx >= pg.tile[0] && tx <= pg.tile[2] && ty >= pg.tile[1] && ty <= pg.tile[3]);
         const float *x = pg.xform;
         float lx = x[0] * px + x[1] * py + x[2];
         float ly = x[3] * px + x[4] * py + x[5];
@@ -845,11 +806,9 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
         if (g.fill.kind != DVG_PAINT_NONE || g.stroke.kind != DVG_PAINT_NONE) {
             for (const PreparedInstance &pi : pg.shapes) {
                 // accumulate_shape_in_group, forward.rs:1420-1503
-                OPS(6, 0);
                 bool inb = pi.b[0] <= pi.b[2] && pi.b[1] <= pi.b[3];
                 if (inb && (lx < pi.b[0] || lx > pi.b[2] || ly < pi.b[1] || ly > pi.b[3])) inb = false;
                 if (!inb) continue;
-                OPS(8, 0);
                 const float *iv = pi.inv;
                 float spx = iv[0] * lx + iv[1] * ly + iv[2];
                 float spy = iv[3] * lx + iv[4] * ly + iv[5];
@@ -858,31 +817,27 @@ static void eval_scene(const Scene &sc, const std::vector<PreparedGroup> &pgs, b
                 if (g.stroke.kind != DVG_PAINT_NONE) shape_stroke(sh, pi, prefilter, spx, spy, acc);
             }
         }
-        OPS(10, 0);
         blend_group(out, g.fill, g.stroke, g.fill_rule, fc, scol, acc.fill_min * gscale, acc.winding, acc.crossings,
                     acc.stroke_min * gscale, acc.stroke_radius * gscale, acc.stroke_hit, prefilter);
     }
-    g_counting = counting;
+    COUNT_ON(counting);
 }
 
-// Count mode: algorithmic operations of the scene evaluation at the points
-// pts (x0, y0, x1, y1, ...), with the reference's tile culling.
-OpCount count_eval_points(const Scene &sc, const dvg_render_options &opt, const float *pts, size_t n) {
+// The 16x16 tile range of every group's canvas box (4 ints a group,
+// min > max when empty): the groups of each tile's list (binning.rs:30-110).
+std::vector<int> group_tile_ranges(const Scene &sc, const dvg_render_options &opt) {
     std::vector<PreparedGroup> pgs;
     prepare(sc, opt, pgs);
-    bool prefilter = opt.use_prefiltering != 0;
-    float bg[4] = {sc.background.x, sc.background.y, sc.background.z, sc.background.w};
-    g_ops = OpCount();
-    g_counting = true;
-    float col[4];
-    for (size_t i = 0; i < n; ++i) eval_scene(sc, pgs, prefilter, pts[2 * i], pts[2 * i + 1], bg, col);
-    g_counting = false;
-    return g_ops;
+    std::vector<int> out;
+    for (const PreparedGroup &pg : pgs) out.insert(out.end(), pg.tile, pg.tile + 4);
+    return out;
 }
 
 // Scene evaluation (no splat) of the samples [i0, i1), for CPU timing.
 void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, size_t i1) {
     std::vector<PreparedGroup> pgs;
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
+    COUNT_ON(false);  // scene preparation is per step, not per sample
     prepare(sc, opt, pgs);
     uint32_t W = sc.width, H = sc.height;
     uint32_t nsx = std::max(opt.samples_x, 1u), nsy = std::max(opt.samples_y, 1u);
@@ -891,7 +846,8 @@ void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, 
     bool prefilter = opt.use_prefiltering != 0;
     bool jitter = !prefilter && opt.jitter != 0;
     float inv_sx = 1.0f / (float)nsx, inv_sy = 1.0f / (float)nsy;
-    volatile float sink = 0.f;
+    float sink = 0.f;
+    COUNT_ON(counting);
     for (size_t idx = i0; idx < i1; ++idx) {
         uint32_t sx = idx % nsx, sy = (idx / nsx) % nsy, x = (idx / (nsx * nsy)) % W, y = idx / ((size_t)nsx * nsy * W);
         float rx, ry;
@@ -902,6 +858,7 @@ void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, 
         eval_scene(sc, pgs, prefilter, px, py, bg, col);
         sink = sink + col[0];
     }
+    g_forward_sink = sink;  // keeps the evaluations observable
 }
 
 int render_forward(const Scene &sc, const dvg_render_options &opt, float *out_img, float *sample_colors,
@@ -1039,7 +996,7 @@ static void closest_point_ellipse(float cx, float cy, float rx_in, float ry_in, 
         float g = rx * dx * s - ry * dy * c + (ry * ry - rx * rx) * s * c;
         float gt = rx * dx * c + ry * dy * s + (ry * ry - rx * rx) * (c * c - s * s);
         if (abs_f(gt) < 1.0e-12f) break;
-        float next = std::min(std::max(t - g / gt, 0.0f), 1.57079633f);
+        float next = std::min<float>(std::max<float>(t - g / gt, 0.0f), 1.57079633f);
         if (abs_f(next - t) < 1.0e-6f) { t = next; break; }
         t = next;
     }

@@ -26,7 +26,9 @@ void jitter_xy(uint32_t x, uint32_t y, uint32_t sx, uint32_t sy, uint32_t width,
                uint32_t seed, bool jitter, float &rx, float &ry);
 }  // namespace gpu
 
-OpCount count_eval_points(const Scene &sc, const dvg_render_options &opt, const float *pts, size_t n);
+// The 16x16 tile range of every group (4 ints a group; the reference's tile
+// lists), for operation counting.
+std::vector<int> group_tile_ranges(const Scene &sc, const dvg_render_options &opt);
 void forward_samples(const Scene &sc, const dvg_render_options &opt, size_t i0, size_t i1);
 // Forward render; sample_colors (optional, whole image only) receives the
 // premultiplied per-sample colours in (pixel, sy, sx) order.  Pixel rows

@@ -481,8 +481,10 @@ V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 
     const Scene &sc = *cx.sc;
     V2 pt(screen_pt.x * (float)sc.width, screen_pt.y * (float)sc.height);
     std::vector<PFrag> frags;
+    [[maybe_unused]] bool counting = COUNT_IS_ON();
     for (uint32_t gi = 0; gi < sc.groups.size(); ++gi) {
         const Group &g = sc.groups[gi];
+        count_gate(cx, counting, pt, gi);
         if (g.stroke.present()) {
             Hit h;
             if (compute_distance(cx, gi, pt, INFINITY, h)) {
@@ -517,6 +519,7 @@ V4 sample_color_prefiltered(const Ctx &cx, V2 screen_pt, const V4 *bg, const V4 
             }
         }
     }
+    COUNT_ON(counting);
     if (frags.empty()) {
         if (bg) {
             if (d_color && gr) accumulate_background_grad(sc, *gr, pix, *d_color);

@@ -262,13 +262,34 @@ def time_slice(desc, opts, d_render, seconds=15.0):
                       f"({ws.value:.2f} s for all samples) is excluded"}
 
 
-def count_ops(desc, opts, i0, i1):
-    """Algorithmic op counts over samples [i0, i1) (see oracle_count_ops)."""
-    L = lib()
-    from pydvg import abi as _abi
-    L.oracle_count_ops.argtypes = [C.POINTER(_abi.SceneDescC), C.POINTER(_abi.RenderOptions), C.c_uint64,
-                                   C.c_uint64, C.POINTER(C.c_double)]
+COUNT_LIB = os.path.join(_HERE, "liboracle_count.so")
+_count_lib = None
+
+
+def count_lib():
+    """liboracle_count.so: the oracle built with op-counting scalars
+    (counted.h); built on first use."""
+    global _count_lib
+    if _count_lib is None:
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle_count.so"])
+        from pydvg import abi as _abi
+        L = C.CDLL(COUNT_LIB)
+        L.oracle_count_slice.argtypes = [C.POINTER(_abi.SceneDescC), C.POINTER(_abi.RenderOptions),
+                                         C.POINTER(C.c_float), C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]
+        L.oracle_render.argtypes = [C.POINTER(_abi.SceneDescC), C.POINTER(_abi.RenderOptions),
+                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.oracle_render.restype = C.c_int
+        _count_lib = L
+    return _count_lib
+
+
+def count_slice(desc, opts, d_render, i0, i1):
+    """Arithmetic the oracle evaluates for the pixel-samples [i0, i1): forward,
+    interior adjoint and boundary sampling, each as (flops, sfu ops), with the
+    reference's tile culling (oracle_count_slice)."""
+    L = count_lib()
     out = (C.c_double * 6)()
-    L.oracle_count_ops(desc.ref(), C.byref(opts.to_c()), i0, i1, out)
-    return {"fwd_flops": out[0], "fwd_sfu": out[1], "fragments": out[2], "bnd_flops": out[3], "bnd_sfu": out[4],
-            "bnd_samples": out[5], "samples": i1 - i0}
+    dr = np.ascontiguousarray(d_render, np.float32)
+    L.oracle_count_slice(desc.ref(), C.byref(opts.to_c()), _fp(dr), i0, i1, out)
+    return {"fwd": (out[0], out[1]), "interior": (out[2], out[3]), "boundary": (out[4], out[5]),
+            "samples": i1 - i0}
