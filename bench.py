@@ -207,15 +207,24 @@ def main():
         if os.path.exists(tr_path):
             with open(tr_path) as f:
                 traffic = json.load(f).get("traffic_bytes", {}).get(dom)
+        valu = None
+        va_path = os.path.join(ROOT, "profiles", f"valu_{args.workload}.json")
+        if os.path.exists(va_path):
+            with open(va_path) as f:
+                valu = json.load(f).get("kernels", {}).get(dom)
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                    "kernel": "k_bnd_eval" if dom == "boundary_eval" else dom, "kernel_ms": ms,
+                    "kernel": {"boundary_eval": "k_bnd_eval_chk" if not args.unchecked_boundary else "k_bnd_eval",
+                               "backward_interior": "k_backward_prefiltered" if opts.use_prefiltering
+                               else "k_backward_interior", "forward": "k_forward"}[dom], "kernel_ms": ms,
+                    "counters": valu,
                     "per_kernel_tflops": {k: round(alg[k] / (per_kernel_ms[k] * 1e-3) / 1e12, 3)
                                           for k in alg if per_kernel_ms.get(k)},
                     "note": "VALU-bound fp32 scalar geometry (no MFMA, not HBM); peak is MI355X's published "
                             "fp32 vector 157.3 TFLOP/s, which counts packed fp32 (this build compiles without "
                             "packed fp32 ops: its own ceiling is half that); algorithmic flops per sample counted "
-                            f"by the oracle's op-counting scalar type, profiles/flops_{args.workload}.json"}
+                            f"by the oracle's op-counting scalar type, profiles/flops_{args.workload}.json; "
+                            f"traffic and counters from profiles/{args.workload}_hbm_traffic.md and _valu.md"}
 
     batch = None
     if headline and args.batch_scenes > 0:
