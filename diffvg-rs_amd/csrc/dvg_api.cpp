@@ -436,6 +436,26 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
             if (pk->group_bvh_indices[(size_t)m[2] + k] >= SP)
                 return set_err(ctx, DVG_INVALID_SCENE, "group BVH index out of range");
     }
+    // prefiltered cut-offs: largest stroke radius and stroke width per group
+    std::vector<float> group_pf((size_t)std::max(G, 1u) * 2, 0.f);
+    for (uint32_t g = 0; g < G; ++g) {
+        const float *gd = pk->group_data + 16 * (size_t)g;
+        uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+        float rmax = 0.f, swmax = 0.f;
+        for (uint32_t i = 0; i < cnt && (uint64_t)off + i < pk->group_shapes_len; ++i) {
+            uint32_t prep = (uint32_t)pk->group_shapes[off + i];
+            if (prep >= SP) continue;
+            const float *sd = pk->shape_data + 15 * (size_t)prep;
+            float sw = sd[3];
+            swmax = std::max(swmax, sw);
+            rmax = std::max(rmax, std::fabs(sw));
+            uint32_t coff = (uint32_t)sd[12], ccnt = (uint32_t)sd[13];
+            for (uint32_t k = 0; k < ccnt && (uint64_t)coff + k < ncurves; ++k)
+                for (int j = 9; j < 13; ++j) rmax = std::max(rmax, std::fabs(pk->curve_data[13 * ((size_t)coff + k) + j]));
+        }
+        group_pf[2 * (size_t)g] = rmax;
+        group_pf[2 * (size_t)g + 1] = swmax;
+    }
     // tiled signed distance: no group's shape-to-canvas map shrinks
     uint32_t sdf_tiled = 1;
     for (uint32_t g = 0; g < G; ++g) {
@@ -508,6 +528,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         i_s2p = addv(s2p.data(), (size_t)std::max(S, 1u) * 4);
     }
     size_t i_gcb = addv(cpu_bounds.data(), cpu_bounds.size() * 4);
+    size_t i_gpf = addv(group_pf.data(), group_pf.size() * 4);
     size_t i_gbm = addv(gbvh_meta.data(), gbvh_meta.size() * 4);
     size_t i_gbb = addv(pk->group_bvh_bounds, pk->group_bvh_bounds ? pk->group_bvh_bounds_len * 4 : 0);
     size_t i_gbn = addv(pk->group_bvh_nodes, pk->group_bvh_nodes ? pk->group_bvh_nodes_len * 4 : 0);
@@ -617,6 +638,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         D.scene_to_prepared = (dvgk::kuint *)at(i_s2p);
     }
     D.group_cpu_bounds = (dvgk::kfloat *)at(i_gcb);
+    D.group_pf = (dvgk::kfloat *)at(i_gpf);
     D.gbvh_meta = (dvgk::kuint *)at(i_gbm);
     D.gbvh_bounds = (dvgk::kfloat *)at(i_gbb);
     D.gbvh_nodes = (dvgk::kuint *)at(i_gbn);

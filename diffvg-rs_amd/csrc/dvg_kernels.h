@@ -69,6 +69,10 @@ struct DevScene {
     // 4 per group, group-local: the root box of the reference's CPU group BVH
     // (union of distance/shape.rs shape_bounds); min > max when it has none
     kfloat *group_cpu_bounds;
+    // 2 per group: the largest stroke radius of its shapes (|stroke width|
+    // and |thickness| over their curves) and the largest stroke width, the
+    // distance cut-offs of the prefiltered forward and adjoint
+    kfloat *group_pf;
     // the pack's group BVHs (renderer/bvh.rs): node boxes (4 floats), nodes
     // [left, skip, start, count], leaf shape indices, meta [node offset, node
     // count, index offset, index count] per group (all zero: no BVH)

@@ -320,9 +320,14 @@ DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, f
 // winding follows the reference CPU's winding_number_segment
 // (distance/winding.rs:118-192): the closed-interval line rule and the f64
 // cubic solve on ill-conditioned segments.
+//
+// `cut` (prefiltered distances): distances at or beyond it give the same
+// coverage as any larger one, so the running minimum starts there and
+// segments farther away are not measured (the minimum comes out as `cut`
+// when the shape is that far).
 template <int M>
 DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool prefilter, float px, float py,
-                        Cov &c) {
+                        Cov &c, float cut = BIG) {
     constexpr bool CPU = M != COV_FWD;
     uint32_t kind = (uint32_t)sd[0];
     float q0 = sd[4], q1 = sd[5], q2 = sd[6], q3 = sd[7];
@@ -349,7 +354,7 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
     }
     uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
     bool approx = sd[14] > 0.5f;
-    float lmin = BIG, lw = 0.f, lc = 0.f;
+    float lmin = cut, lw = 0.f, lc = 0.f;
     for (uint32_t s = 0; s < cnt; ++s) {
         const float *cv = S.curve_data + (off + s) * 13u;
         const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
@@ -415,8 +420,10 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
 // uses the exact closest point of src/distance/shape.rs:117-164 and a shape
 // needs stroke_width > 0, as the CPU query within_distance_bvh does
 // (distance/api.rs:79-135).
+// `cut`: as for shape_fill (a minimum left at `cut` keeps radius 0).
 template <int M>
-DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, float px, float py, Cov &c) {
+DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, float px, float py, Cov &c,
+                          float cut = BIG) {
     constexpr bool CPU = M != COV_FWD;
     uint32_t kind = (uint32_t)sd[0];
     float sw = sd[3];
@@ -456,7 +463,7 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
         return;
     }
     bool approx = sd[14] > 0.5f;
-    float lmin = BIG, lr = 0.f;
+    float lmin = cut, lr = 0.f;
     for (uint32_t s = 0; s < cnt; ++s) {
         const float *cv = S.curve_data + (off + s) * 13u;
         uint32_t k = (uint32_t)cv[0];
@@ -503,9 +510,13 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
 // With CPU = true the fill follows is_inside_bvh (distance/api.rs:259-309):
 // the group's root box gates the test and then every shape's winding counts
 // (an open path can wind outside its own box).
+//
+// fill_cut / stroke_cut: the prefiltered distance cut-offs of shape_fill /
+// shape_stroke (BIG: none).
 template <int M>
 DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx, float ly, bool do_fill,
-                       bool do_stroke, uint32_t rule, bool prefilter, Cov &c) {
+                       bool do_stroke, uint32_t rule, bool prefilter, Cov &c, float fill_cut = BIG,
+                       float stroke_cut = BIG) {
     constexpr bool CPU = M != COV_FWD;
     uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
     if (CPU && do_fill) {
@@ -532,8 +543,8 @@ DVG_DEV void group_cov(const DevScene &S, uint32_t g, const float *gd, float lx,
         float spx = xf[0] * lx + xf[1] * ly + xf[2];
         float spy = xf[3] * lx + xf[4] * ly + xf[5];
         const float *sd = S.shape_data + si * 15u;
-        if (do_fill) shape_fill<COV_FWD>(S, sd, rule, prefilter, spx, spy, c);
-        if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<M>(S, sd, prefilter, spx, spy, c);
+        if (do_fill) shape_fill<COV_FWD>(S, sd, rule, prefilter, spx, spy, c, fill_cut);
+        if (do_stroke && !(c.stroke_hit > 0.f && !prefilter)) shape_stroke<M>(S, sd, prefilter, spx, spy, c, stroke_cut);
     }
 }
 
@@ -566,9 +577,20 @@ DVG_DEV float4 eval_forward(const DevScene &S, const uint32_t *list, uint32_t n,
         float lx, ly;
         group_local(S, g, px, py, lx, ly);
         Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
-        group_cov<COV_FWD>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c);
         float inv_scale = S.group_inv_scale[g];
         float gscale = inv_scale > 0.f ? 1.f / inv_scale : 1.f;
+        // prefiltered coverage saturates: smoothstep((d + 1) / 2) of the
+        // scaled fill distance is 0 or 1 from |d| >= 1, the stroke's from
+        // d >= r + 1 (r <= the group's largest radius); 2 / gscale keeps the
+        // cut-offs clear of that in f32
+        float fcut = BIG, scut = BIG;
+        if (prefilter) {
+            float unit = 2.0f / gscale;
+            fcut = unit;
+            scut = S.group_pf[2u * g] + unit;
+        }
+        group_cov<COV_FWD>(S, g, gd, lx, ly, fk != DVG_PAINT_NONE, sk != DVG_PAINT_NONE, rule, prefilter, c, fcut,
+                           scut);
         // blend_group, forward.rs:339-404: stroke first, then fill
         if (sk != DVG_PAINT_NONE) {
             float cov = 0.f;
@@ -1597,8 +1619,11 @@ DVG_DEV bool group_inside(const DevScene &S, uint32_t g, float px, float py) {
 }
 
 // Stroke fragment of group g: (signed) distance and coverage.
+// Only a boundary nearer than the group's largest stroke width + 1 can give
+// w > 0 (w = S(|d| + sw) - S(|d| - sw) of the nearest shape's sw, 0 once
+// |d| - sw >= 1): the query is bounded there.
 DVG_DEV bool pf_stroke(const DevScene &S, uint32_t g, float px, float py, PHit &h, float &w) {
-    if (!group_closest(S, g, px, py, BIG, h)) return false;
+    if (!group_closest(S, g, px, py, S.group_pf[2u * g + 1u] + 1.001f, h)) return false;
     float sw = S.shape_data[h.prep * 15u + 3u];
     float d = h.dist;
     w = smoothstep_unit(absf_(d) + sw) - smoothstep_unit(absf_(d) - sw);
