@@ -1592,8 +1592,12 @@ DVG_DEV bool group_closest(const DevScene &S, uint32_t g, float px, float py, fl
         float ox, oy, t;
         uint32_t seg;
         bool is_path;
-        if (!closest_shape(S, prep, spx, spy, BIG, ox, oy, seg, t, is_path)) continue;
         const float *T = S.shape_transform + si * 6u;
+        // shape-space distances beyond best / (smallest scale of the shape-to-
+        // canvas map) cannot win: closest_shape need not resolve them
+        float sc = smin * min_scale2(T[0], T[1], T[3], T[4]);
+        float cut = (sc > 0.f && best < BIG) ? best / sc : BIG;
+        if (!closest_shape(S, prep, spx, spy, cut, ox, oy, seg, t, is_path)) continue;
         float gx = T[0] * ox + T[1] * oy + T[2], gy = T[3] * ox + T[4] * oy + T[5];
         float cx = gs[0] * gx + gs[1] * gy + gs[2], cy = gs[3] * gx + gs[4] * gy + gs[5];
         float d = dist_pt(cx, cy, px, py);
