@@ -371,11 +371,10 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         float mxx = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
         float mny = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
         float mxy = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
-        if (prm->use_prefiltering) {
-            // the prefiltered adjoint reaches one canvas pixel past a boundary
-            // (smoothstep band); pad in canvas space, whatever the scaling
-            mnx -= 1.0f; mny -= 1.0f; mxx += 1.0f; mxy += 1.0f;
-        }
+        // pad in canvas space, whatever the scaling (reaches of the
+        // prefiltered adjoint and of the boundary pass's query points)
+        const float pad = dvgk::tile_pad(W, H);
+        mnx -= pad; mny -= pad; mxx += pad; mxy += pad;
         if (mxx < 0.f || mxy < 0.f || mnx >= (float)W || mny >= (float)H) continue;
         auto clampi = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
         int tx0 = clampi((int)std::floor(mnx / (float)dvgk::TILE), (int)tiles_x - 1);
@@ -679,7 +678,8 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
                              npix * 2};
         for (int k = 0; k < 14; ++k) ds->grad_len[k] = (uint32_t)lens[k];
     }
-    dvgk::launch_wind_coef(D.curve_data, (uint32_t)ncurves, (float *)at(i_wind), ctx->stream);
+    dvgk::launch_wind_coef(D.curve_data, (uint32_t)ncurves, (float *)at(i_wind), D.shape_data, D.num_prepared,
+                           ctx->stream);
     // tile lists: count, scan, write (sorted by group id by construction)
     if (ntiles > 0) {
         dvgk::launch_bin_count((const float *)at(i_rng), G, tiles_x, tiles_y, (uint32_t *)at(i_tcnt), ctx->stream);

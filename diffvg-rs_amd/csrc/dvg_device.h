@@ -272,6 +272,11 @@ DVG_DEV void winding_line_cpu(float px, float py, float x0, float y0, float x1, 
 //          and its values Y(t1), Y(t2), Y(1), computed in f64 and rounded;
 //          [21] = 1 / (54 sqrt(q3)), [22] = 2 sqrt(q) + |bb / 3| + 1 (error
 //          bounds of the backward's checked solve)
+//   cubic: [24..29] = min and max of X(t) = ax t^3 + bx t^2 + cx t over each
+//          monotone piece [0, t1], [t1, t2], [t2, 1] (ax, bx, cx the x
+//          coefficients as the reference CPU forms them in f32; the pieces'
+//          ends and X's extrema in f64, rounded), [30] = an absolute bound on
+//          the rounding of those values (the crossing classification below)
 //
 // Root mask: in the three-root case root k is -2 sqrt(q) cos(phi_k) - bb/3
 // with phi_0 in [0, pi/3], phi_1 in [2pi/3, pi], phi_2 in [-2pi/3, -pi/3]
@@ -281,7 +286,46 @@ DVG_DEV void winding_line_cpu(float px, float py, float x0, float y0, float x1, 
 // cosine need not be evaluated.  NaN terms keep every bit set.  Bit 3 marks
 // a well-conditioned segment (fast transcendentals, below).
 
-DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
+// Range of X(t) = A t^3 + B t^2 + C t over [lo, hi] (f64; X's critical points
+// e[0..ne) are the candidates inside).
+DVG_DEV void xrange_piece(double A, double B, double C, const double *e, int ne, double lo, double hi,
+                          double &mn, double &mx) {
+    double v = ((A * lo + B) * lo + C) * lo;
+    mn = mx = v;
+    v = ((A * hi + B) * hi + C) * hi;
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+    for (int j = 0; j < ne; ++j) {
+        if (!(e[j] > lo && e[j] < hi)) continue;
+        v = ((A * e[j] + B) * e[j] + C) * e[j];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+}
+
+// The critical points of A t^3 + B t^2 + C t (roots of 3A t^2 + 2B t + C) in
+// the cancellation-free form; returns how many.
+DVG_DEV int cubic_crit_f64(double A, double B, double C, double e[2]) {
+    if (A == 0.0) {
+        if (B == 0.0) return 0;
+        e[0] = -C / (2.0 * B);
+        return 1;
+    }
+    double disc = 4.0 * B * B - 12.0 * A * C;
+    if (!(disc >= 0.0)) return 0;
+    double sd = sqrt(disc);
+    double qq = -0.5 * (2.0 * B + (B < 0.0 ? -sd : sd));
+    if (qq == 0.0) {
+        e[0] = -B / (3.0 * A);
+        return 1;
+    }
+    e[0] = qq / (3.0 * A);
+    e[1] = C / qq;
+    return 2;
+}
+
+// xs: the segment's four x control values (record slots 24..30), or null.
+DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o, const float *xs = nullptr) {
     float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;
     float b = 3.0f * y0 - 6.0f * y1 + 3.0f * y2;
     float c = -3.0f * y0 + 3.0f * y1;
@@ -345,6 +389,28 @@ DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o) {
     o[21] = 1.0f / (54.0f * o[6]);
     o[22] = 2.0f * fabsf(sq) + fabsf(off) + 1.0f;
     o[23] = 0.f;
+    for (int k = 24; k < WIND_STRIDE; ++k) o[k] = 0.f;
+    if (xs) {
+        // X(t) = ax t^3 + bx t^2 + cx t, the coefficients in f32 exactly as
+        // distance/winding.rs:176-178 forms them
+        float ax = -xs[0] + 3.0f * xs[1] - 3.0f * xs[2] + xs[3];
+        float bx = 3.0f * xs[0] - 6.0f * xs[1] + 3.0f * xs[2];
+        float cx = -3.0f * xs[0] + 3.0f * xs[1];
+        double e[2];
+        int ne = cubic_crit_f64(ax, bx, cx, e);
+        double tb[4] = {0.0, t1, t2, 1.0};
+        double big = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            double mn, mx;
+            xrange_piece(ax, bx, cx, e, ne, tb[k], tb[k + 1], mn, mx);
+            o[24 + 2 * k] = (float)mn;
+            o[25 + 2 * k] = (float)mx;
+            big = fmax(big, fmax(fabs(mn), fabs(mx)));
+        }
+        // f64 -> f32 rounding of the ranges (2^-24 relative) and of the
+        // query's sum, with a wide factor, plus an absolute floor
+        o[30] = (float)(4.0e-6 * big + 1.0e-5);
+    }
 }
 
 DVG_DEV void wind_coef_quad(float y0, float y1, float y2, float *o) {
@@ -610,9 +676,53 @@ DVG_DEV int winding_cubic_pieces(float px, float py, float x0, float y0, float x
 // The backward's cubic inside test: the f32 record solve, or the monotone
 // pieces on segments whose Cardano solve is ill-conditioned (record bit 4).
 // CHK: `uns` is raised where the f64 solve could decide differently.
+//
+// Before any root is solved, the crossings are classified on the monotone
+// pieces: a piece over which P changes sign holds exactly one root, whose
+// curve point lies inside the piece's x-range (record slots 24..29); when
+// that range lies wholly right of px (beyond the record's rounding bound
+// plus the query's) the crossing counts, signed by the piece's direction,
+// and when wholly left it does not, which is what the f64 solve decides
+// with its root anywhere in the piece.  Only when every crossing is so
+// decided and no piece end lies within rounding of py (a root at a piece
+// end, two roots about to merge: the solve's own rounding decides) does the
+// classification stand; otherwise the solve below runs as before.
+DVG_DEV bool winding_cubic_classify(float px, float py, float x0, float y0, const float *wc, int &n) {
+    if (!(fabsf(wc[0]) >= 1.0e-6f)) return false;  // the CPU's quadratic branch
+    float d = y0 - py;
+    float v0 = d, v1 = wc[18] + d, v2 = wc[19] + d, v3 = wc[20] + d;
+    float ys = 1.0e-6f * (fabsf(wc[18]) + fabsf(wc[19]) + fabsf(wc[20]) + fabsf(d)) + 1.0e-6f;
+    if (!(fminf(fminf(fabsf(v0), fabsf(v1)), fminf(fabsf(v2), fabsf(v3))) > ys)) return false;
+    float dx = x0 - px;
+    float mx = wc[30] + 4.0e-6f * fabsf(dx);
+    float t1 = wc[16], t2 = wc[17];
+    int w = 0;
+    bool ok = true;
+    // piece k: ends (ta, tb), values (va, vb), X range (lo, hi)
+    auto piece = [&](bool live, float va, float vb, float lo, float hi) {
+        if (!live || (va < 0.f) == (vb < 0.f)) return;
+        if (lo + dx > mx) w += (vb > va) ? 1 : -1;
+        else if (!(hi + dx < -mx)) ok = false;
+    };
+    piece(t1 > 0.f, v0, v1, wc[24], wc[25]);
+    piece(t1 < t2, v1, v2, wc[26], wc[27]);
+    piece(t2 < 1.f, v2, v3, wc[28], wc[29]);
+    n = w;
+    return ok;
+}
+
 template <bool CHK>
 DVG_DEV void winding_cubic_bwd(float px, float py, float x0, float y0, float x1, float x2, float x3,
                                const float *wc, uint32_t rule, float &w, float &cr, float &uns) {
+    int nc;
+    if (winding_cubic_classify(px, py, x0, y0, wc, nc)) {
+        if (rule == 1u) {
+            if (nc & 1) cr = 1.f - cr;
+        } else {
+            w += (float)nc;
+        }
+        return;
+    }
     if ((uint32_t)wc[11] & 16u) {
         // the even-odd rule toggles per crossing: the parity of the signed sum
         int n = winding_cubic_pieces<CHK>(px, py, x0, y0, x1, x2, x3, wc, uns);

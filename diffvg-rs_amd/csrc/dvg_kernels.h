@@ -11,7 +11,16 @@ constexpr int TILE = 16;            // tile edge in pixels (renderer/constants.r
 constexpr int BLOCK = 256;          // threads per workgroup (4 waves of 64)
 constexpr int FRAG_WINDOW = 8;      // fragments held per lane in the adjoint walk
 constexpr int LDS_GROUP_CAP = 256;  // tile groups whose paint grads are staged in LDS
-constexpr int WIND_STRIDE = 24;     // floats of a segment's winding-term record
+constexpr int WIND_STRIDE = 32;     // floats of a segment's winding-term record
+
+// Canvas pixels by which a group's box is padded when it is binned into tiles.
+// The prefiltered adjoint reaches one pixel past a boundary (smoothstep
+// band); the boundary pass evaluates both query points of a boundary sample,
+// each at most 1e-4 of the canvas size from the sampled point, through the
+// list of the sampled point's tile.  The pad exceeds both reaches.
+inline __host__ __device__ float tile_pad(uint32_t W, uint32_t H) {
+    return 1.0f + floorf(1.0e-4f * (float)(W > H ? W : H));
+}
 
 // Scene arrays are read through the constant address space on the device:
 // they do not change while a render or adjoint kernel runs, so loads at
@@ -233,7 +242,8 @@ void launch_selftest_winding(uint32_t n, uint32_t seed, uint32_t *mismatches, hi
 void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStream_t stream);
 // Per-segment winding terms of curve_data (13 floats a segment) into
 // curve_wind; after every change of curve_data.
-void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream);
+void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, const float *shape_data,
+                      uint32_t nprep, hipStream_t stream);
 void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *path_points, const float *group_colors,
                          hipStream_t stream);
 

@@ -315,6 +315,45 @@ DVG_DEV float seg_radius(uint32_t kind, float t, float r0, float r1, float r2, f
     return tt * tt * tt * r0 + 3.0f * tt * tt * t * r1 + 3.0f * tt * t * t * r2 + t * t * t * r3;
 }
 
+// The winding test of one segment whose ray box the point passed (shape_fill).
+template <int M>
+DVG_DEV void fill_segment(uint32_t k, float px, float py, float x0, float y0, float x1, float y1, float x2, float y2,
+                          float x3, float y3, const float *wc, uint32_t rule, float &lw, float &lc, float &unsure) {
+    constexpr bool CPU = M != COV_FWD;
+    {
+        if (k == 0) {
+            if (CPU) winding_line_cpu(px, py, x0, y0, x1, y1, rule, lw, lc);
+            else winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
+        } else if (M == COV_EXACT) {
+            // the checked f32 test, and the f64 one where it is uncertain
+            float tw = 0.f, tc = 0.f, tu = 0.f;
+            if (k == 1) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, tw, tc, &tu);
+            else winding_cubic_bwd<true>(px, py, x0, y0, x1, x2, x3, wc, rule, tw, tc, tu);
+            int n = (int)tw;
+            bool odd = tc > 0.f;
+            if (tu > 0.f) {
+                float xs[4] = {x0, x1, x2, x3}, ys[4] = {y0, y1, y2, y3};
+                n = (k == 1) ? winding_quad_f64(px, py, xs, ys) : winding_cubic_f64(px, py, xs, ys);
+                odd = (n & 1) != 0;
+            }
+            if (rule == 1u) {
+                if (odd) lc = 1.f - lc;
+            } else {
+                lw += (float)n;
+            }
+        } else if (k == 1) {
+            // (the unchecked tests get no pointer into c: it would keep c in memory)
+            if (M == COV_CHK) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, lw, lc, &unsure);
+            else winding_quad_w<false>(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
+        } else if (CPU) {
+            float u = 0.f;
+            winding_cubic_bwd<M == COV_CHK>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc, M == COV_CHK ? unsure : u);
+        } else {
+            winding_cubic_w<false>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
+        }
+    }
+}
+
 // accumulate_shape_fill, forward.rs:409-560 (distances only when prefiltering:
 // the non-prefiltered blend reads the winding alone).  With CPU = true the
 // winding follows the reference CPU's winding_number_segment
@@ -377,42 +416,58 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         // forward.rs:938-1018: a +x ray only meets segments spanning py with
         // some control point right of px.
         if (py < by0 || py > by1 || px > bx1) continue;
-        if (k == 0) {
-            if (CPU) winding_line_cpu(px, py, x0, y0, x1, y1, rule, lw, lc);
-            else winding_line(px, py, x0, y0, x1, y1, rule, lw, lc);
-        } else if (M == COV_EXACT) {
-            // the checked f32 test, and the f64 one where it is uncertain
-            float tw = 0.f, tc = 0.f, tu = 0.f;
-            if (k == 1) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, tw, tc, &tu);
-            else winding_cubic_bwd<true>(px, py, x0, y0, x1, x2, x3, wc, rule, tw, tc, tu);
-            int n = (int)tw;
-            bool odd = tc > 0.f;
-            if (tu > 0.f) {
-                float xs[4] = {x0, x1, x2, x3}, ys[4] = {y0, y1, y2, y3};
-                n = (k == 1) ? winding_quad_f64(px, py, xs, ys) : winding_cubic_f64(px, py, xs, ys);
-                odd = (n & 1) != 0;
-            }
-            if (rule == 1u) {
-                if (odd) lc = 1.f - lc;
-            } else {
-                lw += (float)n;
-            }
-        } else if (k == 1) {
-            // (the unchecked tests get no pointer into c: it would keep c in memory)
-            if (M == COV_CHK) winding_quad_w<true>(px, py, x0, y0, x1, x2, wc, rule, lw, lc, &c.unsure);
-            else winding_quad_w<false>(px, py, x0, y0, x1, x2, wc, rule, lw, lc);
-        } else if (CPU) {
-            float u = 0.f;
-            winding_cubic_bwd<M == COV_CHK>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc, M == COV_CHK ? c.unsure : u);
-        } else {
-            winding_cubic_w<false>(px, py, x0, y0, x1, x2, x3, wc, rule, lw, lc);
-        }
+        fill_segment<M>(k, px, py, x0, y0, x1, y1, x2, y2, x3, y3, wc, rule, lw, lc, c.unsure);
     }
     if (lmin < c.fill_min) c.fill_min = lmin;
     if (rule == 1u) {
         if (lc > 0.f) c.crossings = 1.f - c.crossings;
     } else {
         c.winding += lw;
+    }
+}
+
+// The CPU fill windings of one shape at the two query points of a boundary
+// sample (shape space; a0 / a1: which points the group's root box admits).
+// On a path the second point's winding is the first's when the path is a
+// closed loop whose CPU winding is the curve's winding number (record slot
+// 31 of its first segment, k_shape_topo) and no segment's control box meets
+// the box spanned by the two points: the segment joining them then crosses
+// no edge, so both lie in one component of the complement of the curve,
+// where the winding number is constant.  Otherwise each point is tested.
+template <int M>
+DVG_DEV void shape_fill_pair(const DevScene &S, const float *sd, uint32_t rule, float px0, float py0, bool a0,
+                             float px1, float py1, bool a1, Cov &c0, Cov &c1) {
+    if ((uint32_t)sd[0] != DVG_SHAPE_PATH || !a0) {
+        if (a0) shape_fill<M>(S, sd, rule, false, px0, py0, c0);
+        if (a1) shape_fill<M>(S, sd, rule, false, px1, py1, c1);
+        return;
+    }
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    float ex0 = fminf(px0, px1), ex1 = fmaxf(px0, px1), ey0 = fminf(py0, py1), ey1 = fmaxf(py0, py1);
+    bool same = a1 && cnt > 0u && S.curve_wind[(size_t)off * WIND_STRIDE + 31] > 0.f;
+    float lw = 0.f, lc = 0.f;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = S.curve_data + (off + s) * 13u;
+        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
+        float bx0 = wc[15], bx1 = wc[14], by0 = wc[12], by1 = wc[13];
+        if (!(ex1 < bx0 || ex0 > bx1 || ey1 < by0 || ey0 > by1)) same = false;
+        if (py0 < by0 || py0 > by1 || px0 > bx1) continue;
+        fill_segment<M>((uint32_t)cv[0], px0, py0, cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7], cv[8], wc, rule, lw,
+                        lc, c0.unsure);
+    }
+    if (rule == 1u) {
+        if (lc > 0.f) c0.crossings = 1.f - c0.crossings;
+    } else {
+        c0.winding += lw;
+    }
+    if (same) {
+        if (rule == 1u) {
+            if (lc > 0.f) c1.crossings = 1.f - c1.crossings;
+        } else {
+            c1.winding += lw;
+        }
+    } else if (a1) {
+        shape_fill<M>(S, sd, rule, false, px1, py1, c1);
     }
 }
 
@@ -887,6 +942,56 @@ DVG_DEV void group_frags(const DevScene &S, uint32_t g, float px, float py, bool
     stroke = sk != DVG_PAINT_NONE && c.stroke_hit > 0.f;
     fill = fk != DVG_PAINT_NONE && ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f);
     if (M == COV_CHK && c.unsure > 0.f) *uns = true;
+}
+
+// group_frags at the two query points of a boundary sample.
+template <int M>
+DVG_DEV void group_frags_pair(const DevScene &S, uint32_t g, float px0, float py0, float px1, float py1, bool &st0,
+                              bool &fi0, bool &st1, bool &fi1, bool &uns) {
+    const float *gd = S.group_data + g * 16u;
+    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+    st0 = fi0 = st1 = fi1 = false;
+    if (fk == DVG_PAINT_NONE && sk == DVG_PAINT_NONE) return;
+    float lx0, ly0, lx1, ly1;
+    group_local(S, g, px0, py0, lx0, ly0);
+    group_local(S, g, px1, py1, lx1, ly1);
+    Cov c0 = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f}, c1 = c0;
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    if (fk != DVG_PAINT_NONE) {
+        // the group's root box gates every shape's winding (group_cov)
+        const float *rb = S.group_cpu_bounds + g * 4u;
+        bool has = rb[0] <= rb[2] && rb[1] <= rb[3];
+        bool a0 = !has || (lx0 >= rb[0] && lx0 <= rb[2] && ly0 >= rb[1] && ly0 <= rb[3]);
+        bool a1 = !has || (lx1 >= rb[0] && lx1 <= rb[2] && ly1 >= rb[1] && ly1 <= rb[3]);
+        if (a0 || a1) {
+            for (uint32_t i = 0; i < cnt; ++i) {
+                uint32_t si = (uint32_t)S.group_shapes[off + i];
+                const float *xf = S.shape_xform + si * 6u;
+                float sx0 = xf[0] * lx0 + xf[1] * ly0 + xf[2], sy0 = xf[3] * lx0 + xf[4] * ly0 + xf[5];
+                float sx1 = xf[0] * lx1 + xf[1] * ly1 + xf[2], sy1 = xf[3] * lx1 + xf[4] * ly1 + xf[5];
+                shape_fill_pair<M>(S, S.shape_data + si * 15u, rule, sx0, sy0, a0, sx1, sy1, a1, c0, c1);
+            }
+        }
+    }
+    if (sk != DVG_PAINT_NONE) {
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint32_t si = (uint32_t)S.group_shapes[off + i];
+            const float *b = S.shape_bounds + si * 4u;
+            bool ok = b[0] <= b[2] && b[1] <= b[3];
+            bool in0 = ok && !(lx0 < b[0] || lx0 > b[2] || ly0 < b[1] || ly0 > b[3]) && !(c0.stroke_hit > 0.f);
+            bool in1 = ok && !(lx1 < b[0] || lx1 > b[2] || ly1 < b[1] || ly1 > b[3]) && !(c1.stroke_hit > 0.f);
+            if (!in0 && !in1) continue;
+            const float *xf = S.shape_xform + si * 6u;
+            const float *sd = S.shape_data + si * 15u;
+            if (in0) shape_stroke<M>(S, sd, false, xf[0] * lx0 + xf[1] * ly0 + xf[2], xf[3] * lx0 + xf[4] * ly0 + xf[5], c0);
+            if (in1) shape_stroke<M>(S, sd, false, xf[0] * lx1 + xf[1] * ly1 + xf[2], xf[3] * lx1 + xf[4] * ly1 + xf[5], c1);
+        }
+    }
+    st0 = sk != DVG_PAINT_NONE && c0.stroke_hit > 0.f;
+    st1 = sk != DVG_PAINT_NONE && c1.stroke_hit > 0.f;
+    fi0 = fk != DVG_PAINT_NONE && ((rule == 1u) ? c0.crossings > 0.f : c0.winding != 0.f);
+    fi1 = fk != DVG_PAINT_NONE && ((rule == 1u) ? c1.crossings > 0.f : c1.winding != 0.f);
+    if (M == COV_CHK && (c0.unsure > 0.f || c1.unsure > 0.f)) uns = true;
 }
 
 DVG_DEV float4 frag_color(const DevScene &S, uint32_t g, bool stroke, float px, float py) {
@@ -2425,78 +2530,40 @@ DVG_DEV bool sample_boundary_path(const DevScene &S, uint32_t si, float path_len
     return true;
 }
 
-// Colour of one boundary query point with the EdgeQuery hit tracking of
-// sampling.rs:76-224 (no adjoint).  (qx, qy) is the normalized point.
+// within_distance_edge / is_inside_edge (sampling.rs:485-549) on the sampled
+// shape alone, in a group of several shapes.
 template <int M>
-DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
-                              uint32_t eq_prep, bool &hit, bool &uns);
-
-// M: COV_CHK (raises `uns` where an f64 evaluation could decide otherwise),
-// COV_CPU or COV_EXACT.
-template <int M>
-DVG_DEV float4 edge_color(const DevScene &S, float qx, float qy, float4 bg, uint32_t eq_group, uint32_t eq_prep,
-                          bool &hit, bool &uns) {
-    float px = qx * (float)S.W, py = qy * (float)S.H;
-    int tx = (int)floorf(px / (float)TILE), ty = (int)floorf(py / (float)TILE);
-    tx = tx < 0 ? 0 : (tx >= (int)S.tiles_x ? (int)S.tiles_x - 1 : tx);
-    ty = ty < 0 ? 0 : (ty >= (int)S.tiles_y ? (int)S.tiles_y - 1 : ty);
-    uint32_t tile = (uint32_t)ty * S.tiles_x + (uint32_t)tx;
-    // samples are sorted by pixel block, so the lanes of a wave almost always
-    // query one tile: then the walk runs with the tile in a scalar register
-    // (list, group and curve data through scalar loads)
-    uint32_t t0 = __builtin_amdgcn_readfirstlane(tile);
-    if (__ballot(tile != t0) == 0) return edge_walk_tile<M>(S, t0, px, py, bg, eq_group, eq_prep, hit, uns);
-    return edge_walk_tile<M>(S, tile, px, py, bg, eq_group, eq_prep, hit, uns);
+DVG_DEV bool edge_shape_hit(const DevScene &S, uint32_t g, uint32_t eq_prep, float px, float py, bool &uns) {
+    const float *gd = S.group_data + g * 16u;
+    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+    float lx, ly;
+    group_local(S, g, px, py, lx, ly);
+    const float *xf = S.shape_xform + eq_prep * 6u;
+    float spx = xf[0] * lx + xf[1] * ly + xf[2];
+    float spy = xf[3] * lx + xf[4] * ly + xf[5];
+    const float *sd = S.shape_data + eq_prep * 15u;
+    Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
+    if (sk != DVG_PAINT_NONE) shape_stroke<M>(S, sd, false, spx, spy, c);
+    if (fk != DVG_PAINT_NONE) shape_fill<M>(S, sd, rule, false, spx, spy, c);
+    if (M == COV_CHK && c.unsure > 0.f) uns = true;
+    bool et = c.stroke_hit > 0.f;
+    if (fk != DVG_PAINT_NONE && ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f)) et = true;
+    return et;
 }
 
-template <int M>
-DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float py, float4 bg, uint32_t eq_group,
-                              uint32_t eq_prep, bool &hit, bool &uns) {
-    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
-    float4 acc = bg;
-    uint32_t nfr = 0;
-    bool edge_test = false;  // hit set while collecting fragments
-    bool h0 = false, h1 = true;  // compositing-loop hit for an initial false / true
-    for (uint32_t e = lb; e < le; ++e) {
-        uint32_t g = S.tile_entries[e];
-        bool st, fi;
-        group_frags<M>(S, g, px, py, st, fi, &uns);
-        if (g == eq_group) {
-            // within_distance_edge / is_inside_edge (sampling.rs:485-549) on
-            // the sampled shape alone; for a group of one shape that is the
-            // group's own stroke / fill test above (bounds are conservative)
-            const float *gd = S.group_data + g * 16u;
-            if ((uint32_t)gd[1] == 1u) {
-                if (st || fi) edge_test = true;
-            } else {
-                uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-                float lx, ly;
-                group_local(S, g, px, py, lx, ly);
-                const float *xf = S.shape_xform + eq_prep * 6u;
-                float spx = xf[0] * lx + xf[1] * ly + xf[2];
-                float spy = xf[3] * lx + xf[4] * ly + xf[5];
-                const float *sd = S.shape_data + eq_prep * 15u;
-                Cov c = {BIG, 0.f, 0.f, BIG, 0.f, 0.f, 0.f};
-                if (sk != DVG_PAINT_NONE) shape_stroke<M>(S, sd, false, spx, spy, c);
-                if (fk != DVG_PAINT_NONE) shape_fill<M>(S, sd, rule, false, spx, spy, c);
-                if (M == COV_CHK && c.unsure > 0.f) uns = true;
-                if (c.stroke_hit > 0.f) edge_test = true;
-                if ((rule == 1u) ? c.crossings > 0.f : c.winding != 0.f) {
-                    if (fk != DVG_PAINT_NONE) edge_test = true;
-                }
-            }
-        }
-        for (int k = 0; k < 2; ++k) {
-            bool is_stroke = k == 0;
-            if (!(is_stroke ? st : fi)) continue;
-            float4 c = frag_color(S, g, is_stroke, px, py);
-            if (c.w >= 1.0f) { h0 = false; h1 = false; }
-            if (g == eq_group) { h0 = true; h1 = true; }
-            blend_cpu(acc, c);
-            ++nfr;
-        }
-    }
-    hit = edge_test ? h1 : h0;
+// One query point's fragment of group g in the EdgeQuery compositing loop
+// (sampling.rs:145-160): blend, count, and track the hit for an initial
+// false (h0) and an initial true (h1).
+DVG_DEV void edge_blend(const DevScene &S, uint32_t g, bool is_stroke, bool eq, float px, float py, float4 &acc,
+                        uint32_t &nfr, bool &h0, bool &h1) {
+    float4 c = frag_color(S, g, is_stroke, px, py);
+    if (c.w >= 1.0f) { h0 = false; h1 = false; }
+    if (eq) { h0 = true; h1 = true; }
+    blend_cpu(acc, c);
+    ++nfr;
+}
+
+DVG_DEV float4 edge_finish(float4 acc, uint32_t nfr, float4 bg) {
     if (nfr == 0) return bg;
     float fa = acc.w;
     if (fa > 1.0e-6f) {
@@ -2504,6 +2571,99 @@ DVG_DEV float4 edge_walk_tile(const DevScene &S, uint32_t tile, float px, float 
         return make_float4(acc.x * sc, acc.y * sc, acc.z * sc, fa);
     }
     return acc;
+}
+
+// Colours of the two query points of a boundary sample with the EdgeQuery hit
+// tracking of sampling.rs:76-224 (no adjoint), in one walk of a tile list:
+// both points' fragment tests per group (group_frags_pair), two composites.
+template <int M>
+DVG_DEV void edge_walk_pair(const DevScene &S, uint32_t tile, float px0, float py0, float px1, float py1, float4 bg,
+                            uint32_t eq_group, uint32_t eq_prep, float4 &col0, float4 &col1, bool &hit0, bool &hit1,
+                            bool &uns) {
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    float4 acc0 = bg, acc1 = bg;
+    uint32_t nfr0 = 0, nfr1 = 0;
+    bool et0 = false, et1 = false;  // hit set while collecting fragments
+    bool h00 = false, h01 = true, h10 = false, h11 = true;
+    for (uint32_t e = lb; e < le; ++e) {
+        uint32_t g = S.tile_entries[e];
+        bool st0, fi0, st1, fi1;
+        group_frags_pair<M>(S, g, px0, py0, px1, py1, st0, fi0, st1, fi1, uns);
+        bool eq = g == eq_group;
+        if (eq) {
+            // for a group of one shape the sampled shape's own tests are the
+            // group's tests above (bounds are conservative)
+            if ((uint32_t)S.group_data[g * 16u + 1] == 1u) {
+                et0 = et0 || st0 || fi0;
+                et1 = et1 || st1 || fi1;
+            } else {
+                if (edge_shape_hit<M>(S, g, eq_prep, px0, py0, uns)) et0 = true;
+                if (edge_shape_hit<M>(S, g, eq_prep, px1, py1, uns)) et1 = true;
+            }
+        }
+        if (st0) edge_blend(S, g, true, eq, px0, py0, acc0, nfr0, h00, h01);
+        if (st1) edge_blend(S, g, true, eq, px1, py1, acc1, nfr1, h10, h11);
+        if (fi0) edge_blend(S, g, false, eq, px0, py0, acc0, nfr0, h00, h01);
+        if (fi1) edge_blend(S, g, false, eq, px1, py1, acc1, nfr1, h10, h11);
+    }
+    hit0 = et0 ? h01 : h00;
+    hit1 = et1 ? h11 : h10;
+    col0 = edge_finish(acc0, nfr0, bg);
+    col1 = edge_finish(acc1, nfr1, bg);
+}
+
+DVG_DEV uint32_t tile_of(const DevScene &S, float px, float py) {
+    int tx = (int)floorf(px / (float)TILE), ty = (int)floorf(py / (float)TILE);
+    tx = tx < 0 ? 0 : (tx >= (int)S.tiles_x ? (int)S.tiles_x - 1 : tx);
+    ty = ty < 0 ? 0 : (ty >= (int)S.tiles_y ? (int)S.tiles_y - 1 : ty);
+    return (uint32_t)ty * S.tiles_x + (uint32_t)tx;
+}
+
+// The colours of a boundary sample's query points (normalized (qx0, qy0) and
+// (qx1, qy1)) through the tile list of the sampled pixel (pxi, pyi): binning
+// pads every group's box by tile_pad pixels, more than either point's
+// distance from the sampled point, so that list holds every group either
+// point can be inside.  Samples are sorted by pixel block, so the lanes of a
+// wave almost always share the tile, and the walk then runs with it in a
+// scalar register (list, group and curve data through scalar loads).  A lane
+// whose points leave the padded tile (a sampled point up to a pixel left of
+// or above the canvas, which truncates into pixel 0) walks each point through
+// its own tile, one per pass.
+template <int M>
+DVG_DEV void edge_colors(const DevScene &S, uint32_t pxi, uint32_t pyi, float qx0, float qy0, float qx1, float qy1,
+                         float4 bg, uint32_t eq_group, uint32_t eq_prep, float4 &c0, float4 &c1, bool &h0, bool &h1,
+                         bool &uns) {
+    float Wf = (float)S.W, Hf = (float)S.H;
+    float px0 = qx0 * Wf, py0 = qy0 * Hf, px1 = qx1 * Wf, py1 = qy1 * Hf;
+    uint32_t tx = pxi / (uint32_t)TILE, ty = pyi / (uint32_t)TILE;
+    float pad = tile_pad(S.W, S.H) - 0.01f;  // less a margin for rounding
+    float rx0 = (float)(tx * TILE) - pad, rx1 = (float)((tx + 1) * TILE) + pad;
+    float ry0 = (float)(ty * TILE) - pad, ry1 = (float)((ty + 1) * TILE) + pad;
+    bool split = !(fminf(px0, px1) >= rx0 && fmaxf(px0, px1) <= rx1 && fminf(py0, py1) >= ry0 &&
+                   fmaxf(py0, py1) <= ry1);
+    uint32_t ptile = ty * S.tiles_x + tx;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && __ballot(split) == 0) break;
+        if (pass == 0 || split) {
+            float ax = px0, ay = py0, bx = px1, by = py1;
+            uint32_t t = ptile;
+            if (split) {
+                if (pass == 0) { bx = px0; by = py0; }
+                else { ax = px1; ay = py1; }
+                t = tile_of(S, ax, ay);
+            }
+            float4 ca, cb;
+            bool ha, hb;
+            uint32_t t0 = __builtin_amdgcn_readfirstlane(t);
+            if (__ballot(t != t0) == 0)
+                edge_walk_pair<M>(S, t0, ax, ay, bx, by, bg, eq_group, eq_prep, ca, cb, ha, hb, uns);
+            else
+                edge_walk_pair<M>(S, t, ax, ay, bx, by, bg, eq_group, eq_prep, ca, cb, ha, hb, uns);
+            if (pass == 0) { c0 = ca; h0 = ha; }
+            if (pass == 1 || !split) { c1 = cb; h1 = hb; }
+        }
+    }
 }
 
 // One boundary sample of boundary_sampling (boundary.rs:138-197): shape by
@@ -2743,14 +2903,14 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
             bool ih, oh;
             float qix = P.qx - P.ncx * 1.0e-4f, qiy = P.qy - P.ncy * 1.0e-4f;
             float qox = P.qx + P.ncx * 1.0e-4f, qoy = P.qy + P.ncy * 1.0e-4f;
-            uint32_t gi = P.gi, prep = P.prep;
-            // the point waits in LDS, not in registers, during the two scene
-            // walks (the empty asm is a compiler barrier: P is reloaded)
+            uint32_t gi = P.gi, prep = P.prep, pxi = P.pxi, pyi = P.pyi;
+            // the point waits in LDS, not in registers, during the scene walk
+            // (the empty asm is a compiler barrier: P is reloaded)
             s_pt[threadIdx.x] = P;
             asm volatile("" ::: "memory");
             bool uns = false;
-            float4 cin = edge_color<M>(S, qix, qiy, bg, gi, prep, ih, uns);
-            float4 cout = edge_color<M>(S, qox, qoy, bg, gi, prep, oh, uns);
+            float4 cin, cout;
+            edge_colors<M>(S, pxi, pyi, qix, qiy, qox, qoy, bg, gi, prep, cin, cout, ih, oh, uns);
             asm volatile("" ::: "memory");
             P = s_pt[threadIdx.x];
             valid = ih || oh;
@@ -3044,7 +3204,7 @@ __global__ void k_selftest_winding(uint32_t n, uint32_t seed, uint32_t *__restri
     float px = cx + 128.0f * rng.next_f32() - 64.0f;
     float py = ymin + (ymax - ymin) * rng.next_f32();
     float rec[WIND_STRIDE];
-    wind_coef_cubic(y[0], y[1], y[2], y[3], rec);
+    wind_coef_cubic(y[0], y[1], y[2], y[3], rec, x);
     float w0 = 0.f, w1 = 0.f, cr = 0.f;
     winding_cubic_libm(px, py, x[0], y[0], x[1], y[1], x[2], y[2], x[3], y[3], w0);
     winding_cubic_w<true>(px, py, x[0], y[0], x[1], x[2], x[3], rec, 0u, w1, cr);
@@ -3077,7 +3237,7 @@ __global__ void k_selftest_inside(uint32_t n, uint32_t seed, uint32_t *__restric
     float px = cx + 128.0f * rng.next_f32() - 64.0f;
     float py = ymin + (ymax - ymin) * rng.next_f32();
     float rec[WIND_STRIDE];
-    wind_coef_cubic(y[0], y[1], y[2], y[3], rec);
+    wind_coef_cubic(y[0], y[1], y[2], y[3], rec, x);
     if (fam == 2u) {
         // a y-extremum of the curve: record slot 16 or 17 when inside (0, 1)
         float tc = rec[16] < 1.f ? rec[16] : (rec[17] < 1.f ? rec[17] : 0.5f);
@@ -3661,17 +3821,54 @@ __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurv
     float *o = out + (size_t)i * WIND_STRIDE;
     uint32_t k = (uint32_t)cv[0];
     if (k == 2u) {
-        wind_coef_cubic(cv[2], cv[4], cv[6], cv[8], o);
+        const float xs[4] = {cv[1], cv[3], cv[5], cv[7]};
+        wind_coef_cubic(cv[2], cv[4], cv[6], cv[8], o, xs);
     } else if (k == 1u) {
         wind_coef_quad(cv[2], cv[4], cv[6], o);
     } else {
-        for (int j = 0; j < 12; ++j) o[j] = 0.f;
+        for (int j = 0; j < WIND_STRIDE; ++j) o[j] = 0.f;
     }
     float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
     o[12] = fminf(fminf(y0, y1), fminf(y2, y3));
     o[13] = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
     o[14] = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
     o[15] = fminf(fminf(x0, x1), fminf(x2, x3));
+}
+
+// Slot 31 of a path's first segment record (after k_wind_coef): 1 when the
+// reference CPU's winding of the path (distance/winding.rs:118-192) is the
+// winding number of its curve, which is constant off the curve: the segments
+// form one closed loop (each begins where the previous ends, the last ends
+// where the first begins, coordinates finite) and no cubic takes the CPU's
+// quadratic branch with a near-zero quadratic term (distance/utils.rs:44-89:
+// |a| < 1e-6 drops the cubic term, and with b near zero the linear root is
+// returned twice, or the dropped term's root can fall inside [0, 1]).
+__global__ void k_shape_topo(const float *__restrict__ shape_data, uint32_t nprep, const float *__restrict__ curve_data,
+                             uint32_t ncurves, float *__restrict__ curve_wind) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprep) return;
+    const float *sd = shape_data + (size_t)p * 15u;
+    if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return;
+    uint32_t off = (uint32_t)sd[12], cnt = (uint32_t)sd[13];
+    if (cnt == 0u || off >= ncurves || cnt > ncurves - off) return;
+    bool ok = true;
+    float sx = curve_data[(size_t)off * 13u + 1], sy = curve_data[(size_t)off * 13u + 2], ex = sx, ey = sy;
+    for (uint32_t s = 0; s < cnt; ++s) {
+        const float *cv = curve_data + (size_t)(off + s) * 13u;
+        uint32_t k = (uint32_t)cv[0];
+        if (cv[1] != ex || cv[2] != ey) ok = false;
+        for (int j = 1; j < 9; ++j)
+            if (!isfinite(cv[j])) ok = false;
+        if (k == 2u) {
+            float a = -cv[2] + 3.0f * cv[4] - 3.0f * cv[6] + cv[8];
+            float b = 3.0f * cv[2] - 6.0f * cv[4] + 3.0f * cv[6];
+            if (!(fabsf(a) >= 1.0e-6f) && !(fabsf(b) >= 1.0e-4f)) ok = false;
+        }
+        ex = k == 0u ? cv[3] : (k == 1u ? cv[5] : cv[7]);
+        ey = k == 0u ? cv[4] : (k == 1u ? cv[6] : cv[8]);
+    }
+    if (ex != sx || ey != sy) ok = false;
+    curve_wind[(size_t)off * WIND_STRIDE + 31] = ok ? 1.f : 0.f;
 }
 
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
@@ -3792,7 +3989,8 @@ __global__ void k_upd_groups(DevScene S, const float *__restrict__ shape_bounds,
                        m[3] * u[2] + m[4] * u[3] + m[5], m[3] * u[0] + m[4] * u[3] + m[5]};
         float mnx = fminf(fminf(xs[0], xs[1]), fminf(xs[2], xs[3])), mxx = fmaxf(fmaxf(xs[0], xs[1]), fmaxf(xs[2], xs[3]));
         float mny = fminf(fminf(ys[0], ys[1]), fminf(ys[2], ys[3])), mxy = fmaxf(fmaxf(ys[0], ys[1]), fmaxf(ys[2], ys[3]));
-        if (S.prefilter) { mnx -= 1.0f; mny -= 1.0f; mxx += 1.0f; mxy += 1.0f; }
+        const float pad = tile_pad(S.W, S.H);  // as the host's binning
+        mnx -= pad; mny -= pad; mxx += pad; mxy += pad;
         if (!(mxx < 0.f || mxy < 0.f || mnx >= (float)S.W || mny >= (float)S.H)) {
             int tx0 = (int)floorf(mnx / (float)TILE), tx1 = (int)floorf(mxx / (float)TILE);
             int ty0 = (int)floorf(mny / (float)TILE), ty1 = (int)floorf(mxy / (float)TILE);
@@ -4220,9 +4418,13 @@ void launch_selftest_inside(uint32_t n, uint32_t seed, uint32_t *counts, hipStre
     hipLaunchKernelGGL(k_selftest_inside, dim3(cdiv(n, 256)), dim3(256), 0, stream, n, seed, counts);
 }
 
-void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, hipStream_t stream) {
+void launch_wind_coef(const float *curve_data, uint32_t ncurves, float *curve_wind, const float *shape_data,
+                      uint32_t nprep, hipStream_t stream) {
     if (ncurves == 0) return;
     hipLaunchKernelGGL(k_wind_coef, dim3(cdiv(ncurves, 256)), dim3(256), 0, stream, curve_data, ncurves, curve_wind);
+    if (nprep)
+        hipLaunchKernelGGL(k_shape_topo, dim3(cdiv(nprep, 256)), dim3(256), 0, stream, shape_data, nprep, curve_data,
+                           ncurves, curve_wind);
 }
 
 // Group BVH boxes refitted to the updated shape bounds, one lane a group: a
@@ -4270,7 +4472,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
     if (P.ncurves)
         hipLaunchKernelGGL(k_upd_curves, dim3(cdiv(P.ncurves, 256)), dim3(256), 0, stream, curve_data, P.curve_pt, pts,
                            P.ncurves);
-    launch_wind_coef(curve_data, P.ncurves, (float *)(S.curve_wind), stream);
+    launch_wind_coef(curve_data, P.ncurves, (float *)(S.curve_wind), shape_data, S.num_prepared, stream);
     if (S.num_prepared)
         hipLaunchKernelGGL(k_upd_instances, dim3(cdiv(S.num_prepared, 256)), dim3(256), 0, stream, S, shape_data,
                            shape_bounds, P.inst_pad, P.cpu_inst_bounds);
