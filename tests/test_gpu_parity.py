@@ -27,7 +27,14 @@ def _prefiltered_opts():
     return RenderOptions(samples_x=1, samples_y=1, seed=0, jitter=False, use_prefiltering=True)
 
 
+def _with_filter(sc, filter_type, radius):
+    from pydvg.scene import Filter
+    sc.filter = Filter(filter_type, radius)
+    return sc
+
+
 def _cases():
+    from pydvg import abi
     return {
         "cubic32": scenes.small_cubic_scene(32, 32, 4, seed=0, spp=2),
         "cubic64_spp4": scenes.small_cubic_scene(64, 64, 12, seed=1, spp=4),
@@ -40,6 +47,12 @@ def _cases():
         # slots and its blocked prefix recomposition (nf well above 32)
         "dense_mixed48": (scenes.mixed_scene(48, 48, 320, seed=7), scenes.config2()[1]),
         "dense_mixed64": (scenes.mixed_scene(64, 64, 640, seed=8), scenes.config2()[1]),
+        # the two filters no workload uses: Hann and radial parabolic (whose
+        # d_filter_radius follows the CPU's form without the d_return factor)
+        "hann_mixed40": (_with_filter(scenes.mixed_scene(40, 40, 12, seed=11), abi.FILTER_HANN, 1.5),
+                         scenes.config2()[1]),
+        "parabolic_cubic40": (_with_filter(scenes.cubic_paths_scene(40, 40, 8, seed=12, spread=12.0),
+                                           abi.FILTER_RADIAL_PARABOLIC, 1.0), scenes.config1()[1]),
     }
 
 
