@@ -25,9 +25,10 @@ namespace dvgk {
 // Waves per SIMD the heavy kernels are compiled for.  The scene walks are
 // VALU-issue bound with memory waits between groups, so more resident waves
 // beat an unspilled register file: measured on the 1024x1024 / 1000-path
-// scene, 7 waves (<= 72 VGPRs) is best for the forward, the boundary pass
-// (4.5 -> 3.7 ms from 4 to 7, slower again at 8) and the colour adjoint
-// (1.64 -> 1.38 ms from 4 to 7, with its LDS cut to 18 KiB a workgroup).
+// scene, 7 waves (<= 72 VGPRs) is best for the forward and the colour adjoint
+// (1.64 -> 1.38 ms from 4 to 7, with its LDS cut to 18 KiB a workgroup, 1.62
+// at 8), 8 (<= 64 VGPRs) for the boundary pass's one walk of both query
+// points (4.87, 3.90, 3.54, 3.53, 3.30, 3.21 ms from 3 to 8 waves).
 #ifndef OCC_FORWARD_PF
 #define OCC_FORWARD_PF 7
 #endif
@@ -38,7 +39,7 @@ namespace dvgk {
 #define OCC_INTERIOR 7
 #endif
 #ifndef OCC_BOUNDARY
-#define OCC_BOUNDARY 7
+#define OCC_BOUNDARY 8
 #endif
 // The prefiltered adjoint (closest-point adjoints, ~200 VGPRs unconstrained)
 // is latency bound at 2 waves; 3 measured best on the 1024x1024 prefiltered
