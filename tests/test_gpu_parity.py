@@ -33,6 +33,42 @@ def _with_filter(sc, filter_type, radius):
     return sc
 
 
+def _edge_cases_scene():
+    """Boundary samples whose query points need care in the one-walk boundary
+    pass: shapes straddling the canvas's left and top edges (sampled points up
+    to a pixel outside the canvas, truncated into pixel 0), a second circle
+    and a copied path 0.05 px from the first (an edge between a sample's two
+    query points, 0.08 px apart at 40 px), and a rotated, scaled shape
+    (the points' box in shape space)."""
+    from pydvg.scene import Circle, Path, Scene, Shape, ShapeGroup, Solid, Filter
+    from pydvg import abi
+    sc = Scene(40, 40, background=(1.0, 1.0, 1.0, 1.0), filter=Filter(abi.FILTER_BOX, 0.5))
+
+    def add(geom, color, transform=None, rule=abi.FILL_NONZERO):
+        sh = Shape(geom)
+        if transform is not None:
+            sh.transform = np.asarray(transform, np.float32)
+        sc.shapes.append(sh)
+        sc.groups.append(ShapeGroup([len(sc.shapes) - 1], fill=Solid(color), fill_rule=rule))
+
+    add(Circle((-0.4, 14.0), 9.0), (0.9, 0.2, 0.1, 0.8))
+    add(Circle((20.0, -0.3), 7.0), (0.1, 0.7, 0.3, 0.6))
+    rng = np.random.default_rng(21)
+    pts = np.array([0.2, 0.3]) + rng.uniform(-6.0, 12.0, size=(12, 2))
+    add(Path(np.full(4, 2, np.uint8), pts.astype(np.float32), is_closed=True), (0.2, 0.3, 0.9, 0.7))
+    add(Circle((24.0, 24.0), 8.0), (0.8, 0.8, 0.1, 0.5))
+    add(Circle((24.0, 24.0), 8.05), (0.3, 0.1, 0.6, 0.5))
+    pts = np.array([26.0, 12.0]) + rng.uniform(-7.0, 7.0, size=(9, 2))
+    add(Path(np.full(3, 2, np.uint8), pts.astype(np.float32), is_closed=True), (0.5, 0.9, 0.5, 0.6))
+    add(Path(np.full(3, 2, np.uint8), (pts + np.array([0.05, 0.0])).astype(np.float32), is_closed=True),
+        (0.9, 0.4, 0.7, 0.6), rule=abi.FILL_EVENODD)
+    c, s_ = np.cos(0.6), np.sin(0.6)
+    pts = rng.uniform(-5.0, 5.0, size=(12, 2))
+    add(Path(np.full(4, 2, np.uint8), pts.astype(np.float32), is_closed=True), (0.1, 0.5, 0.8, 0.9),
+        transform=[[1.3 * c, -1.3 * s_, 12.0], [0.8 * s_, 0.8 * c, 30.0], [0.0, 0.0, 1.0]])
+    return sc
+
+
 def _cases():
     from pydvg import abi
     return {
@@ -53,6 +89,7 @@ def _cases():
                          scenes.config2()[1]),
         "parabolic_cubic40": (_with_filter(scenes.cubic_paths_scene(40, 40, 8, seed=12, spread=12.0),
                                            abi.FILTER_RADIAL_PARABOLIC, 1.0), scenes.config1()[1]),
+        "edge_cases40": (_edge_cases_scene(), scenes.config1()[1]),
     }
 
 
