@@ -4,7 +4,8 @@ Types follow src/scene.rs:1-291, src/geometry.rs:78-89, src/grad.rs:9-126 and
 src/renderer/types.rs:10-39 (names and meaning kept).  `SceneDesc` flattens a
 Scene into the `dvg_scene_desc` struct of include/dvg.h; `FlatGrads` holds the
 14 flat gradient arrays of the C ABI (the layout apply_gpu_grads reads,
-src/renderer/backward_gpu.rs:599-801) and converts them to `SceneGrad`.
+src/renderer/backward_gpu.rs:599-801); `FlatGrads.scene_grad()` unpacks them
+into the `SceneGrad` tree of pydvg.grad.
 """
 from __future__ import annotations
 
@@ -383,9 +384,20 @@ class FlatGrads:
             "d_translation": (np.zeros(sc.width * sc.height * 2, np.float32)
                               if translation else np.zeros(0, np.float32)),
         }
+        self.desc = desc
         self.c = abi.GradsC()
         for k in GRAD_FIELDS:
             setattr(self.c, k, _ptr(self.arrays[k], C.c_float))
+
+    def scene_grad(self):
+        """The arrays unpacked into a SceneGrad of the scene (apply_gpu_grads,
+        src/renderer/backward_gpu.rs:599-801), with the background-image and
+        translation parts when these arrays hold them."""
+        from .grad import SceneGrad, apply_gpu_grads
+        sc = self.desc.scene
+        g = SceneGrad.zeros_from_scene(sc, self.arrays["d_background_image"].size > 0,
+                                       self.arrays["d_translation"].size > 0)
+        return apply_gpu_grads(sc, self.desc, self, g)
 
     def __getitem__(self, k):
         return self.arrays[k]
