@@ -9,6 +9,7 @@ whole reference vector instead of its own norm.
 """
 import numpy as np
 
+from pydvg.grad import DLinearGradient, DSolid
 from pydvg.scene import GRAD_FIELDS
 
 FLOOR = 1e-6
@@ -49,3 +50,32 @@ def assert_grads(g, gref, tol, fields=GRAD_FIELDS, label=""):
           + ", ".join(f"{k} {v:.3g}" for k, v in per.items()), flush=True)
     assert total <= tol and not bad, f"{label}: fields beyond {tol}: {bad} (total {total:.3g})"
     return total, per
+
+
+def tree_leaves(t):
+    """Every leaf of a SceneGrad as one f64 vector, in declaration order (per
+    shape the geometry's fields, transform, stroke width; per group the
+    shape_to_canvas, fill, stroke; then filter radius, background, background
+    image, translation)."""
+    out = []
+    for s in t.shapes:
+        g = s.geometry
+        for v in vars(g).values():
+            if v is not None:
+                out.append(np.asarray(v, np.float64).ravel())
+        out += [np.asarray(s.transform, np.float64).ravel(), np.array([s.stroke_width])]
+    for g in t.shape_groups:
+        out.append(np.asarray(g.shape_to_canvas, np.float64).ravel())
+        for p in (g.fill, g.stroke):
+            if p is None:
+                continue
+            if isinstance(p, DSolid):
+                out.append(np.array(p.color, np.float64))
+            else:
+                for v in (p.start, p.end) if isinstance(p, DLinearGradient) else (p.center, p.radius):
+                    out.append(np.asarray(v, np.float64))
+                for st in p.stops:
+                    out.append(np.array([st.offset, *st.color], np.float64))
+    out += [np.array([t.filter.radius]), np.array(t.background, np.float64)]
+    out += [np.asarray(a, np.float64).ravel() for a in (t.background_image, t.translation) if a is not None]
+    return np.concatenate(out)

@@ -8,6 +8,7 @@ to_diffvg_layout must carry the scene's path metadata.
 """
 import numpy as np
 
+from parity_util import tree_leaves
 from pydvg import abi
 from pydvg.grad import DCircle, DEllipse, DLinearGradient, DPath, DRadialGradient, DRect, DSolid, SceneGrad
 from pydvg.scene import (Circle, Ellipse, FlatGrads, GradientStop, LinearGradient, Path, RadialGradient, Rect,
@@ -98,31 +99,6 @@ def test_apply_gpu_grads_places_every_field():
     assert np.array_equal(t.translation, f["d_translation"])
 
 
-def _leaves(t):
-    out = []
-    for s in t.shapes:
-        g = s.geometry
-        for v in vars(g).values():
-            if v is not None:
-                out.append(np.asarray(v, np.float64).ravel())
-        out += [np.asarray(s.transform, np.float64).ravel(), np.array([s.stroke_width])]
-    for g in t.shape_groups:
-        out.append(np.asarray(g.shape_to_canvas, np.float64).ravel())
-        for p in (g.fill, g.stroke):
-            if p is None:
-                continue
-            if isinstance(p, DSolid):
-                out.append(np.array(p.color, np.float64))
-            else:
-                for v in (p.start, p.end) if isinstance(p, DLinearGradient) else (p.center, p.radius):
-                    out.append(np.asarray(v, np.float64))
-                for st in p.stops:
-                    out.append(np.array([st.offset, *st.color], np.float64))
-    out += [np.array([t.filter.radius]), np.array(t.background, np.float64), t.background_image.astype(np.float64),
-            t.translation.astype(np.float64)]
-    return np.concatenate(out)
-
-
 def test_accumulate_from_adds_the_flat_gradients():
     sc = _scene()
     desc = SceneDesc(sc)
@@ -132,11 +108,11 @@ def test_accumulate_from_adds_the_flat_gradients():
         s.arrays[k][:] = a.arrays[k] + b.arrays[k]
     ta, tb = a.scene_grad(), b.scene_grad()
     ta.accumulate_from(tb)
-    np.testing.assert_allclose(_leaves(ta), _leaves(s.scene_grad()), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(tree_leaves(ta), tree_leaves(s.scene_grad()), rtol=0, atol=1e-6)
     # a zero tree plus a tree is that tree
     z = SceneGrad.zeros_from_scene(sc, True, True)
     z.accumulate_from(tb)
-    assert np.array_equal(_leaves(z), _leaves(tb))
+    assert np.array_equal(tree_leaves(z), tree_leaves(tb))
 
 
 def test_accumulate_from_takes_missing_optional_parts():
