@@ -324,6 +324,36 @@ DVG_DEV int cubic_crit_f64(double A, double B, double C, double e[2]) {
     return 2;
 }
 
+// Extremes over t in [0, 1] of one coordinate of a Bezier segment with
+// control values p[0..n) (n = 2 line, 3 quadratic, 4 cubic), in f64: the
+// ends and the roots of the derivative inside (0, 1).
+DVG_DEV void bezier_range_f64(const double *p, int n, double &lo, double &hi) {
+    lo = fmin(p[0], p[n - 1]);
+    hi = fmax(p[0], p[n - 1]);
+    double A = 0.0, B = 0.0, C = 0.0, e[2];  // p(t) - p[0] = A t^3 + B t^2 + C t
+    int ne = 0;
+    if (n == 3) {
+        B = p[0] - 2.0 * p[1] + p[2];
+        C = 2.0 * (p[1] - p[0]);
+        if (B != 0.0) {
+            e[0] = -C / (2.0 * B);
+            ne = 1;
+        }
+    } else if (n == 4) {
+        A = -p[0] + 3.0 * p[1] - 3.0 * p[2] + p[3];
+        B = 3.0 * p[0] - 6.0 * p[1] + 3.0 * p[2];
+        C = -3.0 * p[0] + 3.0 * p[1];
+        ne = cubic_crit_f64(A, B, C, e);
+    }
+    for (int j = 0; j < ne; ++j) {
+        double t = e[j];
+        if (!(t > 0.0 && t < 1.0)) continue;
+        double v = p[0] + ((A * t + B) * t + C) * t;
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+}
+
 // xs: the segment's four x control values (record slots 24..30), or null.
 DVG_DEV void wind_coef_cubic(float y0, float y1, float y2, float y3, float *o, const float *xs = nullptr) {
     float a = -y0 + 3.0f * y1 - 3.0f * y2 + y3;

@@ -406,12 +406,14 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
         uint32_t k = (uint32_t)cv[0];
         float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
-        // control-point box, from the segment's record
-        float bx0 = wc[15], bx1 = wc[14], by0 = wc[12], by1 = wc[13];
+        // the curve's box from the segment's record (widened: see k_wind_coef)
+        float bx1 = wc[14], by0 = wc[12], by1 = wc[13];
         if (prefilter) {
             // forward.rs:889-937: segments farther than the running minimum
-            // cannot lower it.
-            float ddx = maxf_(maxf_(bx0 - px, 0.f), px - bx1), ddy = maxf_(maxf_(by0 - py, 0.f), py - by1);
+            // cannot lower it (control-point box)
+            float cx0 = fminf(fminf(x0, x1), fminf(x2, x3)), cx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+            float cy0 = fminf(fminf(y0, y1), fminf(y2, y3)), cy1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+            float ddx = maxf_(maxf_(cx0 - px, 0.f), px - cx1), ddy = maxf_(maxf_(cy0 - py, 0.f), py - cy1);
             if (sqrtf(ddx * ddx + ddy * ddy) <= lmin) {
                 float d, t;
                 if (k == 0) d = seg_dist(px, py, x0, y0, x1, y1);
@@ -421,7 +423,7 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
             }
         }
         // forward.rs:938-1018: a +x ray only meets segments spanning py with
-        // some control point right of px.
+        // some point right of px.
         if (py < by0 || py > by1 || px > bx1) continue;
         fill_segment<M>(k, px, py, x0, y0, x1, y1, x2, y2, x3, y3, wc, rule, lw, lc, c.unsure);
     }
@@ -437,8 +439,8 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
 // sample (shape space; a0 / a1: which points the group's root box admits).
 // On a path the second point's winding is the first's when the path is a
 // closed loop whose CPU winding is the curve's winding number (record slot
-// 31 of its first segment, k_shape_topo) and no segment's control box meets
-// the box spanned by the two points: the segment joining them then crosses
+// 31 of its first segment, k_shape_topo) and no segment's curve box (record
+// slots 12..15) meets the box spanned by the two points: the segment joining them then crosses
 // no edge, so both lie in one component of the complement of the curve,
 // where the winding number is constant.  Otherwise each point is tested.
 template <int M>
@@ -3836,10 +3838,47 @@ __global__ void k_wind_coef(const float *__restrict__ curve_data, uint32_t ncurv
         for (int j = 0; j < WIND_STRIDE; ++j) o[j] = 0.f;
     }
     float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
-    o[12] = fminf(fminf(y0, y1), fminf(y2, y3));
-    o[13] = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
-    o[14] = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
-    o[15] = fminf(fminf(x0, x1), fminf(x2, x3));
+    // the ray test's box (slots 12..15: y min, y max, x max, x min): the
+    // curve's own box over t in [0, 1], widened, within the box of the four
+    // stored points.  Widening: the f32 evaluation of a crossing's x and the
+    // f32 coefficients of the CPU's polynomial (1e-6 of the coordinates, plus
+    // 1e-5 px); on cubics also the f32 solve of the forward, whose roots are
+    // off by up to ~1e-5 (2 sqrt|q| + |b / 3a| + 1) in t near the ends, and
+    // whose three-root branch can wrongly open within ~2e-7 |a| (2 |b/a|^3 +
+    // 9 |b c| / a^2 + 27 |d / a|) of a y-extremum (ten times both).  Segments
+    // with an ill-conditioned solve (record bit 4) keep the stored points'
+    // box in y.  A point beyond the box then meets no crossing in any of the
+    // solves, so the box test skips exactly segments that add nothing.
+    float cx0 = fminf(fminf(x0, x1), fminf(x2, x3)), cx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
+    float cy0 = fminf(fminf(y0, y1), fminf(y2, y3)), cy1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+    int np = k == 2u ? 4 : (k == 1u ? 3 : 2);
+    double xs[4] = {x0, x1, x2, x3}, ys[4] = {y0, y1, y2, y3};
+    double xl, xh, yl, yh, sx = 0.0, sy = 0.0;
+    bezier_range_f64(xs, np, xl, xh);
+    bezier_range_f64(ys, np, yl, yh);
+    for (int j = 0; j < np; ++j) {
+        sx += fabs(xs[j]);
+        sy += fabs(ys[j]);
+    }
+    double mx = 1.0e-6 * sx + 1.0e-5, my = 1.0e-6 * sy + 1.0e-5;
+    bool keep_y = false;
+    if (k == 2u) {
+        double a = o[0], b = o[1], c = o[2];
+        if (((uint32_t)o[11] & 16u) || !(fabs(a) > 0.0)) {
+            keep_y = true;
+        } else {
+            double ba = fabs(b / a);
+            double et = 1.0e-4 * (2.0 * sqrt(fabs((double)o[3])) + ba / 3.0 + 1.0);
+            my += et * 3.0 * (fabs(a) + fabs(b) + fabs(c)) +
+                  2.0e-6 * fabs(a) * (2.0 * ba * ba * ba + 9.0 * fabs(b * c) / (a * a) + 27.0 * (sy + fabs(y0)) / fabs(a));
+        }
+    } else if (k == 1u) {
+        my += 1.0e-4 * 4.0 * sy;  // the forward's f32 quadratic, near its extremum
+    }
+    o[12] = keep_y ? cy0 : fmaxf(cy0, (float)(yl - my));
+    o[13] = keep_y ? cy1 : fminf(cy1, (float)(yh + my));
+    o[14] = fminf(cx1, (float)(xh + mx));
+    o[15] = fmaxf(cx0, (float)(xl - mx));
 }
 
 // Slot 31 of a path's first segment record (after k_wind_coef): 1 when the
