@@ -474,13 +474,16 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
            i_sx = addv(pk->shape_xform, SP * 24), i_grd = addv(pk->gradient_data, pk->gradient_data_len * 4),
            i_so = addv(pk->stop_offsets, pk->stop_offsets_len * 4),
            i_sc = addv(pk->stop_colors, pk->stop_offsets_len * 16), i_rng = addv(ranges.data(), ranges.size() * 4),
-           i_toff = addv(nullptr, 0), i_tent = addv(nullptr, 0), i_tcnt = addv(nullptr, 0);
+           i_toff = addv(nullptr, 0), i_tent = addv(nullptr, 0), i_tcnt = addv(nullptr, 0), i_tend = addv(nullptr, 0),
+           i_tbwd = addv(nullptr, 0);
     // tile work buffers (no host source)
     A.srcs[i_toff].second = ((size_t)ntiles + 1) * 4;
     // room for the lists to grow when a device-side update moves shapes
     uint64_t tile_cap = std::max<uint64_t>(2 * n_entries + 4ull * ntiles, 1);
     A.srcs[i_tent].second = tile_cap * 4;
     A.srcs[i_tcnt].second = (size_t)std::max(ntiles, 1u) * 4;
+    A.srcs[i_tend].second = (size_t)std::max(ntiles, 1u) * 4;
+    A.srcs[i_tbwd].second = tile_cap * 4;
     // recompute offsets after resizing the sourceless entries
     {
         size_t t = 0;
@@ -608,6 +611,8 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.tiles_y = tiles_y;
     D.tile_offsets = (dvgk::kuint *)at(i_toff);
     D.tile_entries = (dvgk::kuint *)at(i_tent);
+    D.tile_end = (dvgk::kuint *)at(i_tend);
+    D.bwd_entries = (dvgk::kuint *)at(i_tbwd);
     if (bp) {
         D.num_shapes = S;
         D.prepared_to_scene = (dvgk::kuint *)at(i_p2s);
@@ -686,6 +691,8 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         dvgk::launch_scan((const uint32_t *)at(i_tcnt), ntiles, (uint32_t *)at(i_toff), ctx->stream);
         dvgk::launch_bin_write((const float *)at(i_rng), G, tiles_x, tiles_y, (const uint32_t *)at(i_toff),
                                (uint32_t *)at(i_tent), ctx->stream);
+        dvgk::launch_tile_cull(D, (const uint32_t *)at(i_tent), (uint32_t *)at(i_tbwd), (uint32_t *)at(i_tend),
+                               !D.prefilter, ctx->stream);
         e = hipGetLastError();
         if (e != hipSuccess) {
             (void)hipFree(ds->arena);

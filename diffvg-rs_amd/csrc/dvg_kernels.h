@@ -61,6 +61,10 @@ struct DevScene {
     // tiles
     uint32_t tiles_x, tiles_y;
     kuint *tile_offsets, *tile_entries;
+    // the backward walks' lists: bwd_entries from tile_offsets[t] to
+    // tile_end[t], each tile's list without the groups that add nothing to
+    // the backward's inside tests there (k_tile_cull)
+    kuint *bwd_entries, *tile_end;
     // backward pack (scene-indexed)
     uint32_t num_shapes;
     kuint *prepared_to_scene;
@@ -124,6 +128,11 @@ struct DevGrads {
 };
 
 // Kernel launchers (dvg_kernels.hip).  All asynchronous on `stream`.
+// The backward's tile lists: each list of entries copied to out without the
+// groups that add nothing to the backward's inside tests in the tile (cull
+// false: every entry), each list's end in tile_end.
+void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, bool cull,
+                      hipStream_t stream);
 void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
                       hipStream_t stream);
 void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream);

@@ -1407,8 +1407,8 @@ DVG_DEV void interior_unit(const DevScene &S, uint32_t unit, const float *__rest
     uint32_t *ids = s_ids + threadIdx.x;
     uint32_t upt = units_per_tile(S);
     uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
-    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
-    const uint32_t *list = S.tile_entries + lb;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_end[tile];
+    const uint32_t *list = S.bwd_entries + lb;
     uint32_t n = le - lb;
     bool use_mask = n <= MASK_GROUPS;
     bool use_gmask = !use_mask && gmask != nullptr && n <= gcap;
@@ -2199,8 +2199,8 @@ DVG_DEV void prefiltered_unit(const DevScene &S, uint32_t unit, const float *__r
     float *cov = s_cov + threadIdx.x;
     uint32_t upt = units_per_tile(S);
     uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
-    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
-    const uint32_t *list = S.tile_entries + lb;
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_end[tile];
+    const uint32_t *list = S.bwd_entries + lb;
     uint32_t n = le - lb;
     stage_build(S, list, n, s_off, s_stage, &s_total);
     float acc_bg[4] = {0.f, 0.f, 0.f, 0.f}, acc_fr = 0.f;
@@ -2589,13 +2589,13 @@ template <int M>
 DVG_DEV void edge_walk_pair(const DevScene &S, uint32_t tile, float px0, float py0, float px1, float py1, float4 bg,
                             uint32_t eq_group, uint32_t eq_prep, float4 &col0, float4 &col1, bool &hit0, bool &hit1,
                             bool &uns) {
-    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    uint32_t lb = S.tile_offsets[tile], le = S.tile_end[tile];
     float4 acc0 = bg, acc1 = bg;
     uint32_t nfr0 = 0, nfr1 = 0;
     bool et0 = false, et1 = false;  // hit set while collecting fragments
     bool h00 = false, h01 = true, h10 = false, h11 = true;
     for (uint32_t e = lb; e < le; ++e) {
-        uint32_t g = S.tile_entries[e];
+        uint32_t g = S.bwd_entries[e];
         bool st0, fi0, st1, fi1;
         group_frags_pair<M>(S, g, px0, py0, px1, py1, st0, fi0, st1, fi1, uns);
         bool eq = g == eq_group;
@@ -3917,6 +3917,155 @@ __global__ void k_shape_topo(const float *__restrict__ shape_data, uint32_t npre
     curve_wind[(size_t)off * WIND_STRIDE + 31] = ok ? 1.f : 0.f;
 }
 
+// ---- tile-list culling ----------------------------------------------------------
+// A tile list holds every group whose padded box meets the tile.  A group
+// whose curves all miss the tile's padded region and whose fill is outside
+// there adds nothing to the backward's inside tests anywhere in the region:
+// the winding number of a closed loop is constant off its curve, and those
+// tests find the winding number (f64, or f32 where f64 would decide the
+// same) -- the f64 winding at the region's centre.  k_tile_cull copies each
+// list without such entries into the backward's lists (bwd_entries, from
+// tile_offsets[t] to tile_end[t]) for the colour adjoint, the boundary pass
+// and the prefiltered adjoint.  The forward keeps the full lists: its f32
+// solve can count crossings of ill-conditioned segments far from the curve,
+// and the images follow it there.  Kept: every group with a stroke paint or
+// a shape other than a closed-loop path (record slot 31 of the path's first
+// segment), and every entry of a prefiltered scene (distance coverage).
+
+// Whether a Bezier segment (kind k: 0 line, 1 quadratic, 2 cubic) may meet
+// the box: false when the control box of each of its 32 equal-t pieces
+// misses it (a piece on [t0, t1] has control points P(t0), P(t0) + h P'(t0)
+// / 3, P(t1) - h P'(t1) / 3, P(t1), h = t1 - t0, exact for degree <= 3).
+DVG_DEV bool bezier_meets_box(uint32_t k, const float *xs, const float *ys, float bx0, float by0, float bx1,
+                              float by1) {
+    // power basis P(t) = p0 + c1 t + c2 t^2 + c3 t^3
+    float cx1, cx2 = 0.f, cx3 = 0.f, cy1, cy2 = 0.f, cy3 = 0.f;
+    if (k == 0u) {
+        cx1 = xs[1] - xs[0]; cy1 = ys[1] - ys[0];
+    } else if (k == 1u) {
+        cx1 = 2.0f * (xs[1] - xs[0]); cx2 = xs[0] - 2.0f * xs[1] + xs[2];
+        cy1 = 2.0f * (ys[1] - ys[0]); cy2 = ys[0] - 2.0f * ys[1] + ys[2];
+    } else {
+        cx1 = 3.0f * (xs[1] - xs[0]); cx2 = 3.0f * (xs[0] - 2.0f * xs[1] + xs[2]);
+        cx3 = -xs[0] + 3.0f * (xs[1] - xs[2]) + xs[3];
+        cy1 = 3.0f * (ys[1] - ys[0]); cy2 = 3.0f * (ys[0] - 2.0f * ys[1] + ys[2]);
+        cy3 = -ys[0] + 3.0f * (ys[1] - ys[2]) + ys[3];
+    }
+    constexpr int N = 32;
+    const float h = 1.0f / (float)N, h3 = h / 3.0f;
+    float x0 = xs[0], y0 = ys[0], dx0 = cx1, dy0 = cy1;
+    for (int i = 1; i <= N; ++i) {
+        float t = (float)i * h;
+        float x1 = xs[0] + ((cx3 * t + cx2) * t + cx1) * t, y1 = ys[0] + ((cy3 * t + cy2) * t + cy1) * t;
+        float dx1 = (3.0f * cx3 * t + 2.0f * cx2) * t + cx1, dy1 = (3.0f * cy3 * t + 2.0f * cy2) * t + cy1;
+        float qx1 = x0 + h3 * dx0, qx2 = x1 - h3 * dx1, qy1 = y0 + h3 * dy0, qy2 = y1 - h3 * dy1;
+        float mnx = fminf(fminf(x0, x1), fminf(qx1, qx2)), mxx = fmaxf(fmaxf(x0, x1), fmaxf(qx1, qx2));
+        float mny = fminf(fminf(y0, y1), fminf(qy1, qy2)), mxy = fmaxf(fmaxf(y0, y1), fmaxf(qy1, qy2));
+        if (!(mnx > bx1 || mxx < bx0 || mny > by1 || mxy < by0)) return true;
+        x0 = x1; y0 = y1; dx0 = dx1; dy0 = dy1;
+    }
+    return false;
+}
+
+DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
+    const float *gd = S.group_data + g * 16u;
+    uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
+    if (sk != DVG_PAINT_NONE) return false;
+    if (fk == DVG_PAINT_NONE) return true;  // no paint: every walk skips it
+    // the region: the tile padded as binning pads group boxes, plus a pixel;
+    // on the canvas border 4 px + 2e-4 of the canvas further out (points
+    // clamped into border tiles from beyond the canvas)
+    uint32_t tx = t % S.tiles_x, ty = t / S.tiles_x;
+    float pad = tile_pad(S.W, S.H) + 1.0f;
+    float far = pad + 4.0f + 2.0e-4f * (float)(S.W > S.H ? S.W : S.H);
+    float rx0 = (float)(tx * TILE) - (tx == 0u ? far : pad);
+    float rx1 = (float)((tx + 1u) * TILE) + (tx + 1u == S.tiles_x ? far : pad);
+    float ry0 = (float)(ty * TILE) - (ty == 0u ? far : pad);
+    float ry1 = (float)((ty + 1u) * TILE) + (ty + 1u == S.tiles_y ? far : pad);
+    float qx[4] = {rx0, rx1, rx0, rx1}, qy[4] = {ry0, ry0, ry1, ry1};
+    float lx[4], ly[4], lcx, lcy;
+    for (int c = 0; c < 4; ++c) group_local(S, g, qx[c], qy[c], lx[c], ly[c]);
+    group_local(S, g, 0.5f * (rx0 + rx1), 0.5f * (ry0 + ry1), lcx, lcy);
+    uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
+    float wsum = 0.f, cr = 0.f;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        uint32_t si = (uint32_t)S.group_shapes[off + i];
+        const float *sd = S.shape_data + si * 15u;
+        if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return false;
+        uint32_t co = (uint32_t)sd[12], cc = (uint32_t)sd[13];
+        if (cc == 0u) continue;  // no segment: winding 0 everywhere
+        if (!(S.curve_wind[(size_t)co * WIND_STRIDE + 31] > 0.f)) return false;
+        // the region's box in shape space, widened for rounding
+        const float *xf = S.shape_xform + si * 6u;
+        float sx0 = __builtin_inff(), sy0 = __builtin_inff(), sx1 = -__builtin_inff(), sy1 = -__builtin_inff();
+        for (int c = 0; c < 4; ++c) {
+            float x = xf[0] * lx[c] + xf[1] * ly[c] + xf[2], y = xf[3] * lx[c] + xf[4] * ly[c] + xf[5];
+            sx0 = fminf(sx0, x); sx1 = fmaxf(sx1, x); sy0 = fminf(sy0, y); sy1 = fmaxf(sy1, y);
+        }
+        float m = 1.0e-4f * fmaxf(fmaxf(fabsf(sx0), fabsf(sx1)), fmaxf(fabsf(sy0), fabsf(sy1))) + 1.0e-3f;
+        sx0 -= m; sy0 -= m; sx1 += m; sy1 += m;
+        if (!(sx0 <= sx1 && sy0 <= sy1)) return false;  // non-finite transform
+        float scx = xf[0] * lcx + xf[1] * lcy + xf[2], scy = xf[3] * lcx + xf[4] * lcy + xf[5];
+        float lw = 0.f, lc = 0.f;
+        for (uint32_t s = 0; s < cc; ++s) {
+            const float *cv = S.curve_data + (size_t)(co + s) * 13u;
+            const float *wc = S.curve_wind + (size_t)(co + s) * WIND_STRIDE;
+            uint32_t k = (uint32_t)cv[0];
+            float xs[4] = {cv[1], cv[3], cv[5], cv[7]}, ys[4] = {cv[2], cv[4], cv[6], cv[8]};
+            bool near = !(wc[15] > sx1 || wc[14] < sx0 || wc[12] > sy1 || wc[13] < sy0);
+            if (near && bezier_meets_box(k, xs, ys, sx0, sy0, sx1, sy1)) return false;
+            if (k == 0u) {
+                winding_line_cpu(scx, scy, xs[0], ys[0], xs[1], ys[1], rule, lw, lc);
+            } else {
+                int n = (k == 1u) ? winding_quad_f64(scx, scy, xs, ys) : winding_cubic_f64(scx, scy, xs, ys);
+                if (rule == 1u) {
+                    if (n & 1) lc = 1.f - lc;
+                } else {
+                    lw += (float)n;
+                }
+            }
+        }
+        if (rule == 1u) {
+            if (lc > 0.f) cr = 1.f - cr;
+        } else {
+            wsum += lw;
+        }
+    }
+    return !((rule == 1u) ? cr > 0.f : wsum != 0.f);
+}
+
+// One wave per tile, 64 entries at a time; a kept entry goes to the count of
+// kept entries before it (the lists keep their group order).
+__global__ __launch_bounds__(BLOCK) void k_tile_cull(DevScene S, const uint32_t *__restrict__ entries,
+                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ tile_end,
+                                                     uint32_t cull) {
+    uint32_t t = blockIdx.x * BIN_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
+    if (t >= S.tiles_x * S.tiles_y) return;  // (whole waves)
+    uint32_t lb = S.tile_offsets[t], le = S.tile_offsets[t + 1];
+    uint32_t w = lb;
+    for (uint32_t e0 = lb; e0 < le; e0 += 64u) {
+        uint32_t e = e0 + lane;
+        uint32_t g = 0u;
+        bool keep = false;
+        if (e < le) {
+            g = entries[e];
+            keep = !cull || !tile_entry_drops(S, t, g);
+        }
+        uint64_t m = __ballot(keep);
+        if (keep) out[w + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = g;
+        w += (uint32_t)__popcll(m);
+    }
+    if (lane == 0u) tile_end[t] = w;
+}
+
+void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, bool cull,
+                      hipStream_t stream) {
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    if (ntiles == 0) return;
+    hipLaunchKernelGGL(k_tile_cull, dim3((ntiles + BIN_TILES_PER_WG - 1) / BIN_TILES_PER_WG), dim3(BLOCK), 0, stream,
+                       S, entries, out, tile_end, cull ? 1u : 0u);
+}
+
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
                              const float *__restrict__ pts, uint32_t ncurves) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -4536,6 +4685,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
                            P.tile_cap, P.overflow);
         launch_bin_write(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, offsets,
                          entries, stream);
+        launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), !S.prefilter, stream);
     }
     if (S.n_cdf > 0 && !S.prefilter) {
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
