@@ -130,9 +130,10 @@ struct DevGrads {
 // Kernel launchers (dvg_kernels.hip).  All asynchronous on `stream`.
 // The backward's tile lists: each list of entries copied to out without the
 // groups that add nothing to the backward's inside tests in the tile (cull
-// false: every entry), each list's end in tile_end.
-void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, bool cull,
-                      hipStream_t stream);
+// false: every entry), each list's end in tile_end.  cap: the capacity of
+// entries and out (at least tile_offsets[ntiles]).
+void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t cap,
+                      bool cull, hipStream_t stream);
 void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
                       hipStream_t stream);
 void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream);

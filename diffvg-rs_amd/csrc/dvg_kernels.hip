@@ -4034,23 +4034,35 @@ DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     return !((rule == 1u) ? cr > 0.f : wsum != 0.f);
 }
 
-// One wave per tile, 64 entries at a time; a kept entry goes to the count of
-// kept entries before it (the lists keep their group order).
-__global__ __launch_bounds__(BLOCK) void k_tile_cull(DevScene S, const uint32_t *__restrict__ entries,
-                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ tile_end,
-                                                     uint32_t cull) {
+// Pass 1, one lane per entry of the full lists: 1 in out[e] when entry e is
+// kept.  Pass 2, one wave per tile, 64 entries at a time: a kept entry goes
+// to the count of kept entries before it (the lists keep their group order;
+// a write never passes a flag not yet read).
+__global__ __launch_bounds__(BLOCK) void k_tile_keep(DevScene S, const uint32_t *__restrict__ entries,
+                                                     uint32_t *__restrict__ out, uint32_t cull) {
+    uint32_t ntiles = S.tiles_x * S.tiles_y;
+    uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= S.tile_offsets[ntiles]) return;
+    // the tile holding entry e: the last t with tile_offsets[t] <= e
+    uint32_t lo = 0, hi = ntiles;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (S.tile_offsets[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    out[e] = (!cull || !tile_entry_drops(S, lo, entries[e])) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tile_compact(DevScene S, const uint32_t *__restrict__ entries,
+                                                        uint32_t *__restrict__ out, uint32_t *__restrict__ tile_end) {
     uint32_t t = blockIdx.x * BIN_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
     if (t >= S.tiles_x * S.tiles_y) return;  // (whole waves)
     uint32_t lb = S.tile_offsets[t], le = S.tile_offsets[t + 1];
     uint32_t w = lb;
     for (uint32_t e0 = lb; e0 < le; e0 += 64u) {
         uint32_t e = e0 + lane;
-        uint32_t g = 0u;
-        bool keep = false;
-        if (e < le) {
-            g = entries[e];
-            keep = !cull || !tile_entry_drops(S, t, g);
-        }
+        bool keep = e < le && out[e] != 0u;
+        uint32_t g = keep ? entries[e] : 0u;
         uint64_t m = __ballot(keep);
         if (keep) out[w + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = g;
         w += (uint32_t)__popcll(m);
@@ -4058,12 +4070,15 @@ __global__ __launch_bounds__(BLOCK) void k_tile_cull(DevScene S, const uint32_t 
     if (lane == 0u) tile_end[t] = w;
 }
 
-void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, bool cull,
-                      hipStream_t stream) {
+void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t cap,
+                      bool cull, hipStream_t stream) {
     uint32_t ntiles = S.tiles_x * S.tiles_y;
     if (ntiles == 0) return;
-    hipLaunchKernelGGL(k_tile_cull, dim3((ntiles + BIN_TILES_PER_WG - 1) / BIN_TILES_PER_WG), dim3(BLOCK), 0, stream,
-                       S, entries, out, tile_end, cull ? 1u : 0u);
+    if (cap > 0)
+        hipLaunchKernelGGL(k_tile_keep, dim3((cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, S, entries, out,
+                           cull ? 1u : 0u);
+    hipLaunchKernelGGL(k_tile_compact, dim3((ntiles + BIN_TILES_PER_WG - 1) / BIN_TILES_PER_WG), dim3(BLOCK), 0,
+                       stream, S, entries, out, tile_end);
 }
 
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
@@ -4685,7 +4700,8 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
                            P.tile_cap, P.overflow);
         launch_bin_write(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, offsets,
                          entries, stream);
-        launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), !S.prefilter, stream);
+        launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), P.tile_cap, !S.prefilter,
+                         stream);
     }
     if (S.n_cdf > 0 && !S.prefilter) {
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
