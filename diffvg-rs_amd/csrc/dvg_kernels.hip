@@ -41,6 +41,13 @@ namespace dvgk {
 #ifndef OCC_BOUNDARY
 #define OCC_BOUNDARY 8
 #endif
+// The redo of uncertain boundary samples runs beside the colour adjoint on a
+// second stream; its own occupancy sets how much of each SIMD it takes
+// (1024x1024 step: 4.95 ms at 8, 4.86 at 7 with the colour adjoint 1.24 ->
+// 1.14 ms beside it, 5.14 at 6 and 5.11 at 4, where the redo outlasts it).
+#ifndef OCC_REDO
+#define OCC_REDO 7
+#endif
 // The prefiltered adjoint (closest-point adjoints, ~200 VGPRs unconstrained)
 // is latency bound at 2 waves; 3 measured best on the 1024x1024 prefiltered
 // scene (25.9 -> 20.4 ms; 4-6 spill enough to lose it again).
@@ -3110,7 +3117,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUND
 
 // The samples k_bnd_eval left to the redo, unit by unit, with the f64
 // windings where the f32 ones are uncertain.
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_BOUNDARY))) void k_bnd_redo(DevScene S, const float *__restrict__ weight,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC_REDO))) void k_bnd_redo(DevScene S, const float *__restrict__ weight,
                                                     const float *__restrict__ d_render, DevGrads G,
                                                     const uint32_t *__restrict__ offsets, uint32_t nb, FlagSlots fs) {
     bnd_eval_unit<COV_EXACT, true>(S, blockIdx.x, weight, d_render, G, offsets, nullptr, nb, fs);
