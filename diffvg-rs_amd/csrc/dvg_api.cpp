@@ -475,7 +475,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
            i_so = addv(pk->stop_offsets, pk->stop_offsets_len * 4),
            i_sc = addv(pk->stop_colors, pk->stop_offsets_len * 16), i_rng = addv(ranges.data(), ranges.size() * 4),
            i_toff = addv(nullptr, 0), i_tent = addv(nullptr, 0), i_tcnt = addv(nullptr, 0), i_tend = addv(nullptr, 0),
-           i_tbwd = addv(nullptr, 0);
+           i_tbwd = addv(nullptr, 0), i_fend = addv(nullptr, 0);
     // tile work buffers (no host source)
     A.srcs[i_toff].second = ((size_t)ntiles + 1) * 4;
     // room for the lists to grow when a device-side update moves shapes
@@ -484,6 +484,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     A.srcs[i_tcnt].second = (size_t)std::max(ntiles, 1u) * 4;
     A.srcs[i_tend].second = (size_t)std::max(ntiles, 1u) * 4;
     A.srcs[i_tbwd].second = tile_cap * 4;
+    A.srcs[i_fend].second = (size_t)std::max(ntiles, 1u) * 4;
     // recompute offsets after resizing the sourceless entries
     {
         size_t t = 0;
@@ -613,6 +614,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.tile_entries = (dvgk::kuint *)at(i_tent);
     D.tile_end = (dvgk::kuint *)at(i_tend);
     D.bwd_entries = (dvgk::kuint *)at(i_tbwd);
+    D.fwd_end = (dvgk::kuint *)at(i_fend);
     if (bp) {
         D.num_shapes = S;
         D.prepared_to_scene = (dvgk::kuint *)at(i_p2s);
@@ -691,8 +693,8 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         dvgk::launch_scan((const uint32_t *)at(i_tcnt), ntiles, (uint32_t *)at(i_toff), ctx->stream);
         dvgk::launch_bin_write((const float *)at(i_rng), G, tiles_x, tiles_y, (const uint32_t *)at(i_toff),
                                (uint32_t *)at(i_tent), ctx->stream);
-        dvgk::launch_tile_cull(D, (const uint32_t *)at(i_tent), (uint32_t *)at(i_tbwd), (uint32_t *)at(i_tend),
-                               (uint32_t)tile_cap, !D.prefilter, ctx->stream);
+        dvgk::launch_tile_cull(D, (uint32_t *)at(i_tent), (uint32_t *)at(i_tbwd), (uint32_t *)at(i_tend),
+                               (uint32_t *)at(i_fend), (uint32_t)tile_cap, !D.prefilter, ctx->stream);
         e = hipGetLastError();
         if (e != hipSuccess) {
             (void)hipFree(ds->arena);

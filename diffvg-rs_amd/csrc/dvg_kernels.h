@@ -63,8 +63,10 @@ struct DevScene {
     kuint *tile_offsets, *tile_entries;
     // the backward walks' lists: bwd_entries from tile_offsets[t] to
     // tile_end[t], each tile's list without the groups that add nothing to
-    // the backward's inside tests there (k_tile_cull)
-    kuint *bwd_entries, *tile_end;
+    // the backward's inside tests there; the forward's: tile_entries from
+    // tile_offsets[t] to fwd_end[t], without those of them whose segments
+    // the forward's f32 solve settles exactly (k_tile_keep, k_tile_compact)
+    kuint *bwd_entries, *tile_end, *fwd_end;
     // backward pack (scene-indexed)
     uint32_t num_shapes;
     kuint *prepared_to_scene;
@@ -128,12 +130,14 @@ struct DevGrads {
 };
 
 // Kernel launchers (dvg_kernels.hip).  All asynchronous on `stream`.
-// The backward's tile lists: each list of entries copied to out without the
-// groups that add nothing to the backward's inside tests in the tile (cull
-// false: every entry), each list's end in tile_end.  cap: the capacity of
-// entries and out (at least tile_offsets[ntiles]).
-void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t cap,
-                      bool cull, hipStream_t stream);
+// The backward's and the forward's tile lists from the full lists in
+// entries: the backward's copied to out without the groups that add nothing
+// to the backward's inside tests in the tile, ends in tile_end; the
+// forward's compacted in place, ends in fwd_end (cull false: every entry
+// kept).  cap: the capacity of entries and out (at least
+// tile_offsets[ntiles]).
+void launch_tile_cull(const DevScene &S, uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t *fwd_end,
+                      uint32_t cap, bool cull, hipStream_t stream);
 void launch_bin_count(const float *group_aabb, uint32_t G, uint32_t tiles_x, uint32_t tiles_y, uint32_t *counts,
                       hipStream_t stream);
 void launch_scan(const uint32_t *counts, uint32_t n, uint32_t *offsets, hipStream_t stream);

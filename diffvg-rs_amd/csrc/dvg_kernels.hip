@@ -746,7 +746,7 @@ DVG_DEV void forward_unit(const DevScene &S, uint32_t unit, const float *__restr
     __shared__ float lacc[4][SPLAT_EDGE * SPLAT_EDGE];
     uint32_t upt = units_per_tile(S);
     uint32_t tile = unit / upt, s0 = (unit - tile * upt) * S.unit_samples;
-    uint32_t lb = S.tile_offsets[tile], le = S.tile_offsets[tile + 1];
+    uint32_t lb = S.tile_offsets[tile], le = S.fwd_end[tile];
     const uint32_t *list = S.tile_entries + lb;
     uint32_t n = le - lb;
     uint32_t total = min(TILE * TILE * S.spp, s0 + S.unit_samples);
@@ -3930,14 +3930,18 @@ __global__ void k_shape_topo(const float *__restrict__ shape_data, uint32_t npre
 // there adds nothing to the backward's inside tests anywhere in the region:
 // the winding number of a closed loop is constant off its curve, and those
 // tests find the winding number (f64, or f32 where f64 would decide the
-// same) -- the f64 winding at the region's centre.  k_tile_cull copies each
-// list without such entries into the backward's lists (bwd_entries, from
-// tile_offsets[t] to tile_end[t]) for the colour adjoint, the boundary pass
-// and the prefiltered adjoint.  The forward keeps the full lists: its f32
-// solve can count crossings of ill-conditioned segments far from the curve,
-// and the images follow it there.  Kept: every group with a stroke paint or
-// a shape other than a closed-loop path (record slot 31 of the path's first
-// segment), and every entry of a prefiltered scene (distance coverage).
+// same) -- the f64 winding at the region's centre.  The backward's lists
+// (bwd_entries, from tile_offsets[t] to tile_end[t], for the colour adjoint,
+// the boundary pass and the prefiltered adjoint) go without such entries.
+// The forward's f32 solve counts crossings of ill-conditioned segments (record
+// bit 4) and quadratic segments wrongly over whole bands of rows far from
+// their curve, and the images follow it there; on lines and well-conditioned
+// cubics it can go wrong only within ~1e-4 px of a segment end's y.  So the
+// forward's lists (the full lists compacted in place, to fwd_end[t]) drop an
+// entry only when every segment of the group is a line or a well-conditioned
+// cubic.  Kept in both: every group with a stroke paint or a shape other
+// than a closed-loop path (record slot 31 of the path's first segment), and
+// every entry of a prefiltered scene (distance coverage).
 
 // Whether a Bezier segment (kind k: 0 line, 1 quadratic, 2 cubic) may meet
 // the box: false when the control box of each of its 32 equal-t pieces
@@ -3974,11 +3978,12 @@ DVG_DEV bool bezier_meets_box(uint32_t k, const float *xs, const float *ys, floa
     return false;
 }
 
-DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
+// 0 keep; 1 drop from the backward's list; 3 drop from the forward's too.
+DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     const float *gd = S.group_data + g * 16u;
     uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-    if (sk != DVG_PAINT_NONE) return false;
-    if (fk == DVG_PAINT_NONE) return true;  // no paint: every walk skips it
+    if (sk != DVG_PAINT_NONE) return 0u;
+    if (fk == DVG_PAINT_NONE) return 3u;  // no paint: every walk skips it
     // the region: the tile padded as binning pads group boxes, plus a pixel;
     // on the canvas border 4 px + 2e-4 of the canvas further out (points
     // clamped into border tiles from beyond the canvas)
@@ -3995,13 +4000,14 @@ DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     group_local(S, g, 0.5f * (rx0 + rx1), 0.5f * (ry0 + ry1), lcx, lcy);
     uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
     float wsum = 0.f, cr = 0.f;
+    bool fwd = true;  // every segment a line or a well-conditioned cubic
     for (uint32_t i = 0; i < cnt; ++i) {
         uint32_t si = (uint32_t)S.group_shapes[off + i];
         const float *sd = S.shape_data + si * 15u;
-        if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return false;
+        if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return 0u;
         uint32_t co = (uint32_t)sd[12], cc = (uint32_t)sd[13];
         if (cc == 0u) continue;  // no segment: winding 0 everywhere
-        if (!(S.curve_wind[(size_t)co * WIND_STRIDE + 31] > 0.f)) return false;
+        if (!(S.curve_wind[(size_t)co * WIND_STRIDE + 31] > 0.f)) return 0u;
         // the region's box in shape space, widened for rounding
         const float *xf = S.shape_xform + si * 6u;
         float sx0 = __builtin_inff(), sy0 = __builtin_inff(), sx1 = -__builtin_inff(), sy1 = -__builtin_inff();
@@ -4011,16 +4017,17 @@ DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
         }
         float m = 1.0e-4f * fmaxf(fmaxf(fabsf(sx0), fabsf(sx1)), fmaxf(fabsf(sy0), fabsf(sy1))) + 1.0e-3f;
         sx0 -= m; sy0 -= m; sx1 += m; sy1 += m;
-        if (!(sx0 <= sx1 && sy0 <= sy1)) return false;  // non-finite transform
+        if (!(sx0 <= sx1 && sy0 <= sy1)) return 0u;  // non-finite transform
         float scx = xf[0] * lcx + xf[1] * lcy + xf[2], scy = xf[3] * lcx + xf[4] * lcy + xf[5];
         float lw = 0.f, lc = 0.f;
         for (uint32_t s = 0; s < cc; ++s) {
             const float *cv = S.curve_data + (size_t)(co + s) * 13u;
             const float *wc = S.curve_wind + (size_t)(co + s) * WIND_STRIDE;
             uint32_t k = (uint32_t)cv[0];
+            if (k == 1u || (k == 2u && ((uint32_t)wc[11] & 16u))) fwd = false;
             float xs[4] = {cv[1], cv[3], cv[5], cv[7]}, ys[4] = {cv[2], cv[4], cv[6], cv[8]};
             bool near = !(wc[15] > sx1 || wc[14] < sx0 || wc[12] > sy1 || wc[13] < sy0);
-            if (near && bezier_meets_box(k, xs, ys, sx0, sy0, sx1, sy1)) return false;
+            if (near && bezier_meets_box(k, xs, ys, sx0, sy0, sx1, sy1)) return 0u;
             if (k == 0u) {
                 winding_line_cpu(scx, scy, xs[0], ys[0], xs[1], ys[1], rule, lw, lc);
             } else {
@@ -4038,7 +4045,8 @@ DVG_DEV bool tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
             wsum += lw;
         }
     }
-    return !((rule == 1u) ? cr > 0.f : wsum != 0.f);
+    if ((rule == 1u) ? cr > 0.f : wsum != 0.f) return 0u;
+    return fwd ? 3u : 1u;
 }
 
 // Pass 1, one lane per entry of the full lists: 1 in out[e] when entry e is
@@ -4057,35 +4065,43 @@ __global__ __launch_bounds__(BLOCK) void k_tile_keep(DevScene S, const uint32_t 
         if (S.tile_offsets[mid] <= e) lo = mid;
         else hi = mid;
     }
-    out[e] = (!cull || !tile_entry_drops(S, lo, entries[e])) ? 1u : 0u;
+    out[e] = cull ? tile_entry_drops(S, lo, entries[e]) : 0u;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_tile_compact(DevScene S, const uint32_t *__restrict__ entries,
-                                                        uint32_t *__restrict__ out, uint32_t *__restrict__ tile_end) {
+__global__ __launch_bounds__(BLOCK) void k_tile_compact(DevScene S, uint32_t *__restrict__ entries,
+                                                        uint32_t *__restrict__ out, uint32_t *__restrict__ tile_end,
+                                                        uint32_t *__restrict__ fwd_end) {
     uint32_t t = blockIdx.x * BIN_TILES_PER_WG + threadIdx.x / 64u, lane = threadIdx.x & 63u;
     if (t >= S.tiles_x * S.tiles_y) return;  // (whole waves)
     uint32_t lb = S.tile_offsets[t], le = S.tile_offsets[t + 1];
-    uint32_t w = lb;
+    uint32_t wb = lb, wf = lb;
     for (uint32_t e0 = lb; e0 < le; e0 += 64u) {
         uint32_t e = e0 + lane;
-        bool keep = e < le && out[e] != 0u;
-        uint32_t g = keep ? entries[e] : 0u;
-        uint64_t m = __ballot(keep);
-        if (keep) out[w + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = g;
-        w += (uint32_t)__popcll(m);
+        uint32_t drop = e < le ? out[e] : 3u;
+        uint32_t g = e < le ? entries[e] : 0u;
+        bool kb = !(drop & 1u), kf = !(drop & 2u);
+        uint64_t mb = __ballot(kb), mf = __ballot(kf);
+        uint64_t below = (1ull << lane) - 1ull;
+        if (kb) out[wb + (uint32_t)__popcll(mb & below)] = g;
+        if (kf) entries[wf + (uint32_t)__popcll(mf & below)] = g;
+        wb += (uint32_t)__popcll(mb);
+        wf += (uint32_t)__popcll(mf);
     }
-    if (lane == 0u) tile_end[t] = w;
+    if (lane == 0u) {
+        tile_end[t] = wb;
+        fwd_end[t] = wf;
+    }
 }
 
-void launch_tile_cull(const DevScene &S, const uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t cap,
-                      bool cull, hipStream_t stream) {
+void launch_tile_cull(const DevScene &S, uint32_t *entries, uint32_t *out, uint32_t *tile_end, uint32_t *fwd_end,
+                      uint32_t cap, bool cull, hipStream_t stream) {
     uint32_t ntiles = S.tiles_x * S.tiles_y;
     if (ntiles == 0) return;
     if (cap > 0)
         hipLaunchKernelGGL(k_tile_keep, dim3((cap + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, S, entries, out,
                            cull ? 1u : 0u);
     hipLaunchKernelGGL(k_tile_compact, dim3((ntiles + BIN_TILES_PER_WG - 1) / BIN_TILES_PER_WG), dim3(BLOCK), 0,
-                       stream, S, entries, out, tile_end);
+                       stream, S, entries, out, tile_end, fwd_end);
 }
 
 __global__ void k_upd_curves(float *__restrict__ curve_data, const uint32_t *__restrict__ curve_pt,
@@ -4707,8 +4723,8 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
                            P.tile_cap, P.overflow);
         launch_bin_write(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, offsets,
                          entries, stream);
-        launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), P.tile_cap, !S.prefilter,
-                         stream);
+        launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), (uint32_t *)(S.fwd_end),
+                         P.tile_cap, !S.prefilter, stream);
     }
     if (S.n_cdf > 0 && !S.prefilter) {
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
