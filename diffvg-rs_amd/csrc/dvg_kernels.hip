@@ -55,10 +55,10 @@ namespace dvgk {
 #define OCC_PREFILTER 3
 #endif
 // The batch's boundary evaluation (a persistent loop over its work queue):
-// 7 waves measured better than 8 (four 1024x1024 scenes: 24.0 against
-// 24.6 ms a batched step).
+// four 1024x1024 scenes step in 17.0 ms at 8 waves, 18.7 ms at 7 (with the
+// backward's culled tile lists; 7 was the better before them).
 #ifndef OCC_BATCH_BOUNDARY
-#define OCC_BATCH_BOUNDARY 7
+#define OCC_BATCH_BOUNDARY 8
 #endif
 
 // ---------------------------------------------------------------------------
