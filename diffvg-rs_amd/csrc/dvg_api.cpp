@@ -694,7 +694,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
         dvgk::launch_bin_write((const float *)at(i_rng), G, tiles_x, tiles_y, (const uint32_t *)at(i_toff),
                                (uint32_t *)at(i_tent), ctx->stream);
         dvgk::launch_tile_cull(D, (uint32_t *)at(i_tent), (uint32_t *)at(i_tbwd), (uint32_t *)at(i_tend),
-                               (uint32_t *)at(i_fend), (uint32_t)tile_cap, !D.prefilter, ctx->stream);
+                               (uint32_t *)at(i_fend), (uint32_t)tile_cap, true, ctx->stream);
         e = hipGetLastError();
         if (e != hipSuccess) {
             (void)hipFree(ds->arena);

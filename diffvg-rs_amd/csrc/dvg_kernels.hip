@@ -3937,11 +3937,15 @@ __global__ void k_shape_topo(const float *__restrict__ shape_data, uint32_t npre
 // bit 4) and quadratic segments wrongly over whole bands of rows far from
 // their curve, and the images follow it there; on lines and well-conditioned
 // cubics it can go wrong only within ~1e-4 px of a segment end's y.  So the
-// forward's lists (the full lists compacted in place, to fwd_end[t]) drop an
-// entry only when every segment of the group is a line or a well-conditioned
-// cubic.  Kept in both: every group with a stroke paint or a shape other
-// than a closed-loop path (record slot 31 of the path's first segment), and
-// every entry of a prefiltered scene (distance coverage).
+// forward's lists (the full lists compacted in place, to fwd_end[t]) drop a
+// filled group only when every segment of it is a line or a
+// well-conditioned cubic.  A stroke adds nothing where its curve lies farther than the
+// segment's radius (record radii) from every point of the region: the
+// distance to any point of the curve is at least the least one, so no test
+// finds a hit there (prefiltered coverage saturates one pixel beyond: the
+// region of a prefiltered scene is widened by that band).  Kept in both:
+// every group with a shape other than a path, or with a fill and a path
+// that is not a closed loop (record slot 31 of its first segment).
 
 // Whether a Bezier segment (kind k: 0 line, 1 quadratic, 2 cubic) may meet
 // the box: false when the control box of each of its 32 equal-t pieces
@@ -3982,13 +3986,14 @@ DVG_DEV bool bezier_meets_box(uint32_t k, const float *xs, const float *ys, floa
 DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     const float *gd = S.group_data + g * 16u;
     uint32_t fk = (uint32_t)gd[2], sk = (uint32_t)gd[4], rule = (uint32_t)gd[7];
-    if (sk != DVG_PAINT_NONE) return 0u;
-    if (fk == DVG_PAINT_NONE) return 3u;  // no paint: every walk skips it
-    // the region: the tile padded as binning pads group boxes, plus a pixel;
-    // on the canvas border 4 px + 2e-4 of the canvas further out (points
-    // clamped into border tiles from beyond the canvas)
+    bool fill = fk != DVG_PAINT_NONE, stroke = sk != DVG_PAINT_NONE;
+    if (!fill && !stroke) return 3u;  // no paint: every walk skips it
+    // the region: the tile padded as binning pads group boxes, plus a pixel
+    // (and the prefiltered coverage band); on the canvas border 4 px + 2e-4
+    // of the canvas further out (points clamped into border tiles from
+    // beyond the canvas)
     uint32_t tx = t % S.tiles_x, ty = t / S.tiles_x;
-    float pad = tile_pad(S.W, S.H) + 1.0f;
+    float pad = tile_pad(S.W, S.H) + 1.0f + (S.prefilter ? 1.5f : 0.f);
     float far = pad + 4.0f + 2.0e-4f * (float)(S.W > S.H ? S.W : S.H);
     float rx0 = (float)(tx * TILE) - (tx == 0u ? far : pad);
     float rx1 = (float)((tx + 1u) * TILE) + (tx + 1u == S.tiles_x ? far : pad);
@@ -4000,14 +4005,14 @@ DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     group_local(S, g, 0.5f * (rx0 + rx1), 0.5f * (ry0 + ry1), lcx, lcy);
     uint32_t off = (uint32_t)gd[0], cnt = (uint32_t)gd[1];
     float wsum = 0.f, cr = 0.f;
-    bool fwd = true;  // every segment a line or a well-conditioned cubic
+    bool fwd = true;  // no fill winding the forward's f32 solve could get wrong far away
     for (uint32_t i = 0; i < cnt; ++i) {
         uint32_t si = (uint32_t)S.group_shapes[off + i];
         const float *sd = S.shape_data + si * 15u;
         if ((uint32_t)sd[0] != DVG_SHAPE_PATH) return 0u;
         uint32_t co = (uint32_t)sd[12], cc = (uint32_t)sd[13];
-        if (cc == 0u) continue;  // no segment: winding 0 everywhere
-        if (!(S.curve_wind[(size_t)co * WIND_STRIDE + 31] > 0.f)) return 0u;
+        if (cc == 0u) continue;  // no segment: no winding, no stroke
+        if (fill && !(S.curve_wind[(size_t)co * WIND_STRIDE + 31] > 0.f)) return 0u;
         // the region's box in shape space, widened for rounding
         const float *xf = S.shape_xform + si * 6u;
         float sx0 = __builtin_inff(), sy0 = __builtin_inff(), sx1 = -__builtin_inff(), sy1 = -__builtin_inff();
@@ -4024,10 +4029,19 @@ DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
             const float *cv = S.curve_data + (size_t)(co + s) * 13u;
             const float *wc = S.curve_wind + (size_t)(co + s) * WIND_STRIDE;
             uint32_t k = (uint32_t)cv[0];
-            if (k == 1u || (k == 2u && ((uint32_t)wc[11] & 16u))) fwd = false;
+            if (fill && (k == 1u || (k == 2u && ((uint32_t)wc[11] & 16u)))) fwd = false;
             float xs[4] = {cv[1], cv[3], cv[5], cv[7]}, ys[4] = {cv[2], cv[4], cv[6], cv[8]};
-            bool near = !(wc[15] > sx1 || wc[14] < sx0 || wc[12] > sy1 || wc[13] < sy0);
-            if (near && bezier_meets_box(k, xs, ys, sx0, sy0, sx1, sy1)) return 0u;
+            // a stroke reaches as far as the segment's largest radius
+            float rr = 0.f;
+            if (stroke) {
+                rr = fmaxf(fmaxf(fabsf(cv[9]), fabsf(cv[10])), fmaxf(fabsf(cv[11]), fabsf(cv[12])));
+                if (!(rr < 1.0e30f)) return 0u;
+                rr += 1.0e-4f * rr + 1.0e-3f;
+            }
+            float bx0 = sx0 - rr, by0 = sy0 - rr, bx1 = sx1 + rr, by1 = sy1 + rr;
+            bool near = !(wc[15] > bx1 || wc[14] < bx0 || wc[12] > by1 || wc[13] < by0);
+            if (near && bezier_meets_box(k, xs, ys, bx0, by0, bx1, by1)) return 0u;
+            if (!fill) continue;
             if (k == 0u) {
                 winding_line_cpu(scx, scy, xs[0], ys[0], xs[1], ys[1], rule, lw, lc);
             } else {
@@ -4045,7 +4059,7 @@ DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
             wsum += lw;
         }
     }
-    if ((rule == 1u) ? cr > 0.f : wsum != 0.f) return 0u;
+    if (fill && ((rule == 1u) ? cr > 0.f : wsum != 0.f)) return 0u;
     return fwd ? 3u : 1u;
 }
 
@@ -4724,7 +4738,7 @@ void launch_scene_update(const DevScene &S, const UpdatePlan &P, const float *pa
         launch_bin_write(reinterpret_cast<const float *>(P.ranges), S.num_groups, S.tiles_x, S.tiles_y, offsets,
                          entries, stream);
         launch_tile_cull(S, entries, (uint32_t *)(S.bwd_entries), (uint32_t *)(S.tile_end), (uint32_t *)(S.fwd_end),
-                         P.tile_cap, !S.prefilter, stream);
+                         P.tile_cap, true, stream);
     }
     if (S.n_cdf > 0 && !S.prefilter) {
         hipLaunchKernelGGL(k_upd_lengths, dim3(cdiv(S.num_shapes, 256)), dim3(256), 0, stream, S,
