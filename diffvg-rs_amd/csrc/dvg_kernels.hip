@@ -417,10 +417,9 @@ DVG_DEV void shape_fill(const DevScene &S, const float *sd, uint32_t rule, bool 
         float bx1 = wc[14], by0 = wc[12], by1 = wc[13];
         if (prefilter) {
             // forward.rs:889-937: segments farther than the running minimum
-            // cannot lower it (control-point box)
-            float cx0 = fminf(fminf(x0, x1), fminf(x2, x3)), cx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
-            float cy0 = fminf(fminf(y0, y1), fminf(y2, y3)), cy1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
-            float ddx = maxf_(maxf_(cx0 - px, 0.f), px - cx1), ddy = maxf_(maxf_(cy0 - py, 0.f), py - cy1);
+            // cannot lower it (the curve's box, widened past the rounding of
+            // the distances computed to it)
+            float ddx = maxf_(maxf_(wc[15] - px, 0.f), px - bx1), ddy = maxf_(maxf_(by0 - py, 0.f), py - by1);
             if (sqrtf(ddx * ddx + ddy * ddy) <= lmin) {
                 float d, t;
                 if (k == 0) d = seg_dist(px, py, x0, y0, x1, y1);
@@ -541,14 +540,16 @@ DVG_DEV void shape_stroke(const DevScene &S, const float *sd, bool prefilter, fl
         float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
         float r0 = cv[9], r1 = cv[10], r2 = cv[11], r3 = cv[12];
         float pad = fmaxf(fmaxf(absf_(r0), absf_(r1)), fmaxf(absf_(r2), absf_(r3)));
-        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)) - pad, bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)) + pad;
-        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)) - pad, by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3)) + pad;
+        // the curve's box (record slots 12..15, widened past the rounding of
+        // any distance computed to it), padded by the largest radius
+        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;
+        float bx0 = wc[15] - pad, bx1 = wc[14] + pad, by0 = wc[12] - pad, by1 = wc[13] + pad;
         if (prefilter) {
             float ddx = maxf_(maxf_(bx0 + pad - px, 0.f), px - (bx1 - pad));
             float ddy = maxf_(maxf_(by0 + pad - py, 0.f), py - (by1 - pad));
             if (sqrtf(ddx * ddx + ddy * ddy) > lmin) continue;
         } else if (px < bx0 || px > bx1 || py < by0 || py > by1) {
-            continue;  // forward.rs:1048-1050: outside the radius-padded bounds
+            continue;  // forward.rs:1048-1050: outside the radius-padded bounds (the curve's)
         }
         float d, t;
         if (k == 0) seg_dist_t(px, py, x0, y0, x1, y1, d, t);
@@ -1659,8 +1660,8 @@ DVG_DEV bool closest_shape(const DevScene &S, uint32_t prep, float px, float py,
         const float *cv = S.curve_data + (off + s) * 13u;
         uint32_t k = (uint32_t)cv[0];
         float x0 = cv[1], y0 = cv[2], x1 = cv[3], y1 = cv[4], x2 = cv[5], y2 = cv[6], x3 = cv[7], y3 = cv[8];
-        float bx0 = fminf(fminf(x0, x1), fminf(x2, x3)), bx1 = fmaxf(fmaxf(x0, x1), fmaxf(x2, x3));
-        float by0 = fminf(fminf(y0, y1), fminf(y2, y3)), by1 = fmaxf(fmaxf(y0, y1), fmaxf(y2, y3));
+        const float *wc = S.curve_wind + (size_t)(off + s) * WIND_STRIDE;  // the curve's box
+        float bx0 = wc[15], bx1 = wc[14], by0 = wc[12], by1 = wc[13];
         float lim = any ? best : cut;
         if (box_dist(bx0, by0, bx1, by1, px, py) > lim) continue;
         float d, t, qx, qy;
