@@ -2580,8 +2580,9 @@ DVG_DEV void edge_blend(const DevScene &S, uint32_t g, bool is_stroke, bool eq, 
     ++nfr;
 }
 
-DVG_DEV float4 edge_finish(float4 acc, uint32_t nfr, float4 bg) {
-    if (nfr == 0) return bg;
+// (with no fragment the composite is still the background it started from)
+DVG_DEV float4 edge_finish(float4 acc, uint32_t nfr) {
+    if (nfr == 0) return acc;
     float fa = acc.w;
     if (fa > 1.0e-6f) {
         float sc = 1.0f / fa;
@@ -2625,8 +2626,8 @@ DVG_DEV void edge_walk_pair(const DevScene &S, uint32_t tile, float px0, float p
     }
     hit0 = et0 ? h01 : h00;
     hit1 = et1 ? h11 : h10;
-    col0 = edge_finish(acc0, nfr0, bg);
-    col1 = edge_finish(acc1, nfr1, bg);
+    col0 = edge_finish(acc0, nfr0);
+    col1 = edge_finish(acc1, nfr1);
 }
 
 DVG_DEV uint32_t tile_of(const DevScene &S, float px, float py) {
