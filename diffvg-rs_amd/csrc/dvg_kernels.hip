@@ -2915,7 +2915,6 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
         size_t idx = LIST ? fs.slots[slot_base + pos] : order[pos];
         valid = boundary_point(S, nullptr, idx, P);
         if (valid) {
-            pix = (size_t)P.pyi * S.W + P.pxi;
             float4 bgs = background_at(S, P.pxi, P.pyi);
             float4 bg = make_float4(bgs.x * bgs.w, bgs.y * bgs.w, bgs.z * bgs.w, bgs.w);
             bool ih, oh;
@@ -2931,11 +2930,13 @@ DVG_DEV void bnd_eval_unit(const DevScene &S, uint32_t u, const float *__restric
             edge_colors<M>(S, pxi, pyi, qix, qiy, qox, qoy, bg, gi, prep, cin, cout, ih, oh, uns);
             asm volatile("" ::: "memory");
             P = s_pt[threadIdx.x];
+            pix = (size_t)P.pyi * S.W + P.pxi;
             valid = ih || oh;
             if (M == COV_CHK && !LIST && uns) {
                 uint32_t slot = atomicAdd(&s_nflag, 1u);
                 if (slot < FLAG_SLOTS) {
-                    fs.slots[(size_t)u * FLAG_SLOTS + slot] = (uint32_t)idx;
+                    // (the sample index read again: not held through the walk)
+                    fs.slots[(size_t)u * FLAG_SLOTS + slot] = order[pos];
                     valid = false;
                 }
             }
