@@ -69,6 +69,57 @@ def _edge_cases_scene():
     return sc
 
 
+def _culling_scene():
+    """Groups the tile lists' culling must keep or drop correctly over 6 x 6
+    tiles: closed cubic paths under group transforms that shrink and rotate
+    (the tile regions mapped into shape space), stroked open paths with
+    per-point thickness under a scaling group transform, an open filled path
+    (kept: its winding is not the curve's winding number), a group of two
+    closed paths with the even-odd rule (a hole), and a closed quadratic
+    path."""
+    from pydvg.scene import Path, Scene, Shape, ShapeGroup, Solid, Filter
+    from pydvg import abi
+    sc = Scene(96, 96, background=(1.0, 1.0, 1.0, 1.0), filter=Filter(abi.FILTER_BOX, 0.5))
+    rng = np.random.default_rng(31)
+
+    def closed_cubic(center, spread, n):
+        pts = np.asarray(center) + rng.uniform(-spread, spread, size=(3 * n, 2))
+        return Path(np.full(n, 2, np.uint8), pts.astype(np.float32), is_closed=True)
+
+    def group(shapes, fill=None, stroke=None, xform=None, rule=abi.FILL_NONZERO):
+        idx = []
+        for sh in shapes:
+            sc.shapes.append(sh)
+            idx.append(len(sc.shapes) - 1)
+        g = ShapeGroup(idx, fill=fill, stroke=stroke, fill_rule=rule)
+        if xform is not None:
+            g.set_shape_to_canvas(np.asarray(xform, np.float32))
+        sc.groups.append(g)
+
+    for k in range(4):
+        a, sc_ = 0.4 + 0.7 * k, 0.45 + 0.1 * k
+        c, s_ = np.cos(a) * sc_, np.sin(a) * sc_
+        cx, cy = rng.uniform(15.0, 80.0, size=2)
+        group([Shape(closed_cubic((0.0, 0.0), 40.0, 4))], fill=Solid((0.9, 0.3 * k, 0.2, 0.7)),
+              xform=[[c, -s_, cx], [s_, c, cy], [0.0, 0.0, 1.0]])
+    for k in range(3):
+        start = rng.uniform(10.0, 60.0, size=2)
+        pts = np.vstack([start, start + np.cumsum(rng.normal(0.0, 9.0, size=(6, 2)), axis=0)])
+        th = rng.uniform(0.3, 1.5, size=7).astype(np.float32)
+        sh = Shape(Path(np.full(2, 2, np.uint8), pts.astype(np.float32), thickness=th, is_closed=False),
+                   stroke_width=1.0)
+        group([sh], stroke=Solid((0.1, 0.2, 0.8, 0.8)), xform=[[1.3, 0.0, 5.0], [0.0, 1.3, 4.0], [0.0, 0.0, 1.0]])
+    pts = np.array([60.0, 60.0]) + rng.uniform(-20.0, 20.0, size=(7, 2))
+    group([Shape(Path(np.full(2, 2, np.uint8), pts.astype(np.float32), is_closed=False))],
+          fill=Solid((0.2, 0.8, 0.4, 0.6)))
+    group([Shape(closed_cubic((30.0, 65.0), 18.0, 3)), Shape(closed_cubic((30.0, 65.0), 7.0, 3))],
+          fill=Solid((0.7, 0.2, 0.7, 0.8)), rule=abi.FILL_EVENODD)
+    quad = np.array([70.0, 25.0]) + rng.uniform(-15.0, 15.0, size=(8, 2))
+    group([Shape(Path(np.full(4, 1, np.uint8), quad.astype(np.float32), is_closed=True))],
+          fill=Solid((0.3, 0.6, 0.9, 0.9)))
+    return sc
+
+
 def _cases():
     from pydvg import abi
     return {
@@ -90,6 +141,8 @@ def _cases():
         "parabolic_cubic40": (_with_filter(scenes.cubic_paths_scene(40, 40, 8, seed=12, spread=12.0),
                                            abi.FILTER_RADIAL_PARABOLIC, 1.0), scenes.config1()[1]),
         "edge_cases40": (_edge_cases_scene(), scenes.config1()[1]),
+        "culling96": (_culling_scene(), scenes.config2()[1]),
+        "culling96_prefiltered": (_culling_scene(), _prefiltered_opts()),
     }
 
 
