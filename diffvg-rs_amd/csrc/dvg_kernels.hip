@@ -4066,10 +4066,11 @@ DVG_DEV uint32_t tile_entry_drops(const DevScene &S, uint32_t t, uint32_t g) {
     return fwd ? 3u : 1u;
 }
 
-// Pass 1, one lane per entry of the full lists: 1 in out[e] when entry e is
-// kept.  Pass 2, one wave per tile, 64 entries at a time: a kept entry goes
-// to the count of kept entries before it (the lists keep their group order;
-// a write never passes a flag not yet read).
+// Pass 1, one lane per entry of the full lists: out[e] = the drop bits of
+// entry e (tile_entry_drops).  Pass 2, one wave per tile, 64 entries at a
+// time: an entry kept in a list goes to the count of entries kept in it
+// before it (the lists keep their group order; a write never passes a flag
+// or an entry not yet read).
 __global__ __launch_bounds__(BLOCK) void k_tile_keep(DevScene S, const uint32_t *__restrict__ entries,
                                                      uint32_t *__restrict__ out, uint32_t cull) {
     uint32_t ntiles = S.tiles_x * S.tiles_y;
