@@ -585,7 +585,7 @@ int32_t dvg_device_scene_create(dvg_ctx *ctx, const dvg_scene_pack *pk, const dv
     D.prefilter = prm->use_prefiltering;
     D.filter_type = prm->filter_type;
     D.sblock_log2 = 2;  // 4x4 pixel sample blocks
-    D.bnd_log2 = 2;     // 4x4 pixel boundary buckets
+    D.bnd_log2 = 1;     // 2x2 pixel boundary buckets (1024x1024 step 4.63 -> 4.59 ms against 4x4; 256x256 / 64 paths 6.14 -> 6.01)
     // tile work in units of one sample per lane: many more workgroups than
     // compute units even for small canvases, and no tile on the critical path
     D.unit_samples = std::min((uint32_t)dvgk::BLOCK, (uint32_t)dvgk::TILE * dvgk::TILE * D.spp);

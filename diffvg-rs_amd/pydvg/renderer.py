@@ -206,7 +206,7 @@ class DeviceScene:
     def check(self):
         _check(self.r.lib, self.r.ctx, self.r.lib.dvg_device_scene_check(self.r.ctx, self.handle))
 
-    def set_shapes(self, sblock_log2=2, bnd_log2=2, unit_log2=0):
+    def set_shapes(self, sblock_log2=2, bnd_log2=1, unit_log2=0):
         """Launch shapes (see dvg_device_scene_set_shapes)."""
         st = self.r.lib.dvg_device_scene_set_shapes(self.handle, sblock_log2, bnd_log2, unit_log2)
         _check(self.r.lib, self.r.ctx, st)

@@ -481,7 +481,7 @@ int32_t dvg_device_scene_check(dvg_ctx *ctx, dvg_device_scene *scene);
 /* Launch shapes of one device scene, as in dvg_batch_config (sample blocks
  * 1 << sblock_log2 pixels, boundary buckets 1 << bnd_log2 pixels, tile work
  * in units of 256 << unit_log2 samples); results do not depend on them
- * beyond the order of float additions.  Defaults: 2, 2, 0. */
+ * beyond the order of float additions.  Defaults: 2, 1, 0. */
 int32_t dvg_device_scene_set_shapes(dvg_device_scene *scene, uint32_t sblock_log2, uint32_t bnd_log2,
                                     uint32_t unit_log2);
 
